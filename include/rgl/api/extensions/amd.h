@@ -1,0 +1,33 @@
+// RGL-MI355X extensions: entry points with no counterpart in the reference API.
+//
+// They expose what the MI355X implementation adds on top of RGL's C interface: control of the seeded noise
+// stream (the reference seeds noise from std::random_device, src/graph/NodesCore.hpp:536,563,590, so it cannot
+// be replayed), the TLAS update policy for moving entities, and introspection used by tests and benchmarks.
+#pragma once
+
+#include <rgl/api/core.h>
+
+// Sets the seed of a Gaussian noise node's stream and restarts its run counter. The noise drawn for element i in
+// run r is a pure function of (seed, r, node kind, i), so the same seed replays the same noise on CPU and GPU.
+RGL_API rgl_status_t rgl_node_gaussian_noise_set_seed(rgl_node_t node, uint64_t seed);
+
+// Reports the stream identity (seed, next run index, node-kind tag) of a Gaussian noise node.
+RGL_API rgl_status_t rgl_node_gaussian_noise_get_stream(rgl_node_t node, uint64_t* out_seed, uint32_t* out_run,
+                                                        uint32_t* out_tag);
+
+// Chooses how the scene's TLAS follows entity transform changes when no entity was added or removed:
+// refit = true keeps the topology and refits boxes; false rebuilds the TLAS (default).
+RGL_API rgl_status_t rgl_scene_set_tlas_refit(rgl_scene_t scene, bool refit);
+
+// Per-ray traversal counters of a raytrace node's last configuration: internal BVH nodes visited, triangles tested
+// and instances entered. Arrays hold `ray_count` entries (the node's ray count).
+RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t* nodes_visited, int32_t* tris_tested,
+                                                       int32_t* instances_entered, int32_t ray_count);
+
+// Closest hit of every primary ray by testing all triangles of all instances (no BVH). Misses give t = -1,
+// instance = -1, triangle = -1. Intended for verifying the BVH traversal.
+RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t, int32_t* out_instance,
+                                                   int32_t* out_prim, int32_t ray_count);
+
+// Standard normal variates normal(seed, run, tag, i) for i in [0, count), computed on the GPU.
+RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);

@@ -1,0 +1,1053 @@
+// extern "C" entry points of the RGL API (include/rgl/api/core.h), implemented over the HIP runtime.
+//
+// Conventions follow the reference's API layer (src/api/apiCommon.hpp:39-139, src/api/apiCommon.cpp:44-90,
+// src/api/apiCore.cpp): every call runs inside safeCall(), which performs lazy initialization, refuses work after
+// an unrecoverable status, and maps exceptions to rgl_status_t; argument checks produce
+// "RGL API Error: Invalid argument, condition unsatisfied: <expr>"; nodes are created or updated in place.
+
+#include <rgl/api/core.h>
+#include <rgl/api/extensions/amd.h>
+
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <set>
+#include <string>
+
+#include "graph.hpp"
+#include "scene.hpp"
+
+using namespace rgl;
+
+namespace {
+
+rgl_status_t lastStatus = RGL_SUCCESS;
+std::optional<std::string> lastMessage;
+int logLevel = RGL_LOG_LEVEL_INFO;
+bool logStdout = true;
+std::unique_ptr<std::ofstream> logFile;
+
+struct NotImplementedCall : std::runtime_error
+{
+	using std::runtime_error::runtime_error;
+};
+
+const char* fallbackMessage(rgl_status_t s)
+{
+	switch (s) {
+		case RGL_SUCCESS: return "operation successful";
+		case RGL_INVALID_ARGUMENT: return "invalid argument";
+		case RGL_INTERNAL_EXCEPTION: return "unhandled internal exception";
+		case RGL_INVALID_STATE: return "invalid state - unrecoverable error occurred";
+		case RGL_NOT_IMPLEMENTED: return "operation not (yet) implemented";
+		case RGL_LOGGING_ERROR: return "logging error";
+		case RGL_INVALID_FILE_PATH: return "invalid file path";
+		case RGL_TAPE_ERROR: return "tape error";
+		case RGL_UDP_ERROR: return "UDP error";
+		case RGL_ROS2_ERROR: return "ROS2 error";
+		default: return "???";
+	}
+}
+
+bool recoverable(rgl_status_t s)
+{
+	return s == RGL_SUCCESS || s == RGL_INVALID_ARGUMENT || s == RGL_INVALID_API_OBJECT || s == RGL_INVALID_PIPELINE ||
+	       s == RGL_INVALID_FILE_PATH || s == RGL_NOT_IMPLEMENTED || s == RGL_TAPE_ERROR || s == RGL_UDP_ERROR ||
+	       s == RGL_ROS2_ERROR;
+}
+
+void log(int level, const std::string& msg)
+{
+	if (level < logLevel || logLevel >= RGL_LOG_LEVEL_OFF) {
+		return;
+	}
+	static const char* names[] = {"trace", "debug", "info", "warn", "error", "critical"};
+	const std::string line = std::string("[RGL ") + names[std::min(level, 5)] + "] " + msg + "\n";
+	if (logStdout) {
+		std::fputs(line.c_str(), stdout);
+		std::fflush(stdout);
+	}
+	if (logFile) {
+		*logFile << line;
+		logFile->flush();
+	}
+}
+
+rgl_status_t updateState(rgl_status_t s, std::optional<std::string> msg = std::nullopt)
+{
+	lastStatus = s;
+	lastMessage = std::move(msg);
+	if (s != RGL_SUCCESS) {
+		const std::string text = lastMessage.value_or(fallbackMessage(s));
+		log(recoverable(s) ? RGL_LOG_LEVEL_ERROR : RGL_LOG_LEVEL_CRITICAL,
+		    (recoverable(s) ? "Recoverable error (code=" : "Unrecoverable error (code=") + std::to_string(s) + "): " + text);
+	}
+	return s;
+}
+
+template<typename Fn>
+rgl_status_t safeCall(Fn&& fn);
+
+void lazyInit()
+{
+	static bool called = false;
+	if (called) {
+		return;
+	}
+	called = true;
+	rgl_status_t st = safeCall([]() {
+		int count = 0;
+		RGL_HIP(hipGetDeviceCount(&count));
+		if (count <= 0) {
+			throw std::runtime_error("no HIP device available");
+		}
+		RGL_HIP(hipSetDevice(0));
+		hipDeviceProp_t prop{};
+		RGL_HIP(hipGetDeviceProperties(&prop, 0));
+		log(RGL_LOG_LEVEL_INFO, std::string("RGL-MI355X on ") + prop.name + " (" + prop.gcnArchName + ")");
+	});
+	if (st != RGL_SUCCESS) {
+		updateState(RGL_INITIALIZATION_ERROR, lastMessage);
+	}
+}
+
+template<typename Fn>
+rgl_status_t safeCall(Fn&& fn)
+{
+	lazyInit();
+	if (!recoverable(lastStatus)) {
+		return updateState(RGL_INVALID_STATE);
+	}
+	try {
+		fn();
+	} catch (NotImplementedCall& e) {
+		return updateState(RGL_NOT_IMPLEMENTED, e.what());
+	} catch (InvalidAPIObject& e) {
+		return updateState(RGL_INVALID_API_OBJECT, e.what());
+	} catch (InvalidPipeline& e) {
+		return updateState(RGL_INVALID_PIPELINE, e.what());
+	} catch (std::invalid_argument& e) {
+		return updateState(RGL_INVALID_ARGUMENT, e.what());
+	} catch (std::exception& e) {
+		return updateState(RGL_INTERNAL_EXCEPTION, e.what());
+	} catch (...) {
+		return updateState(RGL_INTERNAL_EXCEPTION, "exceptional exception");
+	}
+	return updateState(RGL_SUCCESS);
+}
+
+#define CHECK_ARG(expr)                                                                                                \
+	do {                                                                                                               \
+		if (!(expr)) {                                                                                                 \
+			throw std::invalid_argument(std::string("RGL API Error: Invalid argument, condition unsatisfied: ") + #expr); \
+		}                                                                                                              \
+	} while (0)
+
+rgl_node_t handleOf(const rgl::Node::Ptr& n) { return reinterpret_cast<rgl_node_t>(n.get()); }
+
+template<typename T>
+std::shared_ptr<T> nodeAs(rgl_node_t h, const char* expected)
+{
+	auto base = lookupNode(h);
+	auto typed = std::dynamic_pointer_cast<T>(base);
+	if (!typed) {
+		throw InvalidAPIObject(std::string("RGL API Error: Node type mismatch: expected ") + expected + ", got " +
+		                       base->getName());
+	}
+	return typed;
+}
+
+// src/api/apiCommon.hpp:108-139
+template<typename T, typename Fn>
+void createOrUpdateNode(rgl_node_t* handle, const char* typeName, bool graphValidationNeeded, Fn&& setParams)
+{
+	std::shared_ptr<T> node;
+	if (*handle == nullptr) {
+		node = std::make_shared<T>();
+	} else {
+		node = nodeAs<T>(*handle, typeName);
+	}
+	if (node->hasCtx()) {
+		node->getCtx()->synchronize();
+	}
+	if (graphValidationNeeded && node->hasCtx()) {
+		node->getCtx()->markNodesDirty();
+	}
+	setParams(*node);
+	node->markDirty();
+	if (*handle == nullptr) {
+		registerNode(node);
+	}
+	*handle = handleOf(node);
+}
+
+void syncNodeGraph(const rgl::Node::Ptr& n)
+{
+	if (n->hasCtx()) {
+		n->getCtx()->synchronize();
+	}
+}
+
+void checkOverflow(const rgl::Node::Ptr& n)
+{
+	if (!n->hasCtx()) {
+		return;
+	}
+	for (auto& m : n->getCtx()->getNodes()) {
+		if (auto rt = std::dynamic_pointer_cast<RaytraceNode>(m)) {
+			if (rt->overflowed()) {
+				throw std::runtime_error("BVH traversal stack overflow: scene hierarchy is deeper than supported");
+			}
+		}
+	}
+}
+
+Mat3x4f toMat(const rgl_mat3x4f* m)
+{
+	Mat3x4f r;
+	std::memcpy(r.rc, m->value, sizeof(r.rc));
+	return r;
+}
+
+} // namespace
+
+namespace rgl {
+std::string fieldName(rgl_field_t f)
+{
+	switch (f) {
+		case RGL_FIELD_XYZ_VEC3_F32: return "XYZ_VEC3_F32";
+		case RGL_FIELD_INTENSITY_F32: return "INTENSITY_F32";
+		case RGL_FIELD_INTENSITY_U8: return "INTENSITY_U8";
+		case RGL_FIELD_IS_HIT_I32: return "IS_HIT_I32";
+		case RGL_FIELD_IS_GROUND_I32: return "IS_GROUND_I32";
+		case RGL_FIELD_RAY_IDX_U32: return "RAY_IDX_U32";
+		case RGL_FIELD_ENTITY_ID_I32: return "ENTITY_ID_I32";
+		case RGL_FIELD_DISTANCE_F32: return "DISTANCE_F32";
+		case RGL_FIELD_AZIMUTH_F32: return "AZIMUTH_F32";
+		case RGL_FIELD_ELEVATION_F32: return "ELEVATION_F32";
+		case RGL_FIELD_RING_ID_U16: return "RING_ID_U16";
+		case RGL_FIELD_RETURN_TYPE_U8: return "RETURN_TYPE_U8";
+		case RGL_FIELD_TIME_STAMP_F64: return "TIME_STAMP_F64";
+		case RGL_FIELD_TIME_STAMP_U32: return "TIME_STAMP_U32";
+		case RGL_FIELD_ABSOLUTE_VELOCITY_VEC3_F32: return "ABSOLUTE_VELOCITY_VEC3_F32";
+		case RGL_FIELD_RELATIVE_VELOCITY_VEC3_F32: return "RELATIVE_VELOCITY_VEC3_F32";
+		case RGL_FIELD_RADIAL_SPEED_F32: return "RADIAL_SPEED_F32";
+		case RGL_FIELD_POWER_F32: return "POWER_F32";
+		case RGL_FIELD_RCS_F32: return "RCS_F32";
+		case RGL_FIELD_NOISE_F32: return "NOISE_F32";
+		case RGL_FIELD_SNR_F32: return "SNR_F32";
+		case RGL_FIELD_NORMAL_VEC3_F32: return "NORMAL_VEC3_F32";
+		case RGL_FIELD_INCIDENT_ANGLE_F32: return "INCIDENT_ANGLE_F32";
+		case RGL_FIELD_RAY_POSE_MAT3x4_F32: return "RAY_POSE_MAT3x4_F32";
+		case RGL_FIELD_LASER_RETRO_F32: return "LASER_RETRO_F32";
+		case RGL_FIELD_PADDING_8: return "PADDING_8";
+		case RGL_FIELD_PADDING_16: return "PADDING_16";
+		case RGL_FIELD_PADDING_32: return "PADDING_32";
+		case RGL_FIELD_DYNAMIC_FORMAT: return "DYNAMIC_FORMAT";
+		default: return "UNKNOWN(" + std::to_string(static_cast<int>(f)) + ")";
+	}
+}
+
+void throwHip(hipError_t e, const char* expr, const char* file, int line)
+{
+	throw HipError(std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") at " + file + ":" +
+	               std::to_string(line) + ": " + expr);
+}
+} // namespace rgl
+
+// ============================================================================================ general
+RGL_API rgl_status_t rgl_get_version_info(int32_t* out_major, int32_t* out_minor, int32_t* out_patch)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_major != nullptr);
+		CHECK_ARG(out_minor != nullptr);
+		CHECK_ARG(out_patch != nullptr);
+		*out_major = RGL_VERSION_MAJOR;
+		*out_minor = RGL_VERSION_MINOR;
+		*out_patch = RGL_VERSION_PATCH;
+	});
+}
+
+RGL_API rgl_status_t rgl_get_extension_info(rgl_extension_t extension, int32_t* out_available)
+{
+	return safeCall([&]() {
+		CHECK_ARG(0 <= extension && extension < RGL_EXTENSION_COUNT);
+		CHECK_ARG(out_available != nullptr);
+		*out_available = 0;
+	});
+}
+
+RGL_API rgl_status_t rgl_configure_logging(rgl_log_level_t log_level, const char* log_file_path, bool use_stdout)
+{
+	return safeCall([&]() {
+		CHECK_ARG(0 <= log_level && log_level <= 6);
+		logLevel = log_level;
+		logStdout = use_stdout;
+		logFile.reset();
+		if (log_file_path != nullptr) {
+			logFile = std::make_unique<std::ofstream>(log_file_path, std::ios::trunc);
+			if (!logFile->good()) {
+				logFile.reset();
+				throw std::runtime_error(std::string("cannot open log file ") + log_file_path);
+			}
+		}
+	});
+}
+
+RGL_API void rgl_get_last_error_string(const char** out_error_string)
+{
+	if (out_error_string == nullptr) {
+		return;
+	}
+	*out_error_string = lastMessage.has_value() ? lastMessage->c_str() : fallbackMessage(lastStatus);
+}
+
+RGL_API rgl_status_t rgl_cleanup(void)
+{
+	return safeCall([&]() {
+		for (auto& n : allNodes()) {
+			if (n->hasCtx()) {
+				n->getCtx()->detachAndDestroy();
+			}
+		}
+		for (auto& n : allNodes()) {
+			n->disconnectConnectedNodes();
+			releaseNode(n.get());
+		}
+		Registry<rgl::Entity>::map().clear();
+		Registry<rgl::Mesh>::map().clear();
+		Registry<rgl::Texture>::map().clear();
+		rgl::Scene::instance().clear();
+	});
+}
+
+// ============================================================================================ mesh
+RGL_API rgl_status_t rgl_mesh_create(rgl_mesh_t* out_mesh, const rgl_vec3f* vertices, int32_t vertex_count,
+                                     const rgl_vec3i* indices, int32_t index_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_mesh != nullptr);
+		CHECK_ARG(vertices != nullptr);
+		CHECK_ARG(vertex_count > 0);
+		CHECK_ARG(indices != nullptr);
+		CHECK_ARG(index_count > 0);
+		GraphRunCtx::synchronizeAll();
+		auto m = std::make_shared<rgl::Mesh>(reinterpret_cast<const Vec3f*>(vertices), vertex_count,
+		                                reinterpret_cast<const Vec3i*>(indices), index_count);
+		Registry<rgl::Mesh>::add(m);
+		*out_mesh = reinterpret_cast<rgl_mesh_t>(m.get());
+	});
+}
+
+RGL_API rgl_status_t rgl_mesh_set_texture_coords(rgl_mesh_t mesh, const rgl_vec2f* uvs, int32_t uv_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(mesh != nullptr);
+		CHECK_ARG(uvs != nullptr);
+		CHECK_ARG(uv_count > 0);
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Mesh>::get(mesh, "Mesh")->setTexCoords(reinterpret_cast<const Vec2f*>(uvs), uv_count);
+		rgl::Scene::instance().markDirty(false);
+	});
+}
+
+RGL_API rgl_status_t rgl_mesh_set_bone_weights(rgl_mesh_t mesh, const rgl_bone_weights_t* bone_weights,
+                                               int32_t bone_weights_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(mesh != nullptr);
+		CHECK_ARG(bone_weights != nullptr);
+		CHECK_ARG(bone_weights_count > 0);
+		throw NotImplementedCall("skeletal animation is not part of this library");
+	});
+}
+
+RGL_API rgl_status_t rgl_mesh_set_restposes(rgl_mesh_t mesh, const rgl_mat3x4f* restposes, int32_t bones_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(mesh != nullptr);
+		CHECK_ARG(restposes != nullptr);
+		CHECK_ARG(bones_count > 0);
+		throw NotImplementedCall("skeletal animation is not part of this library");
+	});
+}
+
+RGL_API rgl_status_t rgl_mesh_destroy(rgl_mesh_t mesh)
+{
+	return safeCall([&]() {
+		CHECK_ARG(mesh != nullptr);
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Mesh>::release(mesh, "Mesh");
+	});
+}
+
+RGL_API rgl_status_t rgl_mesh_is_alive(rgl_mesh_t mesh, bool* out_alive)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_alive != nullptr);
+		*out_alive = Registry<rgl::Mesh>::contains(mesh);
+	});
+}
+
+// ============================================================================================ entity
+RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t scene, rgl_mesh_t mesh)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_entity != nullptr);
+		CHECK_ARG(mesh != nullptr);
+		CHECK_ARG(scene == nullptr);
+		GraphRunCtx::synchronizeAll();
+		auto e = std::make_shared<rgl::Entity>(Registry<rgl::Mesh>::get(mesh, "Mesh"));
+		Registry<rgl::Entity>::add(e);
+		rgl::Scene::instance().addEntity(e);
+		*out_entity = reinterpret_cast<rgl_entity_t>(e.get());
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_destroy(rgl_entity_t entity)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		GraphRunCtx::synchronizeAll();
+		auto e = Registry<rgl::Entity>::get(entity, "Entity");
+		rgl::Scene::instance().removeEntity(e);
+		Registry<rgl::Entity>::release(entity, "Entity");
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_set_transform(rgl_entity_t entity, const rgl_mat3x4f* transform)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		CHECK_ARG(transform != nullptr);
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Entity>::get(entity, "Entity")->setTransform(toMat(transform));
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_set_pose_world(rgl_entity_t entity, const rgl_mat3x4f* pose, int32_t bones_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		CHECK_ARG(pose != nullptr);
+		CHECK_ARG(bones_count > 0);
+		throw NotImplementedCall("skeletal animation is not part of this library");
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		CHECK_ARG(id != RGL_ENTITY_INVALID_ID);
+		GraphRunCtx::synchronizeAll();
+		auto e = Registry<rgl::Entity>::get(entity, "Entity");
+		constexpr int maxId = 1 << 28; // entity ids are 28-bit (src/scene/Entity.cpp:45-48)
+		if (id >= maxId) {
+			throw std::invalid_argument("Entity ID (" + std::to_string(id) + ") must be less than " + std::to_string(maxId));
+		}
+		e->id = id;
+		rgl::Scene::instance().markDirty(false);
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_texture_t texture)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		CHECK_ARG(texture != nullptr);
+		GraphRunCtx::synchronizeAll();
+		auto e = Registry<rgl::Entity>::get(entity, "Entity");
+		e->texture = Registry<rgl::Texture>::get(texture, "Texture");
+		rgl::Scene::instance().markDirty(false);
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_set_laser_retro(rgl_entity_t entity, float retro)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Entity>::get(entity, "Entity")->laserRetro = retro;
+		rgl::Scene::instance().markDirty(false);
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_apply_external_animation(rgl_entity_t entity, const rgl_vec3f* vertices,
+                                                         int32_t vertex_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(entity != nullptr);
+		CHECK_ARG(vertices != nullptr);
+		CHECK_ARG(vertex_count > 0);
+		throw NotImplementedCall("external vertex animation is not part of this library yet");
+	});
+}
+
+RGL_API rgl_status_t rgl_entity_is_alive(rgl_entity_t entity, bool* out_alive)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_alive != nullptr);
+		*out_alive = Registry<rgl::Entity>::contains(entity);
+	});
+}
+
+// ============================================================================================ texture
+RGL_API rgl_status_t rgl_texture_create(rgl_texture_t* out_texture, const void* texels, int32_t width, int32_t height)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_texture != nullptr);
+		CHECK_ARG(texels != nullptr);
+		CHECK_ARG(width > 0);
+		CHECK_ARG(height > 0);
+		GraphRunCtx::synchronizeAll();
+		auto t = std::make_shared<rgl::Texture>(texels, width, height);
+		Registry<rgl::Texture>::add(t);
+		*out_texture = reinterpret_cast<rgl_texture_t>(t.get());
+	});
+}
+
+RGL_API rgl_status_t rgl_texture_destroy(rgl_texture_t texture)
+{
+	return safeCall([&]() {
+		CHECK_ARG(texture != nullptr);
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Texture>::release(texture, "Texture");
+	});
+}
+
+RGL_API rgl_status_t rgl_texture_is_alive(rgl_texture_t texture, bool* out_alive)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_alive != nullptr);
+		*out_alive = Registry<rgl::Texture>::contains(texture);
+	});
+}
+
+// ============================================================================================ scene
+RGL_API rgl_status_t rgl_scene_set_time(rgl_scene_t scene, uint64_t nanoseconds)
+{
+	return safeCall([&]() {
+		CHECK_ARG(scene == nullptr);
+		GraphRunCtx::synchronizeAll();
+		rgl::Scene::instance().setTime(nanoseconds);
+		rgl::Scene::instance().markDirty(false);
+	});
+}
+
+// ============================================================================================ rays nodes
+RGL_API rgl_status_t rgl_node_rays_from_mat3x4f(rgl_node_t* node, const rgl_mat3x4f* rays, int32_t ray_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(rays != nullptr);
+		CHECK_ARG(ray_count > 0);
+		createOrUpdateNode<FromMat3x4fRaysNode>(node, "FromMat3x4fRaysNode", true, [&](FromMat3x4fRaysNode& n) {
+			n.setParameters(reinterpret_cast<const Mat3x4f*>(rays), ray_count);
+		});
+	});
+}
+
+RGL_API rgl_status_t rgl_node_rays_set_ring_ids(rgl_node_t* node, const int32_t* ring_ids, int32_t ring_ids_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(ring_ids != nullptr);
+		CHECK_ARG(ring_ids_count > 0);
+		createOrUpdateNode<SetRingIdsRaysNode>(node, "SetRingIdsRaysNode", false,
+		                                       [&](SetRingIdsRaysNode& n) { n.setParameters(ring_ids, ring_ids_count); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_rays_set_range(rgl_node_t* node, const rgl_vec2f* ranges, int32_t ranges_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(ranges != nullptr);
+		CHECK_ARG(ranges_count > 0);
+		createOrUpdateNode<SetRangeRaysNode>(node, "SetRangeRaysNode", false, [&](SetRangeRaysNode& n) {
+			n.setParameters(reinterpret_cast<const Vec2f*>(ranges), ranges_count);
+		});
+	});
+}
+
+RGL_API rgl_status_t rgl_node_rays_set_time_offsets(rgl_node_t* node, const float* offsets, int32_t offsets_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(offsets != nullptr);
+		CHECK_ARG(offsets_count > 0);
+		createOrUpdateNode<SetTimeOffsetsRaysNode>(node, "SetTimeOffsetsRaysNode", false,
+		                                           [&](SetTimeOffsetsRaysNode& n) { n.setParameters(offsets, offsets_count); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_rays_transform(rgl_node_t* node, const rgl_mat3x4f* transform)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(transform != nullptr);
+		createOrUpdateNode<TransformRaysNode>(node, "TransformRaysNode", false,
+		                                      [&](TransformRaysNode& n) { n.setParameters(toMat(transform)); });
+	});
+}
+
+// ============================================================================================ points nodes
+RGL_API rgl_status_t rgl_node_points_transform(rgl_node_t* node, const rgl_mat3x4f* transform)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(transform != nullptr);
+		createOrUpdateNode<TransformPointsNode>(node, "TransformPointsNode", false,
+		                                        [&](TransformPointsNode& n) { n.setParameters(toMat(transform)); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace(rgl_node_t* node, rgl_scene_t scene)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(scene == nullptr);
+		createOrUpdateNode<RaytraceNode>(node, "RaytraceNode", false, [&](RaytraceNode& n) { n.setParameters(); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_velocity(rgl_node_t node, const rgl_vec3f* linear_velocity,
+                                                          const rgl_vec3f* angular_velocity)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(linear_velocity != nullptr);
+		CHECK_ARG(angular_velocity != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setVelocity(*reinterpret_cast<const Vec3f*>(linear_velocity), *reinterpret_cast<const Vec3f*>(angular_velocity));
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_distortion(rgl_node_t node, bool enable)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setDistortion(enable);
+		n->markDirty();
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_non_hits(rgl_node_t node, float nearDistance, float farDistance)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setNonHits(nearDistance, farDistance);
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_mask(rgl_node_t node, const int8_t* rays_mask, int32_t rays_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(rays_mask != nullptr);
+		CHECK_ARG(rays_count > 0);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setMask(rays_mask, rays_count);
+		n->markDirty();
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_beam_divergence(rgl_node_t node, float horizontal_beam_divergence,
+                                                                 float vertical_beam_divergence)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG((horizontal_beam_divergence > 0.0f && vertical_beam_divergence > 0.0f) ||
+		          (horizontal_beam_divergence == 0.0f && vertical_beam_divergence == 0.0f));
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setBeamDivergence(horizontal_beam_divergence, vertical_beam_divergence);
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_default_intensity(rgl_node_t node, float default_intensity)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(default_intensity >= 0.0f);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setDefaultIntensity(default_intensity);
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_configure_return_mode(rgl_node_t node, rgl_return_mode_t return_mode)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(return_mode != RGL_RETURN_UNKNOWN);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setReturnMode(return_mode);
+		if (n->hasCtx()) {
+			n->getCtx()->markNodesDirty();
+		}
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_format(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(fields != nullptr);
+		CHECK_ARG(field_count > 0);
+		std::vector<rgl_field_t> f(fields, fields + field_count);
+		createOrUpdateNode<FormatPointsNode>(node, "FormatPointsNode", true, [&](FormatPointsNode& n) { n.setParameters(f); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_yield(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(fields != nullptr);
+		CHECK_ARG(field_count > 0);
+		std::vector<rgl_field_t> f(fields, fields + field_count);
+		createOrUpdateNode<YieldPointsNode>(node, "YieldPointsNode", true, [&](YieldPointsNode& n) { n.setParameters(f); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_compact_by_field(rgl_node_t* node, rgl_field_t field)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(field == RGL_FIELD_IS_HIT_I32 || field == RGL_FIELD_IS_GROUND_I32);
+		createOrUpdateNode<CompactByFieldPointsNode>(node, "CompactByFieldPointsNode", false,
+		                                             [&](CompactByFieldPointsNode& n) { n.setParameters(field); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_compact(rgl_node_t* node) { return rgl_node_points_compact_by_field(node, RGL_FIELD_IS_HIT_I32); }
+
+RGL_API rgl_status_t rgl_node_points_spatial_merge(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(fields != nullptr);
+		CHECK_ARG(field_count > 0);
+		throw NotImplementedCall("spatial merge is outside the raytrace-and-point-cloud path");
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_temporal_merge(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(fields != nullptr);
+		CHECK_ARG(field_count > 0);
+		throw NotImplementedCall("temporal merge is outside the raytrace-and-point-cloud path");
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_from_array(rgl_node_t* node, const void* points, int32_t points_count,
+                                                const rgl_field_t* fields, int32_t field_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(points != nullptr);
+		CHECK_ARG(points_count > 0);
+		CHECK_ARG(fields != nullptr);
+		CHECK_ARG(field_count > 0);
+		std::vector<rgl_field_t> f(fields, fields + field_count);
+		createOrUpdateNode<FromArrayPointsNode>(node, "FromArrayPointsNode", true,
+		                                        [&](FromArrayPointsNode& n) { n.setParameters(points, points_count, f); });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const rgl_radar_scope_t* radar_scopes,
+                                                       int32_t radar_scopes_count, float ray_azimuth_step,
+                                                       float ray_elevation_step, float frequency, float power_transmitted,
+                                                       float cumulative_device_gain, float received_noise_mean,
+                                                       float received_noise_st_dev)
+{
+	return safeCall([&]() { throw NotImplementedCall("radar postprocessing is not implemented in this library yet"); });
+}
+
+RGL_API rgl_status_t rgl_node_points_radar_track_objects(rgl_node_t* node, float object_distance_threshold,
+                                                         float object_azimuth_threshold, float object_elevation_threshold,
+                                                         float object_radial_speed_threshold, float max_matching_distance,
+                                                         float max_prediction_time_frame, float movement_sensitivity)
+{
+	return safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+}
+
+RGL_API rgl_status_t rgl_node_points_radar_set_classes(rgl_node_t node, const int32_t* entity_ids,
+                                                       const rgl_radar_object_class_t* object_classes, int32_t count)
+{
+	return safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+}
+
+RGL_API rgl_status_t rgl_node_points_filter_ground(rgl_node_t* node, const rgl_vec3f* sensor_up_vector,
+                                                   float ground_angle_threshold)
+{
+	return safeCall([&]() { throw NotImplementedCall("ground filtering is outside the raytrace-and-point-cloud path"); });
+}
+
+// ============================================================================================ noise nodes
+RGL_API rgl_status_t rgl_node_gaussian_noise_angular_ray(rgl_node_t* node, float mean, float st_dev, rgl_axis_t rotation_axis)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(st_dev >= 0);
+		CHECK_ARG((rotation_axis == RGL_AXIS_X) || (rotation_axis == RGL_AXIS_Y) || (rotation_axis == RGL_AXIS_Z));
+		createOrUpdateNode<GaussianNoiseAngularRaysNode>(node, "GaussianNoiseAngularRaysNode", false,
+		                                                 [&](GaussianNoiseAngularRaysNode& n) {
+			                                                 n.setParameters(mean, st_dev, rotation_axis);
+		                                                 });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_gaussian_noise_angular_hitpoint(rgl_node_t* node, float mean, float st_dev,
+                                                              rgl_axis_t rotation_axis)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(st_dev >= 0);
+		CHECK_ARG((rotation_axis == RGL_AXIS_X) || (rotation_axis == RGL_AXIS_Y) || (rotation_axis == RGL_AXIS_Z));
+		createOrUpdateNode<GaussianNoiseAngularHitpointNode>(node, "GaussianNoiseAngularHitpointNode", false,
+		                                                     [&](GaussianNoiseAngularHitpointNode& n) {
+			                                                     n.setParameters(mean, st_dev, rotation_axis);
+		                                                     });
+	});
+}
+
+RGL_API rgl_status_t rgl_node_gaussian_noise_distance(rgl_node_t* node, float mean, float st_dev_base,
+                                                      float st_dev_rise_per_meter)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(st_dev_base >= 0);
+		CHECK_ARG(st_dev_rise_per_meter >= 0);
+		createOrUpdateNode<GaussianNoiseDistanceNode>(
+		    node, "GaussianNoiseDistanceNode", false,
+		    [&](GaussianNoiseDistanceNode& n) { n.setParameters(mean, st_dev_base, st_dev_rise_per_meter); });
+	});
+}
+
+// ============================================================================================ graph
+RGL_API rgl_status_t rgl_node_is_alive(rgl_node_t node, bool* out_alive)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_alive != nullptr);
+		*out_alive = nodeAlive(node);
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_run(rgl_node_t raw_node)
+{
+	return safeCall([&]() {
+		CHECK_ARG(raw_node != nullptr);
+		auto node = lookupNode(raw_node);
+		if (!node->hasCtx()) {
+			GraphRunCtx::createAndAttach(node);
+		}
+		node->getCtx()->executeAsync();
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_destroy(rgl_node_t raw_node)
+{
+	return safeCall([&]() {
+		CHECK_ARG(raw_node != nullptr);
+		auto node = lookupNode(raw_node);
+		if (node->hasCtx()) {
+			node->getCtx()->detachAndDestroy();
+		}
+		for (auto& n : node->disconnectConnectedNodes()) {
+			releaseNode(n.get());
+		}
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_get_result_size(rgl_node_t node, rgl_field_t field, int32_t* out_count, int32_t* out_size_of)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto pn = nodeAs<IPointsNode>(node, "IPointsNode");
+		pn->waitForResults();
+		checkOverflow(pn);
+		const int32_t count = pn->hostCount();
+		const int32_t size = static_cast<int32_t>(pn->fieldPointSize(field));
+		if (out_count != nullptr) {
+			*out_count = count;
+		}
+		if (out_size_of != nullptr) {
+			*out_size_of = size;
+		}
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_get_result_data(rgl_node_t node, rgl_field_t field, void* dst)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(dst != nullptr);
+		auto pn = nodeAs<IPointsNode>(node, "IPointsNode");
+		if (!pn->hasField(field)) {
+			throw InvalidPipeline("node " + pn->getName() + " does not provide field " + fieldName(field));
+		}
+		pn->waitForResults();
+		checkOverflow(pn);
+		const size_t bytes = static_cast<size_t>(pn->hostCount()) * pn->fieldPointSize(field);
+		if (bytes > 0) {
+			RGL_HIP(hipMemcpy(dst, pn->fieldData(field), bytes, hipMemcpyDeviceToHost));
+		}
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_node_add_child(rgl_node_t parent, rgl_node_t child)
+{
+	return safeCall([&]() {
+		CHECK_ARG(parent != nullptr);
+		CHECK_ARG(child != nullptr);
+		CHECK_ARG(parent != child);
+		lookupNode(parent)->addChild(lookupNode(child));
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_node_remove_child(rgl_node_t parent, rgl_node_t child)
+{
+	return safeCall([&]() {
+		CHECK_ARG(parent != nullptr);
+		CHECK_ARG(child != nullptr);
+		lookupNode(parent)->removeChild(lookupNode(child));
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_node_set_priority(rgl_node_t node, int32_t priority)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = lookupNode(node);
+		syncNodeGraph(n);
+		n->setPriority(priority);
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_node_get_priority(rgl_node_t node, int32_t* out_priority)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(out_priority != nullptr);
+		*out_priority = lookupNode(node)->getPriority();
+	});
+}
+
+// ============================================================================================ extensions
+RGL_API rgl_status_t rgl_node_gaussian_noise_set_seed(rgl_node_t node, uint64_t seed)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = lookupNode(node);
+		syncNodeGraph(n);
+		NoiseStream* s = nullptr;
+		if (auto a = std::dynamic_pointer_cast<GaussianNoiseAngularRaysNode>(n)) {
+			s = &a->rng;
+		} else if (auto b = std::dynamic_pointer_cast<GaussianNoiseAngularHitpointNode>(n)) {
+			s = &b->rng;
+		} else if (auto c = std::dynamic_pointer_cast<GaussianNoiseDistanceNode>(n)) {
+			s = &c->rng;
+		}
+		if (s == nullptr) {
+			throw InvalidAPIObject("RGL API Error: Node type mismatch: expected a Gaussian noise node, got " + n->getName());
+		}
+		s->seed = seed;
+		s->run = 0;
+	});
+}
+
+RGL_API rgl_status_t rgl_node_gaussian_noise_get_stream(rgl_node_t node, uint64_t* out_seed, uint32_t* out_run,
+                                                        uint32_t* out_tag)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(out_seed != nullptr && out_run != nullptr && out_tag != nullptr);
+		auto n = lookupNode(node);
+		const NoiseStream* s = nullptr;
+		if (auto a = std::dynamic_pointer_cast<GaussianNoiseAngularRaysNode>(n)) {
+			s = &a->rng;
+		} else if (auto b = std::dynamic_pointer_cast<GaussianNoiseAngularHitpointNode>(n)) {
+			s = &b->rng;
+		} else if (auto c = std::dynamic_pointer_cast<GaussianNoiseDistanceNode>(n)) {
+			s = &c->rng;
+		}
+		if (s == nullptr) {
+			throw InvalidAPIObject("RGL API Error: Node type mismatch: expected a Gaussian noise node, got " + n->getName());
+		}
+		*out_seed = s->seed;
+		*out_run = s->run;
+		*out_tag = s->tag;
+	});
+}
+
+RGL_API rgl_status_t rgl_scene_set_tlas_refit(rgl_scene_t scene, bool refit)
+{
+	return safeCall([&]() {
+		CHECK_ARG(scene == nullptr);
+		GraphRunCtx::synchronizeAll();
+		rgl::Scene::instance().tlasUpdate = refit ? rgl::TlasUpdate::Refit : rgl::TlasUpdate::Rebuild;
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t* nodes_visited, int32_t* tris_tested,
+                                                       int32_t* instances_entered, int32_t ray_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(nodes_visited != nullptr && tris_tested != nullptr && instances_entered != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		n->waitForResults();
+		const int rays = ray_count;
+		DeviceBuffer a, b, c;
+		n->traversalStats(a.ensureT<int32_t>(rays), b.ensureT<int32_t>(rays), c.ensureT<int32_t>(rays));
+		RGL_HIP(hipStreamSynchronize(n->stream()));
+		RGL_HIP(hipMemcpy(nodes_visited, a.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+		RGL_HIP(hipMemcpy(tris_tested, b.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+		RGL_HIP(hipMemcpy(instances_entered, c.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t, int32_t* out_instance,
+                                                   int32_t* out_prim, int32_t ray_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(out_t != nullptr && out_instance != nullptr && out_prim != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		n->waitForResults();
+		const int rays = ray_count;
+		DeviceBuffer a, b, c;
+		n->bruteForce(a.ensureT<float>(rays), b.ensureT<int32_t>(rays), c.ensureT<int32_t>(rays));
+		RGL_HIP(hipMemcpy(out_t, a.get(), sizeof(float) * rays, hipMemcpyDeviceToHost));
+		RGL_HIP(hipMemcpy(out_instance, b.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+		RGL_HIP(hipMemcpy(out_prim, c.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+	});
+}
+
+RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out != nullptr);
+		CHECK_ARG(count > 0);
+		DeviceBuffer d;
+		gpu::noiseNormals(nullptr, count, d.ensureT<float>(count), seed, run, tag);
+		RGL_HIP(hipGetLastError());
+		RGL_HIP(hipDeviceSynchronize());
+		RGL_HIP(hipMemcpy(out, d.get(), sizeof(float) * count, hipMemcpyDeviceToHost));
+	});
+}

@@ -1,0 +1,532 @@
+// LBVH construction and refit on gfx950.
+//
+// Replaces the reference's OptiX GAS/IAS builds (src/scene/GASBuilder.hpp:31-100, src/scene/Scene.cpp:131-190),
+// whose algorithms are closed source. The builder here is a Morton-code LBVH:
+//   1. primitive boxes -> centroid bounds (ordered-int atomics),
+//   2. 63-bit Morton keys (21 bits per axis),
+//   3. stable LSD radix sort of (key, primitive) pairs, 8 bits per pass; each tile ranks its keys in LDS
+//      with wave64 ballot matching,
+//   4. Karras 2012 ("Maximizing parallelism in the construction of BVHs, octrees and k-d trees") topology,
+//   5. bottom-up box fit with acquire/release counters; the same pass is the refit path for moved geometry.
+// Node layout: BvhNode (two child boxes + two child references, 64 B).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <utility>
+
+#include "device_types.hpp"
+#include "kernels.hpp"
+
+namespace rgl::gpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems; // 4096 keys per tile
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ uint32_t floatToOrdered(float f)
+{
+	uint32_t b = __float_as_uint(f);
+	return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float orderedToFloat(uint32_t o)
+{
+	uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+	return __uint_as_float(b);
+}
+
+__global__ void kInitBounds(uint32_t* bounds)
+{
+	if (threadIdx.x < 3) {
+		bounds[threadIdx.x] = 0xffffffffu;     // min
+		bounds[3 + threadIdx.x] = 0u;          // max
+	}
+}
+
+__global__ void kCentroidBounds(int n, const Aabb* boxes, uint32_t* bounds)
+{
+	__shared__ float smin[3][kBlock];
+	__shared__ float smax[3][kBlock];
+	float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+	float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+		const Aabb b = boxes[i];
+		for (int a = 0; a < 3; ++a) {
+			const float c = 0.5f * (b.lo[a] + b.hi[a]);
+			lo[a] = fminf(lo[a], c);
+			hi[a] = fmaxf(hi[a], c);
+		}
+	}
+	for (int a = 0; a < 3; ++a) {
+		smin[a][threadIdx.x] = lo[a];
+		smax[a][threadIdx.x] = hi[a];
+	}
+	__syncthreads();
+	for (int s = kBlock / 2; s > 0; s >>= 1) {
+		if (threadIdx.x < s) {
+			for (int a = 0; a < 3; ++a) {
+				smin[a][threadIdx.x] = fminf(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+				smax[a][threadIdx.x] = fmaxf(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		for (int a = 0; a < 3; ++a) {
+			atomicMin(&bounds[a], floatToOrdered(smin[a][0]));
+			atomicMax(&bounds[3 + a], floatToOrdered(smax[a][0]));
+		}
+	}
+}
+
+__device__ __forceinline__ uint64_t expandBits21(uint32_t v)
+{
+	uint64_t x = v & 0x1fffffu;
+	x = (x | (x << 32)) & 0x1f00000000ffffull;
+	x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+	x = (x | (x << 8)) & 0x100f00f00f00f00full;
+	x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+	x = (x | (x << 2)) & 0x1249249249249249ull;
+	return x;
+}
+
+__global__ void kMorton(int n, const Aabb* boxes, const uint32_t* bounds, uint64_t* keys, int32_t* vals)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	float lo[3], ext[3];
+	for (int a = 0; a < 3; ++a) {
+		lo[a] = orderedToFloat(bounds[a]);
+		ext[a] = orderedToFloat(bounds[3 + a]) - lo[a];
+	}
+	const Aabb b = boxes[i];
+	uint32_t q[3];
+	for (int a = 0; a < 3; ++a) {
+		const float c = 0.5f * (b.lo[a] + b.hi[a]);
+		float t = ext[a] > 0.0f ? (c - lo[a]) / ext[a] : 0.5f;
+		t = fminf(fmaxf(t, 0.0f), 1.0f);
+		q[a] = static_cast<uint32_t>(fminf(t * 2097152.0f, 2097151.0f));
+	}
+	keys[i] = (expandBits21(q[0]) << 2) | (expandBits21(q[1]) << 1) | expandBits21(q[2]);
+	vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------------------- radix sort
+__global__ void kSortHistogram(int n, const uint64_t* keys, int shift, int numTiles, uint32_t* hist)
+{
+	__shared__ uint32_t sh[256];
+	sh[threadIdx.x] = 0;
+	__syncthreads();
+	const int base = blockIdx.x * kSortTile;
+	for (int r = 0; r < kSortItems; ++r) {
+		const int i = base + r * kBlock + threadIdx.x;
+		if (i < n) {
+			atomicAdd(&sh[(keys[i] >> shift) & 0xffu], 1u);
+		}
+	}
+	__syncthreads();
+	hist[threadIdx.x * numTiles + blockIdx.x] = sh[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major histogram in one block of 1024 threads.
+__global__ void kSortScan(int count, uint32_t* hist)
+{
+	__shared__ uint32_t partial[1024];
+	const int per = (count + 1023) / 1024;
+	const int beg = threadIdx.x * per;
+	const int end = min(beg + per, count);
+	uint32_t sum = 0;
+	for (int i = beg; i < end; ++i) {
+		sum += hist[i];
+	}
+	partial[threadIdx.x] = sum;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		uint32_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0u;
+		__syncthreads();
+		partial[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint32_t run = partial[threadIdx.x] - sum; // exclusive prefix of this chunk
+	for (int i = beg; i < end; ++i) {
+		const uint32_t v = hist[i];
+		hist[i] = run;
+		run += v;
+	}
+}
+
+__global__ void kSortScatter(int n, const uint64_t* keysIn, const int32_t* valsIn, uint64_t* keysOut,
+                             int32_t* valsOut, int shift, int numTiles, const uint32_t* hist)
+{
+	__shared__ uint32_t gOff[256];
+	__shared__ uint32_t base[256];
+	__shared__ uint32_t cnt[kWaves][256];
+	const int tid = threadIdx.x;
+	const int wave = tid >> 6;
+	const int lane = tid & 63;
+	gOff[tid] = hist[tid * numTiles + blockIdx.x];
+	base[tid] = 0;
+	const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	const int tileBase = blockIdx.x * kSortTile;
+	for (int r = 0; r < kSortItems; ++r) {
+		for (int w = 0; w < kWaves; ++w) {
+			cnt[w][tid] = 0;
+		}
+		__syncthreads();
+		const int i = tileBase + r * kBlock + tid;
+		const bool valid = i < n;
+		uint64_t key = valid ? keysIn[i] : 0ull;
+		int32_t val = valid ? valsIn[i] : 0;
+		const uint32_t digit = valid ? static_cast<uint32_t>((key >> shift) & 0xffu) : 0u;
+		uint64_t peers = __ballot(valid);
+		for (int b = 0; b < 8; ++b) {
+			const bool bit = (digit >> b) & 1u;
+			const uint64_t m = __ballot(bit);
+			peers &= bit ? m : ~m;
+		}
+		const uint32_t rankInWave = static_cast<uint32_t>(__popcll(peers & lanesBelow));
+		if (valid && rankInWave == 0) {
+			cnt[wave][digit] = static_cast<uint32_t>(__popcll(peers));
+		}
+		__syncthreads();
+		{
+			uint32_t run = base[tid];
+			for (int w = 0; w < kWaves; ++w) {
+				const uint32_t c = cnt[w][tid];
+				cnt[w][tid] = run;
+				run += c;
+			}
+			base[tid] = run;
+		}
+		__syncthreads();
+		if (valid) {
+			const uint32_t pos = gOff[digit] + cnt[wave][digit] + rankInWave;
+			keysOut[pos] = key;
+			valsOut[pos] = val;
+		}
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------------------- topology
+__device__ __forceinline__ int prefixDelta(int n, const uint64_t* keys, int i, int j)
+{
+	if (j < 0 || j >= n) {
+		return -1;
+	}
+	const uint64_t a = keys[i], b = keys[j];
+	if (a == b) {
+		return 64 + __clz(static_cast<uint32_t>(i ^ j));
+	}
+	return __clzll(a ^ b);
+}
+
+// parentSlot[x] = (parent internal node << 1) | child slot; leaves at x in [0, n), internal node k at n + k.
+__global__ void kKarras(int n, const uint64_t* keys, const int32_t* leafPrim, int leafRefIsPrim, BvhNode* nodes,
+                        int32_t* parentSlot)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n - 1) {
+		return;
+	}
+	const int d = (prefixDelta(n, keys, i, i + 1) - prefixDelta(n, keys, i, i - 1)) >= 0 ? 1 : -1;
+	const int dmin = prefixDelta(n, keys, i, i - d);
+	int lmax = 2;
+	while (prefixDelta(n, keys, i, i + lmax * d) > dmin) {
+		lmax <<= 1;
+	}
+	int l = 0;
+	for (int t = lmax >> 1; t >= 1; t >>= 1) {
+		if (prefixDelta(n, keys, i, i + (l + t) * d) > dmin) {
+			l += t;
+		}
+	}
+	const int j = i + l * d;
+	const int dnode = prefixDelta(n, keys, i, j);
+	int s = 0;
+	int t = l;
+	do {
+		t = (t + 1) >> 1;
+		if (prefixDelta(n, keys, i, i + (s + t) * d) > dnode) {
+			s += t;
+		}
+	} while (t > 1);
+	const int gamma = i + s * d + min(d, 0);
+	const int lo = min(i, j), hi = max(i, j);
+	int c0, c1;
+	if (lo == gamma) {
+		c0 = ~(leafRefIsPrim ? leafPrim[gamma] : gamma);
+		parentSlot[gamma] = (i << 1) | 0;
+	} else {
+		c0 = gamma;
+		parentSlot[n + gamma] = (i << 1) | 0;
+	}
+	if (hi == gamma + 1) {
+		c1 = ~(leafRefIsPrim ? leafPrim[gamma + 1] : gamma + 1);
+		parentSlot[gamma + 1] = (i << 1) | 1;
+	} else {
+		c1 = gamma + 1;
+		parentSlot[n + gamma + 1] = (i << 1) | 1;
+	}
+	nodes[i].child[0] = c0;
+	nodes[i].child[1] = c1;
+	nodes[i].pad[0] = 0;
+	nodes[i].pad[1] = 0;
+	if (i == 0) {
+		parentSlot[n] = -1;
+	}
+}
+
+__global__ void kSingleLeafRoot(const Aabb* boxes, int leafRefIsPrim, BvhNode* nodes, int32_t* height)
+{
+	BvhNode r;
+	for (int a = 0; a < 3; ++a) {
+		r.f[a] = boxes[0].lo[a];
+		r.f[3 + a] = boxes[0].hi[a];
+		r.f[6 + a] = FLT_MAX;
+		r.f[9 + a] = -FLT_MAX;
+	}
+	r.child[0] = ~0;
+	r.child[1] = ~0;
+	r.pad[0] = r.pad[1] = 0;
+	nodes[0] = r;
+	height[0] = 1;
+	(void) leafRefIsPrim;
+}
+
+__global__ void kClearInts(int n, int32_t* p)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) {
+		p[i] = 0;
+	}
+}
+
+__device__ __forceinline__ void storeSlot(BvhNode* nodes, int slotCode, const float* box)
+{
+	float* f = nodes[slotCode >> 1].f + 6 * (slotCode & 1);
+	for (int a = 0; a < 6; ++a) {
+		f[a] = box[a];
+	}
+}
+
+// Bottom-up fit. One thread per leaf; the second thread to reach an internal node unions its two child boxes
+// and continues upward. Counters use acquire/release at agent scope so the sibling's box is visible.
+// `height[k]` receives the height of internal node k (leaves have height 0); height[0] is the tree depth.
+__global__ void kFit(int n, const Aabb* boxes, const int32_t* leafPrim, const int32_t* parentSlot, BvhNode* nodes,
+                     int32_t* counters, int32_t* height)
+{
+	const int leaf = blockIdx.x * kBlock + threadIdx.x;
+	if (leaf >= n) {
+		return;
+	}
+	const Aabb b = boxes[leafPrim[leaf]];
+	float box[6] = {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
+	int code = parentSlot[leaf];
+	while (code >= 0) {
+		storeSlot(nodes, code, box);
+		const int p = code >> 1;
+		const int prev = __hip_atomic_fetch_add(&counters[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		if (prev == 0) {
+			return; // sibling not done yet; it will carry the union upward
+		}
+		// Both children are fitted: union them. Loads after the acquire see the sibling's stores.
+		const float* f = nodes[p].f;
+		for (int a = 0; a < 3; ++a) {
+			box[a] = fminf(f[a], f[6 + a]);
+			box[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+		}
+		const int c0 = nodes[p].child[0], c1 = nodes[p].child[1];
+		const int hc0 = c0 >= 0 ? height[c0] : 0;
+		const int hc1 = c1 >= 0 ? height[c1] : 0;
+		height[p] = 1 + max(hc0, hc1);
+		code = parentSlot[n + p];
+	}
+}
+
+__global__ void kTriangleBoxes(int n, const Vec3f* v, const Vec3i* idx, Aabb* out)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const Vec3i t = idx[i];
+	const Vec3f a = v[t.x], b = v[t.y], c = v[t.z];
+	Aabb r;
+	r.lo[0] = fminf(a.x, fminf(b.x, c.x));
+	r.lo[1] = fminf(a.y, fminf(b.y, c.y));
+	r.lo[2] = fminf(a.z, fminf(b.z, c.z));
+	r.hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+	r.hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+	r.hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+	out[i] = r;
+}
+
+__global__ void kGatherTris(int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const int p = leafPrim[i];
+	const Vec3i t = idx[p];
+	const Vec3f a = v[t.x], b = v[t.y], c = v[t.z];
+	LeafTri r;
+	r.ax = a.x;
+	r.ay = a.y;
+	r.az = a.z;
+	r.prim = p;
+	r.bx = b.x;
+	r.by = b.y;
+	r.bz = b.z;
+	r.pad0 = 0.0f;
+	r.cx = c.x;
+	r.cy = c.y;
+	r.cz = c.z;
+	r.pad1 = 0.0f;
+	out[i] = r;
+}
+
+// World box of each instance: the 8 corners of its BLAS root box through objectToWorld, widened by a relative
+// 2^-20 so that FP32 rounding of the corners never makes the box smaller than the true image.
+__global__ void kInstanceBoxes(int n, const InstanceShade* shades, const InstanceRec* recs, Aabb* out)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const float* f = recs[i].blasNodes[0].f;
+	float lo[3], hi[3];
+	for (int a = 0; a < 3; ++a) {
+		lo[a] = fminf(f[a], f[6 + a]);
+		hi[a] = fmaxf(f[3 + a], f[9 + a]);
+	}
+	const Mat3x4f& m = shades[i].objectToWorld;
+	Aabb r;
+	for (int a = 0; a < 3; ++a) {
+		r.lo[a] = FLT_MAX;
+		r.hi[a] = -FLT_MAX;
+	}
+	for (int c = 0; c < 8; ++c) {
+		const Vec3f p{(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+		const Vec3f w = m * p;
+		for (int a = 0; a < 3; ++a) {
+			r.lo[a] = fminf(r.lo[a], w[a]);
+			r.hi[a] = fmaxf(r.hi[a], w[a]);
+		}
+	}
+	for (int a = 0; a < 3; ++a) {
+		const float pad = (fabsf(r.lo[a]) + fabsf(r.hi[a]) + (r.hi[a] - r.lo[a])) * 9.5367431640625e-7f + 1e-30f;
+		r.lo[a] -= pad;
+		r.hi[a] += pad;
+	}
+	out[i] = r;
+}
+
+inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
+
+} // namespace
+
+size_t lbvhScratchBytes(int n)
+{
+	const size_t tiles = (static_cast<size_t>(n) + kSortTile - 1) / kSortTile;
+	size_t bytes = 0;
+	bytes += 2 * sizeof(uint64_t) * n;     // keys, keysAlt
+	bytes += 2 * sizeof(int32_t) * n;      // vals, valsAlt
+	bytes += sizeof(uint32_t) * 256 * tiles;
+	bytes += sizeof(uint32_t) * 8;
+	bytes += 4096;                          // alignment slack
+	return bytes;
+}
+
+void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim, BvhNode* nodes, int32_t* leafPrim,
+               int32_t* parentSlot, int32_t* counters, int32_t* height, void* scratch)
+{
+	if (n <= 0) {
+		return;
+	}
+	if (n == 1) {
+		hipLaunchKernelGGL(kSingleLeafRoot, dim3(1), dim3(1), 0, stream, boxes, leafRefIsPrim ? 1 : 0, nodes, height);
+		hipLaunchKernelGGL(kClearInts, dim3(1), dim3(kBlock), 0, stream, 1, leafPrim);
+		return;
+	}
+	const int tiles = (n + kSortTile - 1) / kSortTile;
+	char* s = static_cast<char*>(scratch);
+	auto take = [&](size_t bytes) {
+		char* p = s;
+		s += (bytes + 255) & ~static_cast<size_t>(255);
+		return p;
+	};
+	uint64_t* keys = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * n));
+	uint64_t* keysAlt = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * n));
+	int32_t* valsAlt = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+	uint32_t* hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256 * tiles));
+	uint32_t* bounds = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 8));
+
+	hipLaunchKernelGGL(kInitBounds, dim3(1), dim3(64), 0, stream, bounds);
+	const int rb = std::min(blocksFor(n), 1024);
+	hipLaunchKernelGGL(kCentroidBounds, dim3(rb), dim3(kBlock), 0, stream, n, boxes, bounds);
+	hipLaunchKernelGGL(kMorton, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, boxes, bounds, keys, leafPrim);
+
+	uint64_t* kIn = keys;
+	uint64_t* kOut = keysAlt;
+	int32_t* vIn = leafPrim;
+	int32_t* vOut = valsAlt;
+	for (int pass = 0; pass < 8; ++pass) {
+		const int shift = pass * 8;
+		hipLaunchKernelGGL(kSortHistogram, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, shift, tiles, hist);
+		hipLaunchKernelGGL(kSortScan, dim3(1), dim3(1024), 0, stream, 256 * tiles, hist);
+		hipLaunchKernelGGL(kSortScatter, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, vIn, kOut, vOut, shift, tiles,
+		                   hist);
+		std::swap(kIn, kOut);
+		std::swap(vIn, vOut);
+	}
+	// 8 passes: sorted data is back in (keys, leafPrim).
+	hipLaunchKernelGGL(kKarras, dim3(blocksFor(n - 1)), dim3(kBlock), 0, stream, n, keys, leafPrim,
+	                   leafRefIsPrim ? 1 : 0, nodes, parentSlot);
+	refitLbvh(stream, n, boxes, leafPrim, parentSlot, nodes, counters, height);
+}
+
+void refitLbvh(hipStream_t stream, int n, const Aabb* boxes, const int32_t* leafPrim, const int32_t* parentSlot,
+               BvhNode* nodes, int32_t* counters, int32_t* height)
+{
+	if (n <= 1) {
+		if (n == 1) {
+			hipLaunchKernelGGL(kSingleLeafRoot, dim3(1), dim3(1), 0, stream, boxes, 0, nodes, height);
+		}
+		return;
+	}
+	hipLaunchKernelGGL(kClearInts, dim3(blocksFor(n - 1)), dim3(kBlock), 0, stream, n - 1, counters);
+	hipLaunchKernelGGL(kFit, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, boxes, leafPrim, parentSlot, nodes,
+	                   counters, height);
+}
+
+void triangleBoxes(hipStream_t stream, int n, const Vec3f* v, const Vec3i* idx, Aabb* out)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kTriangleBoxes, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, v, idx, out);
+	}
+}
+
+void gatherTris(hipStream_t stream, int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kGatherTris, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, leafPrim, v, idx, out);
+	}
+}
+
+void instanceBoxes(hipStream_t stream, int n, const InstanceShade* shades, const InstanceRec* recs, Aabb* out)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kInstanceBoxes, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, shades, recs, out);
+	}
+}
+
+} // namespace rgl::gpu

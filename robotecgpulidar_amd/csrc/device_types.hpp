@@ -1,0 +1,145 @@
+// Data layouts that live in HBM and are shared between the host runtime and the HIP kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "math.hpp"
+
+namespace rgl {
+
+// ------------------------------------------------------------------------------------------------ BVH
+// Binary BVH node, 64 B: the two child boxes stored in the parent (so one node fetch decides both children)
+// and the two child references.
+//   f[0..5]   child 0 box (lo.x lo.y lo.z hi.x hi.y hi.z)
+//   f[6..11]  child 1 box
+//   i[12..13] child references: >= 0 internal node index, < 0 leaf (~leafIndex)
+//   i[14..15] unused (keeps the node at 64 B so it is fetched as four 16 B loads)
+// An empty child slot has an inverted box (lo = +inf, hi = -inf), which no slab test accepts.
+struct alignas(16) BvhNode
+{
+	float f[12];
+	int32_t child[2];
+	int32_t pad[2];
+};
+static_assert(sizeof(BvhNode) == 64, "BVH node must be 64 B");
+
+// Leaf triangle in BVH leaf order, 48 B: object-space vertices A, B, C. w of the first vec4 holds the
+// triangle's index in the mesh index buffer (bit pattern of an int32), which the shading stage uses for UVs
+// and which breaks distance ties deterministically.
+struct alignas(16) LeafTri
+{
+	float ax, ay, az;
+	int32_t prim;
+	float bx, by, bz, pad0;
+	float cx, cy, cz, pad1;
+};
+static_assert(sizeof(LeafTri) == 48, "leaf triangle must be 48 B");
+
+// Instance record read by traversal when it enters an instance (64 B + 64 B, two cache-friendly halves).
+struct alignas(16) InstanceRec
+{
+	Mat3x4f worldToObject;      // FP32, composed in FP64 on the host and rounded once
+	const BvhNode* blasNodes;   // BLAS of the instance's mesh
+	const LeafTri* blasTris;
+};
+static_assert(sizeof(InstanceRec) == 64, "instance record must be 64 B");
+
+// Per-instance data used only when shading a hit (closest-hit stage).
+struct alignas(16) InstanceShade
+{
+	Mat3x4f objectToWorld;      // entity transform as the user gave it (FP32)
+	Mat3x4f prevObjectToWorld;  // previous-frame transform for velocities
+	const Vec3f* vertices;      // mesh vertices (object space)
+	const Vec3i* indices;       // mesh index buffer
+	const Vec2f* uvs;           // nullptr if the mesh has no texture coordinates
+	const uint8_t* texels;      // nullptr if the entity has no intensity texture
+	int32_t texWidth;
+	int32_t texHeight;
+	int32_t entityId;
+	float laserRetro;
+	int32_t hasPrev;
+	int32_t pad[3];
+};
+
+// Device view of a scene for one launch.
+struct SceneView
+{
+	const BvhNode* tlasNodes;       // nullptr if the scene is empty
+	const InstanceRec* instances;
+	const InstanceShade* shades;
+	int32_t instanceCount;
+	int32_t* overflow;              // set to 1 if a traversal stack would overflow (the ray is then cut short)
+};
+
+// ------------------------------------------------------------------------------------------------ raytrace
+// Output field pointers of a raytrace launch (nullptr = field not requested). Layout: return point
+// `beam * returnCount + r`.
+struct RaytraceOutputs
+{
+	Vec3f* xyz;
+	int32_t* isHit;
+	uint32_t* rayIdx;
+	uint16_t* ringId;
+	uint8_t* returnType;
+	float* distance;
+	float* intensityF32;
+	uint8_t* intensityU8;
+	float* laserRetro;
+	double* timeStampF64;
+	uint32_t* timeStampU32;
+	int32_t* entityId;
+	Vec3f* absVelocity;
+	Vec3f* relVelocity;
+	float* radialSpeed;
+	float* azimuth;
+	float* elevation;
+	Vec3f* normal;
+	float* incidentAngle;
+};
+
+struct RaytraceParams
+{
+	const Mat3x4f* rays;          // world-frame rays (after the rays-node chain)
+	int32_t rayCount;
+	Mat3x4f rayOriginToWorld;     // cumulative ray transform of the rays chain
+	Mat3x4f worldToRayOrigin;     // its inverse, computed once on the host
+	const Vec2f* ranges;          // 1 or rayCount entries
+	int32_t rangesCount;
+	const int32_t* ringIds;
+	int32_t ringIdsCount;
+	const float* timeOffsetsMs;
+	const int8_t* mask;
+	int32_t doDistortion;
+	Vec3f sensorLinearVelocity;
+	Vec3f sensorAngularVelocity;
+	float nearNonHitDistance;
+	float farNonHitDistance;
+	float defaultIntensity;
+	float sceneDeltaTime;         // seconds, 0 if unknown
+	float hHalfDivergence;        // radians
+	float vHalfDivergence;
+	int32_t returnMode;
+	int32_t returnCount;
+	int32_t velocityRequested;
+};
+
+// ------------------------------------------------------------------------------------------------ points
+constexpr int kMaxFormatFields = 32;
+constexpr int kMaxCompactFields = 32;
+
+struct FieldCopy
+{
+	const char* src;
+	char* dst;
+	int32_t size;    // bytes per element
+	int32_t offset;  // byte offset inside an AoS record (format / from-array only)
+};
+
+struct FieldCopyList
+{
+	FieldCopy f[kMaxCompactFields];
+	int32_t count;
+};
+
+} // namespace rgl

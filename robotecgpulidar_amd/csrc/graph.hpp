@@ -1,0 +1,525 @@
+// Graph model: nodes, the run context of a connected graph, and the node kinds on the raytrace-and-point-cloud
+// path.
+//
+// Structure follows the reference (src/graph/Node.{hpp,cpp}, src/graph/Interfaces.hpp:24-145,
+// src/graph/GraphRunCtx.cpp:23-211, src/graph/NodesCore.hpp): create-or-update nodes, a DAG executed in the
+// reference's priority-DFS order, one stream per graph, and node interfaces for rays and point clouds.
+// Differences that matter for MI355X: nodes are enqueued from the calling thread (HIP launches are already
+// asynchronous, so no thread is spawned per run), and point counts that are only known on the device after
+// compaction stay on the device; downstream kernels read them there and the host reads them only when a result
+// is requested.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <optional>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "runtime.hpp"
+#include "scene.hpp"
+
+namespace rgl {
+
+struct GraphRunCtx;
+
+struct Node : std::enable_shared_from_this<Node>
+{
+	using Ptr = std::shared_ptr<Node>;
+	virtual ~Node() = default;
+
+	void addChild(const Ptr& child);
+	void removeChild(const Ptr& child);
+	void setPriority(int32_t p);
+	int32_t getPriority() const { return priority; }
+
+	const std::vector<Ptr>& getInputs() const { return inputs; }
+	const std::vector<Ptr>& getOutputs() const { return outputs; }
+	std::set<Ptr> connectedComponent();
+	std::set<Ptr> disconnectConnectedNodes();
+
+	bool hasCtx() const { return ctx != nullptr; }
+	std::shared_ptr<GraphRunCtx> getCtx() const { return ctx; }
+	void setCtx(std::shared_ptr<GraphRunCtx> c);
+
+	void validate();
+	void enqueue();
+	// Waits until this node's device work of the last run is complete; throws if the node never ran.
+	void waitForResults();
+	void markDirty() { dirty = true; }
+	bool isValid() const { return !dirty; }
+
+	virtual std::string getName() const = 0;
+	hipStream_t stream() const;
+
+	template<typename T>
+	std::shared_ptr<T> exactlyOneInputOfType() const;
+
+protected:
+	virtual void validateImpl() = 0;
+	virtual void enqueueImpl() = 0;
+
+	std::vector<Ptr> inputs;
+	std::vector<Ptr> outputs; // sorted by priority, descending
+	int32_t priority = 0;
+	bool dirty = true;
+	bool ranOnce = false;
+	std::shared_ptr<GraphRunCtx> ctx;
+	std::unique_ptr<Event> done;
+	friend struct GraphRunCtx;
+};
+
+// ------------------------------------------------------------------------------------------------ rays
+struct IRaysNode : virtual Node
+{
+	static constexpr const char* kTypeName = "IRaysNode";
+	virtual int rayCount() const = 0;
+	virtual const Mat3x4f* rays() const = 0;
+	virtual const int32_t* ringIds() const { return nullptr; }
+	virtual int ringIdsCount() const { return 0; }
+	virtual const Vec2f* ranges() const { return nullptr; }
+	virtual int rangesCount() const { return 0; }
+	virtual const float* timeOffsets() const { return nullptr; }
+	virtual int timeOffsetsCount() const { return 0; }
+	virtual Mat3x4f cumulativeRayTransform() const { return Mat3x4f::identity(); }
+};
+
+struct IRaysNodeSingleInput : IRaysNode
+{
+	void validateImpl() override;
+	int rayCount() const override { return input->rayCount(); }
+	const Mat3x4f* rays() const override { return input->rays(); }
+	const int32_t* ringIds() const override { return input->ringIds(); }
+	int ringIdsCount() const override { return input->ringIdsCount(); }
+	const Vec2f* ranges() const override { return input->ranges(); }
+	int rangesCount() const override { return input->rangesCount(); }
+	const float* timeOffsets() const override { return input->timeOffsets(); }
+	int timeOffsetsCount() const override { return input->timeOffsetsCount(); }
+	Mat3x4f cumulativeRayTransform() const override { return input->cumulativeRayTransform(); }
+
+protected:
+	std::shared_ptr<IRaysNode> input;
+};
+
+// ------------------------------------------------------------------------------------------------ points
+struct IPointsNode : virtual Node
+{
+	static constexpr const char* kTypeName = "IPointsNode";
+	virtual std::vector<rgl_field_t> requiredFields() const { return {}; }
+	virtual bool isDense() const = 0;
+	virtual bool hasField(rgl_field_t f) const = 0;
+	virtual std::vector<rgl_field_t> fieldList() const = 0;
+	// Upper bound of the point count, known on the host at enqueue time.
+	virtual int capacity() const = 0;
+	// Device-resident live point count, or nullptr when the count equals capacity().
+	virtual const int32_t* countDevice() const { return nullptr; }
+	// Live point count on the host; the caller has waited for this node's results.
+	virtual int hostCount() { return capacity(); }
+	virtual rgl_return_mode_t returnMode() const = 0;
+	virtual int returnCount() const = 0;
+	virtual Mat3x4f lookAtOriginTransform() const { return Mat3x4f::identity(); }
+	// Device pointer of a field (SoA), valid for the live count.
+	virtual const void* fieldData(rgl_field_t f) = 0;
+	virtual size_t fieldPointSize(rgl_field_t f) const { return fieldSizeChecked(f); }
+};
+
+struct IPointsNodeSingleInput : IPointsNode
+{
+	void validateImpl() override;
+	bool isDense() const override { return input->isDense(); }
+	bool hasField(rgl_field_t f) const override { return input->hasField(f); }
+	std::vector<rgl_field_t> fieldList() const override { return input->fieldList(); }
+	int capacity() const override { return input->capacity(); }
+	const int32_t* countDevice() const override { return input->countDevice(); }
+	int hostCount() override { return input->hostCount(); }
+	rgl_return_mode_t returnMode() const override { return input->returnMode(); }
+	int returnCount() const override { return input->returnCount(); }
+	Mat3x4f lookAtOriginTransform() const override { return input->lookAtOriginTransform(); }
+	const void* fieldData(rgl_field_t f) override { return input->fieldData(f); }
+
+protected:
+	std::shared_ptr<IPointsNode> input;
+};
+
+// ------------------------------------------------------------------------------------------------ run context
+struct GraphRunCtx : std::enable_shared_from_this<GraphRunCtx>
+{
+	static std::shared_ptr<GraphRunCtx> createAndAttach(const Node::Ptr& node);
+	static void synchronizeAll();
+	static std::vector<std::shared_ptr<GraphRunCtx>>& instances();
+
+	void executeAsync();
+	void synchronize();
+	void detachAndDestroy();
+	void markNodesDirty();
+	void clearExecutionOrder() { order.clear(); }
+	hipStream_t stream() const { return stream_->get(); }
+	const std::set<Node::Ptr>& getNodes() const { return nodes; }
+
+	static std::vector<Node::Ptr> findExecutionOrder(std::set<Node::Ptr> nodes);
+
+private:
+	std::set<Node::Ptr> nodes;
+	std::vector<Node::Ptr> order;
+	std::unique_ptr<Stream> stream_ = std::make_unique<Stream>();
+	bool running = false;
+};
+
+// ------------------------------------------------------------------------------------------------ node kinds
+struct FromMat3x4fRaysNode : IRaysNode
+{
+	void setParameters(const Mat3x4f* r, int count);
+	std::string getName() const override { return "FromMat3x4fRaysNode"; }
+	int rayCount() const override { return count; }
+	const Mat3x4f* rays() const override { return buf.as<Mat3x4f>(); }
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override {}
+
+private:
+	DeviceBuffer buf;
+	int count = 0;
+};
+
+struct SetRingIdsRaysNode : IRaysNodeSingleInput
+{
+	void setParameters(const int32_t* ids, int count);
+	std::string getName() const override { return "SetRingIdsRaysNode"; }
+	const int32_t* ringIds() const override { return buf.as<int32_t>(); }
+	int ringIdsCount() const override { return count; }
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override {}
+
+private:
+	DeviceBuffer buf;
+	int count = 0;
+};
+
+struct SetRangeRaysNode : IRaysNodeSingleInput
+{
+	void setParameters(const Vec2f* r, int count);
+	std::string getName() const override { return "SetRangeRaysNode"; }
+	const Vec2f* ranges() const override { return buf.as<Vec2f>(); }
+	int rangesCount() const override { return count; }
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override {}
+
+private:
+	DeviceBuffer buf;
+	int count = 0;
+};
+
+struct SetTimeOffsetsRaysNode : IRaysNodeSingleInput
+{
+	void setParameters(const float* offsets, int count);
+	std::string getName() const override { return "SetTimeOffsetsRaysNode"; }
+	const float* timeOffsets() const override { return buf.as<float>(); }
+	int timeOffsetsCount() const override { return count; }
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override {}
+
+private:
+	DeviceBuffer buf;
+	int count = 0;
+};
+
+struct TransformRaysNode : IRaysNodeSingleInput
+{
+	void setParameters(const Mat3x4f& tf) { transform = tf; }
+	std::string getName() const override { return "TransformRaysNode"; }
+	const Mat3x4f* rays() const override { return out.as<Mat3x4f>(); }
+	Mat3x4f cumulativeRayTransform() const override { return transform * input->cumulativeRayTransform(); }
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	Mat3x4f transform = Mat3x4f::identity();
+	DeviceBuffer out;
+};
+
+// Seeded noise: stream identity of one noise node.
+struct NoiseStream
+{
+	uint64_t seed = 0;
+	uint32_t run = 0;
+	uint32_t tag = 0;
+	void init(uint32_t tagValue);
+};
+
+struct GaussianNoiseAngularRaysNode : IRaysNodeSingleInput
+{
+	GaussianNoiseAngularRaysNode() { rng.init(1); }
+	void setParameters(float mean, float stDev, int axis);
+	std::string getName() const override { return "GaussianNoiseAngularRaysNode"; }
+	const Mat3x4f* rays() const override { return out.as<Mat3x4f>(); }
+	NoiseStream rng;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	float mean = 0, stDev = 0;
+	int axis = 3;
+	DeviceBuffer out;
+};
+
+struct RaytraceNode : IPointsNode
+{
+	void setParameters() {}
+	std::string getName() const override { return "RaytraceNode"; }
+	bool isDense() const override { return false; }
+	bool hasField(rgl_field_t f) const override { return fields.count(f) != 0; }
+	std::vector<rgl_field_t> fieldList() const override { return {fields.begin(), fields.end()}; }
+	int capacity() const override { return raysNode ? returnCount() * raysNode->rayCount() : 0; }
+	rgl_return_mode_t returnMode() const override { return mode; }
+	int returnCount() const override { return static_cast<int>(static_cast<uint32_t>(mode) >> 24); }
+	Mat3x4f lookAtOriginTransform() const override { return raysNode->cumulativeRayTransform().inverse(); }
+	const void* fieldData(rgl_field_t f) override;
+
+	void setVelocity(const Vec3f& lin, const Vec3f& ang)
+	{
+		linVel = lin;
+		angVel = ang;
+	}
+	void setDistortion(bool on) { distortion = on; }
+	void setNonHits(float nearD, float farD)
+	{
+		nearDist = nearD;
+		farDist = farD;
+	}
+	void setMask(const int8_t* m, int count);
+	void setDefaultIntensity(float v) { defaultIntensity = v; }
+	void setReturnMode(rgl_return_mode_t m)
+	{
+		if (m != RGL_RETURN_UNKNOWN) {
+			mode = m;
+		}
+	}
+	void setBeamDivergence(float h, float v)
+	{
+		hHalf = h / 2.0f;
+		vHalf = v / 2.0f;
+	}
+	// Traversal statistics of the last configuration (used by the benchmark for byte accounting).
+	void traversalStats(int32_t* nodes, int32_t* tris, int32_t* insts);
+	void bruteForce(float* t, int32_t* inst, int32_t* prim);
+	RaytraceParams buildParams(const SceneView& view) const;
+	bool overflowed();
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override;
+
+private:
+	std::set<rgl_field_t> findFieldsToCompute();
+
+	std::shared_ptr<IRaysNode> raysNode;
+	std::set<rgl_field_t> fields;
+	std::map<rgl_field_t, std::unique_ptr<DeviceBuffer>> data;
+	DeviceBuffer defaultRange;
+	DeviceBuffer maskBuf;
+	int maskCount = 0;
+	bool hasMask = false;
+	bool distortion = false;
+	Vec3f linVel{0, 0, 0}, angVel{0, 0, 0};
+	float nearDist = INFINITY, farDist = INFINITY;
+	float defaultIntensity = 0.0f;
+	rgl_return_mode_t mode = RGL_RETURN_FIRST;
+	float hHalf = 0.0f, vHalf = 0.0f;
+	PinnedBuffer overflowHost;
+};
+
+struct TransformPointsNode : IPointsNodeSingleInput
+{
+	void setParameters(const Mat3x4f& tf) { transform = tf; }
+	std::string getName() const override { return "TransformPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override { return {RGL_FIELD_XYZ_VEC3_F32}; }
+	Mat3x4f lookAtOriginTransform() const override { return transform.inverse() * input->lookAtOriginTransform(); }
+	const void* fieldData(rgl_field_t f) override;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	Mat3x4f transform = Mat3x4f::identity();
+	DeviceBuffer out;
+};
+
+struct GaussianNoiseAngularHitpointNode : IPointsNodeSingleInput
+{
+	GaussianNoiseAngularHitpointNode() { rng.init(2); }
+	void setParameters(float mean, float stDev, int axis);
+	std::string getName() const override { return "GaussianNoiseAngularHitpointNode"; }
+	std::vector<rgl_field_t> requiredFields() const override { return {RGL_FIELD_XYZ_VEC3_F32}; }
+	const void* fieldData(rgl_field_t f) override;
+	NoiseStream rng;
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override;
+
+private:
+	float mean = 0, stDev = 0;
+	int axis = 3;
+	bool withDistance = false;
+	DeviceBuffer outXyz, outDistance;
+};
+
+struct GaussianNoiseDistanceNode : IPointsNodeSingleInput
+{
+	GaussianNoiseDistanceNode() { rng.init(3); }
+	void setParameters(float mean, float base, float rise);
+	std::string getName() const override { return "GaussianNoiseDistanceNode"; }
+	std::vector<rgl_field_t> requiredFields() const override
+	{
+		return {RGL_FIELD_XYZ_VEC3_F32, RGL_FIELD_DISTANCE_F32};
+	}
+	const void* fieldData(rgl_field_t f) override;
+	NoiseStream rng;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	float mean = 0, base = 0, rise = 0;
+	DeviceBuffer outXyz, outDistance;
+};
+
+struct CompactByFieldPointsNode : IPointsNodeSingleInput
+{
+	void setParameters(rgl_field_t f) { by = f; }
+	std::string getName() const override { return "CompactByFieldPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override { return {by}; }
+	bool isDense() const override { return true; }
+	const int32_t* countDevice() const override { return count.as<int32_t>(); }
+	int hostCount() override;
+	const void* fieldData(rgl_field_t f) override;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	rgl_field_t by = RGL_FIELD_IS_HIT_I32;
+	std::map<rgl_field_t, std::unique_ptr<DeviceBuffer>> data;
+	DeviceBuffer count, tileCounts;
+	PinnedBuffer countHost;
+};
+
+struct YieldPointsNode : IPointsNodeSingleInput
+{
+	void setParameters(const std::vector<rgl_field_t>& f);
+	std::string getName() const override { return "YieldPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override { return fields; }
+	bool hasField(rgl_field_t f) const override;
+
+protected:
+	void enqueueImpl() override {}
+
+private:
+	std::vector<rgl_field_t> fields;
+};
+
+struct FormatPointsNode : IPointsNodeSingleInput
+{
+	void setParameters(const std::vector<rgl_field_t>& f);
+	std::string getName() const override { return "FormatPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override { return fields; }
+	bool hasField(rgl_field_t f) const override;
+	const void* fieldData(rgl_field_t f) override;
+	size_t fieldPointSize(rgl_field_t f) const override;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	std::vector<rgl_field_t> fields;
+	DeviceBuffer out;
+};
+
+struct FromArrayPointsNode : IPointsNode
+{
+	void setParameters(const void* points, int count, const std::vector<rgl_field_t>& f);
+	std::string getName() const override { return "FromArrayPointsNode"; }
+	bool isDense() const override { return data.count(RGL_FIELD_IS_HIT_I32) == 0; }
+	bool hasField(rgl_field_t f) const override { return data.count(f) != 0; }
+	std::vector<rgl_field_t> fieldList() const override;
+	int capacity() const override { return width; }
+	rgl_return_mode_t returnMode() const override { return RGL_RETURN_UNKNOWN; }
+	int returnCount() const override { return 1; }
+	const void* fieldData(rgl_field_t f) override;
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override {}
+
+private:
+	std::map<rgl_field_t, std::unique_ptr<DeviceBuffer>> data;
+	int width = 0;
+};
+
+// Registry of live API objects of one kind (handles are raw pointers; src/APIObject.hpp:34-104).
+template<typename T>
+struct Registry
+{
+	static std::map<const void*, std::shared_ptr<T>>& map()
+	{
+		static std::map<const void*, std::shared_ptr<T>> m;
+		return m;
+	}
+	static void add(const std::shared_ptr<T>& p) { map()[p.get()] = p; }
+	static bool contains(const void* p) { return map().count(p) != 0; }
+	static std::shared_ptr<T> get(const void* p, const char* kind)
+	{
+		auto it = map().find(p);
+		if (it == map().end()) {
+			char buf[64];
+			std::snprintf(buf, sizeof(buf), "%p", p);
+			throw InvalidAPIObject(std::string("RGL API Error: Object does not exist: ") + kind + " " + buf);
+		}
+		return it->second;
+	}
+	static void release(const void* p, const char* kind)
+	{
+		get(p, kind);
+		map().erase(p);
+	}
+};
+
+// Node registry key is the Node* of the most-derived object.
+void registerNode(const Node::Ptr& n);
+Node::Ptr lookupNode(const void* handle);
+bool nodeAlive(const void* handle);
+void releaseNode(const Node* n);
+std::vector<Node::Ptr> allNodes();
+
+template<typename T>
+std::shared_ptr<T> Node::exactlyOneInputOfType() const
+{
+	std::vector<std::shared_ptr<T>> typed;
+	std::string names;
+	for (auto& in : inputs) {
+		if (auto t = std::dynamic_pointer_cast<T>(in)) {
+			typed.push_back(t);
+		}
+		if (!names.empty()) {
+			names += ", ";
+		}
+		names += in->getName();
+	}
+	if (typed.size() != 1) {
+		throw InvalidPipeline("looked for " + std::string(T::kTypeName) + ", but found [" + names + "]");
+	}
+	return typed[0];
+}
+
+} // namespace rgl

@@ -1,0 +1,71 @@
+// Host-callable launch wrappers of the HIP kernels. Every wrapper only enqueues work on `stream`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device_types.hpp"
+
+namespace rgl {
+
+struct Aabb
+{
+	float lo[3];
+	float hi[3];
+};
+
+namespace gpu {
+
+// ---------------------------------------------------------------- BVH (bvh.hip)
+size_t lbvhScratchBytes(int n);
+// Builds an LBVH over n primitive boxes. nodes: max(n-1, 1) entries; leafPrim: n (primitive of each leaf in
+// leaf order); parentSlot: 2n entries; counters: max(n-1,1); height: max(n-1,1) (height[0] = tree depth).
+// When leafRefIsPrim, leaf references in nodes hold the primitive index (TLAS); otherwise the leaf index
+// (BLAS, whose triangles are gathered in leaf order).
+void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim, BvhNode* nodes, int32_t* leafPrim,
+               int32_t* parentSlot, int32_t* counters, int32_t* height, void* scratch);
+// Refits the boxes of an existing LBVH topology to new primitive boxes.
+void refitLbvh(hipStream_t stream, int n, const Aabb* boxes, const int32_t* leafPrim, const int32_t* parentSlot,
+               BvhNode* nodes, int32_t* counters, int32_t* height);
+void triangleBoxes(hipStream_t stream, int n, const Vec3f* v, const Vec3i* idx, Aabb* out);
+void gatherTris(hipStream_t stream, int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out);
+void instanceBoxes(hipStream_t stream, int n, const InstanceShade* shades, const InstanceRec* recs, Aabb* out);
+
+// ---------------------------------------------------------------- raytrace (raytrace.hip)
+void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, const RaytraceOutputs& out);
+// Per-ray traversal statistics (internal nodes visited, triangles tested, instances entered) on the same BVH and
+// with the same traversal order as raytrace(); used for algorithmic-byte accounting.
+void traversalStats(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, int32_t* nodesVisited,
+                    int32_t* trisTested, int32_t* instancesEntered);
+// Brute-force closest hit over every triangle of every instance (testing aid; same intersection routine).
+void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
+                       float* outT, int32_t* outInstance, int32_t* outPrim);
+void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
+
+// ---------------------------------------------------------------- point-cloud kernels (points.hip)
+// Point clouds are processed over `capacity` elements; `count` (device, may be nullptr) holds the live count
+// when it is only known on the device (after compaction).
+int compactTiles(int capacity);
+void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
+             int32_t* tileCounts, int32_t* countOut);
+void transformPoints(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* in, Vec3f* out,
+                     const Mat3x4f& tf);
+void noiseAngularRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, float mean, float stDev, int axis,
+                      const Mat3x4f& lookAt, uint64_t seed, uint32_t run, uint32_t tag);
+void noiseAngularHitpoint(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, Vec3f* outXyz,
+                          float* outDistance, float mean, float stDev, int axis, const Mat3x4f& lookAt, uint64_t seed,
+                          uint32_t run, uint32_t tag);
+void noiseDistance(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, const float* inDistance,
+                   Vec3f* outXyz, float* outDistance, float mean, float stDevBase, float stDevRise, const Mat3x4f& lookAt,
+                   uint64_t seed, uint32_t run, uint32_t tag);
+void noiseNormals(hipStream_t stream, int n, float* out, uint64_t seed, uint32_t run, uint32_t tag);
+// SoA -> AoS (format) and AoS -> SoA (from-array) over `capacity` records of `recordSize` bytes.
+void formatSoaToAos(hipStream_t stream, int capacity, const int32_t* count, int recordSize, const FieldCopyList& fields,
+                    char* out);
+void formatAosToSoa(hipStream_t stream, int n, int recordSize, const char* in, const FieldCopyList& fields);
+void fillInt(hipStream_t stream, int n, int32_t* p, int32_t value);
+
+} // namespace gpu
+} // namespace rgl

@@ -1,0 +1,195 @@
+// Small vector / affine-matrix algebra shared by host and device code.
+//
+// Semantics follow the reference's math (src/math/Mat3x4f.hpp:25-267, src/math/Vector.hpp:139-196):
+// row-major 3x4 affine matrices, rotationRad(x,y,z) composed in the DirectXMath order, TRS = T*R(deg)*S,
+// and a 4x4 cofactor inverse that returns all zeros for a singular matrix. The formulas are the standard
+// closed forms; they are restated here so that results match the reference bit-for-bit where the reference
+// tests compare exactly (axis-aligned transforms).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RGL_HD __host__ __device__ __forceinline__
+#else
+#define RGL_HD inline
+#endif
+
+namespace rgl {
+
+struct Vec2f
+{
+	float x, y;
+};
+
+struct Vec3i
+{
+	int32_t x, y, z;
+};
+
+struct Vec3f
+{
+	float x, y, z;
+	RGL_HD float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+	RGL_HD float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+RGL_HD Vec3f make3(float x, float y, float z) { return Vec3f{x, y, z}; }
+RGL_HD Vec3f operator+(Vec3f a, Vec3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+RGL_HD Vec3f operator-(Vec3f a, Vec3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+RGL_HD Vec3f operator*(Vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+RGL_HD Vec3f operator*(float s, Vec3f a) { return {a.x * s, a.y * s, a.z * s}; }
+RGL_HD Vec3f operator/(Vec3f a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+RGL_HD float dot(Vec3f a, Vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RGL_HD Vec3f cross(Vec3f a, Vec3f b)
+{
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+RGL_HD float length(Vec3f a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Component-wise division by the length, as src/math/Vector.hpp:152 does.
+RGL_HD Vec3f normalized(Vec3f a)
+{
+	float l = length(a);
+	return {a.x / l, a.y / l, a.z / l};
+}
+
+struct Mat3x4f
+{
+	float rc[3][4];
+
+	static RGL_HD Mat3x4f identity() { return {{{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}}}; }
+	static RGL_HD Mat3x4f zero() { return {{{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}}}; }
+	static RGL_HD Mat3x4f translation(float x, float y, float z) { return {{{1, 0, 0, x}, {0, 1, 0, y}, {0, 0, 1, z}}}; }
+	static RGL_HD Mat3x4f scale(float x, float y, float z) { return {{{x, 0, 0, 0}, {0, y, 0, 0}, {0, 0, z, 0}}}; }
+
+	// Euler composition Rz(z)*Ry(y)*Rx(x) written out (src/math/Mat3x4f.hpp:38-60).
+	static RGL_HD Mat3x4f rotationRad(float x, float y, float z)
+	{
+		float cr = cosf(x), sr = sinf(x);
+		float cp = cosf(y), sp = sinf(y);
+		float cy = cosf(z), sy = sinf(z);
+		Mat3x4f m = identity();
+		m.rc[0][0] = cp * cy;
+		m.rc[0][1] = sr * sp * cy - cr * sy;
+		m.rc[0][2] = sr * sy + cr * sp * cy;
+		m.rc[1][0] = cp * sy;
+		m.rc[1][1] = cr * cy + sr * sp * sy;
+		m.rc[1][2] = cr * sp * sy - sr * cy;
+		m.rc[2][0] = -sp;
+		m.rc[2][1] = sr * cp;
+		m.rc[2][2] = cr * cp;
+		return m;
+	}
+
+	// Rotation about one axis (src/math/Mat3x4f.hpp:64-70); axis uses rgl_axis_t values 1..3.
+	static RGL_HD Mat3x4f rotationAxisRad(int axis, float angle)
+	{
+		return rotationRad(axis == 1 ? angle : 0.0f, axis == 2 ? angle : 0.0f, axis == 3 ? angle : 0.0f);
+	}
+
+	static RGL_HD Mat3x4f rotationDeg(float x, float y, float z)
+	{
+		const float k = static_cast<float>(M_PI) / 180.0f;
+		return rotationRad(x * k, y * k, z * k);
+	}
+
+	RGL_HD Vec3f translationVec() const { return {rc[0][3], rc[1][3], rc[2][3]}; }
+
+	RGL_HD Mat3x4f rotation() const
+	{
+		return {{{rc[0][0], rc[0][1], rc[0][2], 0}, {rc[1][0], rc[1][1], rc[1][2], 0}, {rc[2][0], rc[2][1], rc[2][2], 0}}};
+	}
+
+	// Euler extraction for the Unity-style ZXY left-handed convention (src/math/Mat3x4f.hpp:146-175).
+	RGL_HD float toRotationXOrderZXYLeftHandRad() const
+	{
+		if (rc[1][2] < 1) {
+			if (rc[1][2] > -1) {
+				return asinf(rc[1][2]);
+			}
+			return -static_cast<float>(M_PI) / 2.0f;
+		}
+		return static_cast<float>(M_PI) / 2.0f;
+	}
+	RGL_HD float toRotationYOrderZXYLeftHandRad() const
+	{
+		if (rc[2][1] < 1 && rc[2][1] > -1) {
+			return atan2f(-rc[2][0], rc[2][2]);
+		}
+		return 0.0f;
+	}
+
+	// Affine inverse through the 4x4 cofactor expansion; zero matrix if singular (src/math/Mat3x4f.hpp:188-245).
+	RGL_HD Mat3x4f inverse() const
+	{
+		const float m00 = rc[0][0], m01 = rc[0][1], m02 = rc[0][2], m03 = rc[0][3];
+		const float m10 = rc[1][0], m11 = rc[1][1], m12 = rc[1][2], m13 = rc[1][3];
+		const float m20 = rc[2][0], m21 = rc[2][1], m22 = rc[2][2], m23 = rc[2][3];
+		// With the implicit last row (0,0,0,1) the 2x2 minors that involve row 3 collapse to single entries
+		// of row 2 (or to zero), so the 4x4 cofactor expansion reduces to the 3x3 adjugate below.
+		const float det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+		if (det == 0.0f) {
+			return zero();
+		}
+		const float id = 1.0f / det;
+		Mat3x4f r;
+		r.rc[0][0] = id * (m11 * m22 - m12 * m21);
+		r.rc[0][1] = id * -(m01 * m22 - m02 * m21);
+		r.rc[0][2] = id * (m01 * m12 - m02 * m11);
+		r.rc[1][0] = id * -(m10 * m22 - m12 * m20);
+		r.rc[1][1] = id * (m00 * m22 - m02 * m20);
+		r.rc[1][2] = id * -(m00 * m12 - m02 * m10);
+		r.rc[2][0] = id * (m10 * m21 - m11 * m20);
+		r.rc[2][1] = id * -(m00 * m21 - m01 * m20);
+		r.rc[2][2] = id * (m00 * m11 - m01 * m10);
+		// Translation column: -(R^-1 * t) expanded through the same cofactors.
+		const float A2312 = m12 * m23 - m13 * m22, A1312 = m11 * m23 - m13 * m21, A1212 = m11 * m22 - m12 * m21;
+		const float A0312 = m10 * m23 - m13 * m20, A0212 = m10 * m22 - m12 * m20, A0112 = m10 * m21 - m11 * m20;
+		r.rc[0][3] = id * -(m01 * A2312 - m02 * A1312 + m03 * A1212);
+		r.rc[1][3] = id * (m00 * A2312 - m02 * A0312 + m03 * A0212);
+		r.rc[2][3] = id * -(m00 * A1312 - m01 * A0312 + m03 * A0112);
+		return r;
+	}
+};
+
+RGL_HD Mat3x4f operator*(const Mat3x4f& a, const Mat3x4f& b)
+{
+	Mat3x4f r;
+	for (int y = 0; y < 3; ++y) {
+		for (int x = 0; x < 4; ++x) {
+			r.rc[y][x] = (a.rc[y][0] * b.rc[0][x]) + (a.rc[y][1] * b.rc[1][x]) + (a.rc[y][2] * b.rc[2][x]);
+		}
+		r.rc[y][3] += a.rc[y][3];
+	}
+	return r;
+}
+
+RGL_HD Vec3f operator*(const Mat3x4f& m, Vec3f v)
+{
+	return {(m.rc[0][0] * v.x) + (m.rc[0][1] * v.y) + (m.rc[0][2] * v.z) + m.rc[0][3],
+	        (m.rc[1][0] * v.x) + (m.rc[1][1] * v.y) + (m.rc[1][2] * v.z) + m.rc[1][3],
+	        (m.rc[2][0] * v.x) + (m.rc[2][1] * v.y) + (m.rc[2][2] * v.z) + m.rc[2][3]};
+}
+
+// Linear part only (direction transform).
+RGL_HD Vec3f mulDir(const Mat3x4f& m, Vec3f v)
+{
+	return {(m.rc[0][0] * v.x) + (m.rc[0][1] * v.y) + (m.rc[0][2] * v.z),
+	        (m.rc[1][0] * v.x) + (m.rc[1][1] * v.y) + (m.rc[1][2] * v.z),
+	        (m.rc[2][0] * v.x) + (m.rc[2][1] * v.y) + (m.rc[2][2] * v.z)};
+}
+
+// TRS = T * R(deg) * S (src/math/Mat3x4f.hpp:91-97).
+RGL_HD Mat3x4f TRS(Vec3f t, Vec3f rDeg, Vec3f s)
+{
+	return Mat3x4f::translation(t.x, t.y, t.z) * Mat3x4f::rotationDeg(rDeg.x, rDeg.y, rDeg.z) *
+	       Mat3x4f::scale(s.x, s.y, s.z);
+}
+
+static_assert(sizeof(Mat3x4f) == 48, "Mat3x4f must match rgl_mat3x4f");
+static_assert(sizeof(Vec3f) == 12, "Vec3f must match rgl_vec3f");
+
+} // namespace rgl

@@ -1,0 +1,681 @@
+// Ray generation, two-level BVH traversal, closest-hit shading and beam reduction, fused in one kernel.
+//
+// Per-ray semantics follow the reference's OptiX programs and beam reduction:
+//   raygen      src/gpu/optixPrograms.cu:46-87 (mask, distortion, ranges, 58-sample beams :75-86, :230-248)
+//   closesthit  src/gpu/optixPrograms.cu:96-216 (FP64 distance :122-129, min range :132-136, normal :139-145,
+//               incident angle :148-149, texture intensity :151-167, velocities :169-212)
+//   miss        src/gpu/optixPrograms.cu:89-94
+//   beam reduce src/gpu/nodeKernels.cu:334-422 with saveReturnAsHit / saveReturnAsNonHit :103-207
+//   shared data src/gpu/optixPrograms.cu:295-326
+// The reference stores 58 samples per beam in HBM and reduces them in a second kernel; here each thread keeps
+// the five return trackers in registers and shades only the samples that are returned, so no sample buffer
+// exists. Traversal replaces OptiX: closest-child-first stack traversal of a TLAS (instances) and per-mesh
+// BLASes, conservative slab tests, and the watertight ray/triangle test of Woop, Benthin and Wald (JCGT 2013).
+// Hits are accepted for t in (0, tmax]; equal distances are resolved by (instance index, triangle index) so the
+// result does not depend on traversal order.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "device_types.hpp"
+#include "kernels.hpp"
+
+namespace rgl::gpu {
+
+namespace {
+
+constexpr int kRtBlock = 128;
+constexpr int kLdsStack = 24;
+constexpr int kOvfStack = 104;
+constexpr int kSentinel = 0x7fffffff;
+constexpr int kPop = INT_MIN;
+constexpr int kBeamSamples = 58;  // 1 primary + 3 layers x 19 vertices (src/gpu/MultiReturn.hpp:19-21)
+constexpr int kBeamLayers = 3;
+constexpr int kBeamVertices = 19;
+// Slab-test widening: |rounding error| of (bound - origin) * invDir is below 3 ulp; widen by 2^-20.
+constexpr float kSlabGrow = 1.0f + 9.5367431640625e-7f;
+constexpr float kSlabShrink = 1.0f - 9.5367431640625e-7f;
+
+struct TravRay
+{
+	Vec3f o;
+	Vec3f inv;
+	float sx, sy, sz;
+	int kx, ky, kz;
+};
+
+__device__ __forceinline__ float sel3(const Vec3f& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ TravRay makeTravRay(Vec3f o, Vec3f d)
+{
+	TravRay r;
+	r.o = o;
+	const float tiny = 1e-30f;
+	r.inv.x = 1.0f / (d.x != 0.0f ? d.x : tiny);
+	r.inv.y = 1.0f / (d.y != 0.0f ? d.y : tiny);
+	r.inv.z = 1.0f / (d.z != 0.0f ? d.z : tiny);
+	const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+	int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+	int kx = kz == 2 ? 0 : kz + 1;
+	int ky = kx == 2 ? 0 : kx + 1;
+	if (sel3(d, kz) < 0.0f) {
+		const int t = kx;
+		kx = ky;
+		ky = t;
+	}
+	const float dz = sel3(d, kz);
+	r.sx = sel3(d, kx) / dz;
+	r.sy = sel3(d, ky) / dz;
+	r.sz = 1.0f / dz;
+	r.kx = kx;
+	r.ky = ky;
+	r.kz = kz;
+	return r;
+}
+
+struct HitRec
+{
+	float t;
+	int inst;
+	int prim;
+	int leaf;
+	float u, v;
+};
+
+// Watertight test; on success writes t, u (weight of B), v (weight of C).
+__device__ __forceinline__ bool intersectTri(const TravRay& r, const LeafTri& tri, float& tOut, float& uOut, float& vOut)
+{
+	const Vec3f A{tri.ax - r.o.x, tri.ay - r.o.y, tri.az - r.o.z};
+	const Vec3f B{tri.bx - r.o.x, tri.by - r.o.y, tri.bz - r.o.z};
+	const Vec3f C{tri.cx - r.o.x, tri.cy - r.o.y, tri.cz - r.o.z};
+	const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+	const float Ax = sel3(A, r.kx) - r.sx * Akz;
+	const float Ay = sel3(A, r.ky) - r.sy * Akz;
+	const float Bx = sel3(B, r.kx) - r.sx * Bkz;
+	const float By = sel3(B, r.ky) - r.sy * Bkz;
+	const float Cx = sel3(C, r.kx) - r.sx * Ckz;
+	const float Cy = sel3(C, r.ky) - r.sy * Ckz;
+	float U = Cx * By - Cy * Bx;
+	float V = Ax * Cy - Ay * Cx;
+	float W = Bx * Ay - By * Ax;
+	if (U == 0.0f || V == 0.0f || W == 0.0f) {
+		const double CxBy = static_cast<double>(Cx) * By, CyBx = static_cast<double>(Cy) * Bx;
+		const double AxCy = static_cast<double>(Ax) * Cy, AyCx = static_cast<double>(Ay) * Cx;
+		const double BxAy = static_cast<double>(Bx) * Ay, ByAx = static_cast<double>(By) * Ax;
+		U = static_cast<float>(CxBy - CyBx);
+		V = static_cast<float>(AxCy - AyCx);
+		W = static_cast<float>(BxAy - ByAx);
+	}
+	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) {
+		return false;
+	}
+	const float det = U + V + W;
+	if (det == 0.0f) {
+		return false;
+	}
+	const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+	const float T = U * Az + V * Bz + W * Cz;
+	tOut = T / det;
+	uOut = V / det;
+	vOut = W / det;
+	return true;
+}
+
+__device__ __forceinline__ bool better(float t, int inst, int prim, const HitRec& b)
+{
+	return t < b.t || (t == b.t && (inst < b.inst || (inst == b.inst && prim < b.prim)));
+}
+
+struct TravStats
+{
+	int nodes, tris, insts;
+};
+
+__device__ __forceinline__ void slab(const TravRay& r, const float* f, float tBest, bool& hit, float& tNear)
+{
+	const float tx0 = (f[0] - r.o.x) * r.inv.x, tx1 = (f[3] - r.o.x) * r.inv.x;
+	const float ty0 = (f[1] - r.o.y) * r.inv.y, ty1 = (f[4] - r.o.y) * r.inv.y;
+	const float tz0 = (f[2] - r.o.z) * r.inv.z, tz1 = (f[5] - r.o.z) * r.inv.z;
+	const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+	const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+	const float tnC = tn > 0.0f ? tn * kSlabShrink : 0.0f;
+	const float tfC = tf * kSlabGrow;
+	hit = tnC <= tfC && tnC <= tBest * kSlabGrow && tfC >= 0.0f;
+	tNear = tn;
+}
+
+// Closest hit of one ray against the two-level BVH; returns false if nothing is hit in (0, tmax].
+template<bool kStats>
+__device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
+{
+	// Short stack in LDS, [depth][thread] so a wave's pushes hit distinct banks; deeper entries spill to scratch.
+	__shared__ int32_t ldsStack[kLdsStack * kRtBlock];
+	best.t = tmax;
+	best.inst = INT_MAX;
+	best.prim = INT_MAX;
+	best.leaf = -1;
+	best.u = best.v = 0.0f;
+	if (sc.tlasNodes == nullptr || sc.instanceCount == 0 || !(tmax > 0.0f)) {
+		return false;
+	}
+	const int tid = threadIdx.x;
+	int32_t ovf[kOvfStack];
+	int sp = 0;
+	bool overflow = false;
+#define RGL_PUSH(v)                                                                                                    \
+	do {                                                                                                               \
+		if (sp < kLdsStack) {                                                                                          \
+			ldsStack[sp * kRtBlock + tid] = (v);                                                                       \
+			++sp;                                                                                                      \
+		} else if (sp < kLdsStack + kOvfStack) {                                                                       \
+			ovf[sp - kLdsStack] = (v);                                                                                 \
+			++sp;                                                                                                      \
+		} else {                                                                                                       \
+			overflow = true;                                                                                           \
+		}                                                                                                              \
+	} while (0)
+#define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kRtBlock + tid] : ovf[sp - kLdsStack])
+
+	const TravRay world = makeTravRay(o, d);
+	TravRay cur = world;
+	const BvhNode* nodes = sc.tlasNodes;
+	const LeafTri* tris = nullptr;
+	int inst = -1;
+	bool found = false;
+	int node = 0;
+	while (true) {
+		if (kStats) {
+			st->nodes++;
+		}
+		const float4* np = reinterpret_cast<const float4*>(nodes + node);
+		const float4 q0 = np[0], q1 = np[1], q2 = np[2];
+		const int4 q3 = reinterpret_cast<const int4*>(np)[3];
+		const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+		const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+		bool h0, h1;
+		float n0, n1;
+		slab(cur, b0, best.t, h0, n0);
+		slab(cur, b1, best.t, h1, n1);
+		int next;
+		if (h0 && h1) {
+			const bool firstNear = n0 <= n1;
+			RGL_PUSH(firstNear ? q3.y : q3.x);
+			next = firstNear ? q3.x : q3.y;
+		} else if (h0) {
+			next = q3.x;
+		} else if (h1) {
+			next = q3.y;
+		} else {
+			next = kPop;
+		}
+		while (true) {
+			if (next == kPop) {
+				if (sp == 0 || overflow) {
+					if (overflow) {
+						*sc.overflow = 1;
+					}
+					return found;
+				}
+				next = RGL_POP();
+				if (next == kSentinel) {
+					cur = world;
+					nodes = sc.tlasNodes;
+					inst = -1;
+					next = kPop;
+					continue;
+				}
+			}
+			if (next >= 0) {
+				break;
+			}
+			const int leaf = ~next;
+			if (inst < 0) {
+				inst = leaf;
+				const InstanceRec& ir = sc.instances[inst];
+				cur = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
+				nodes = ir.blasNodes;
+				tris = ir.blasTris;
+				if (kStats) {
+					st->insts++;
+				}
+				RGL_PUSH(kSentinel);
+				next = 0;
+				continue;
+			}
+			if (kStats) {
+				st->tris++;
+			}
+			const LeafTri tri = tris[leaf];
+			float t, u, v;
+			if (intersectTri(cur, tri, t, u, v) && t > 0.0f && t <= tmax && better(t, inst, tri.prim, best)) {
+				best.t = t;
+				best.inst = inst;
+				best.prim = tri.prim;
+				best.leaf = leaf;
+				best.u = u;
+				best.v = v;
+				found = true;
+			}
+			next = kPop;
+		}
+		node = next;
+	}
+}
+
+// Beam sample transform (src/gpu/optixPrograms.cu:230-248): rotation about X by the vertical angle times
+// rotation about Y by the horizontal angle, on a ring shrinking with the layer index.
+__device__ Mat3x4f beamSampleTransform(float hHalf, float vHalf, int layer, int vertex)
+{
+	const float shrink = 1.0f - static_cast<float>(layer) / kBeamLayers;
+	const float hCur = hHalf * shrink;
+	const float vCur = vHalf * shrink;
+	const float step = 2.0f * static_cast<float>(M_PI) / kBeamVertices;
+	const float hAngle = hCur * cosf(static_cast<float>(vertex) * step);
+	const float vAngle = vCur * sinf(static_cast<float>(vertex) * step);
+	return Mat3x4f::rotationRad(vAngle, 0.0f, 0.0f) * Mat3x4f::rotationRad(0.0f, hAngle, 0.0f);
+}
+
+// Ray origin = M*(0,0,0); direction = M*(0,0,1) - origin, evaluated as the reference does (not column 2 verbatim).
+__device__ __forceinline__ void rayOriginDir(const Mat3x4f& m, Vec3f& o, Vec3f& d)
+{
+	o = Vec3f{m.rc[0][3], m.rc[1][3], m.rc[2][3]};
+	d = Vec3f{__fsub_rn(__fadd_rn(m.rc[0][2], m.rc[0][3]), m.rc[0][3]),
+	          __fsub_rn(__fadd_rn(m.rc[1][2], m.rc[1][3]), m.rc[1][3]),
+	          __fsub_rn(__fadd_rn(m.rc[2][2], m.rc[2][3]), m.rc[2][3])};
+}
+
+struct Shade
+{
+	int isHit;
+	float distance;
+	float intensity;
+	Vec3f normal;
+	float incident;
+	Vec3f absVel, relVel;
+	float radial;
+	int entityId;
+	float retro;
+};
+
+// Closest-hit program for one sample (src/gpu/optixPrograms.cu:96-216). `full` also computes the fields that the
+// beam reduction does not need (normal, velocities).
+__device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float minRange, Vec3f so, Vec3f sd,
+                             const HitRec& h, bool full)
+{
+	Shade s;
+	const InstanceShade& is = sc.shades[h.inst];
+	const Vec3i tri = is.indices[h.prim];
+	const Vec3f A = is.vertices[tri.x], B = is.vertices[tri.y], C = is.vertices[tri.z];
+	// Barycentric point A + u(B-A) + v(C-A): exact when A, B, C share a coordinate (axis-aligned faces).
+	const Vec3f hitObject = A + h.u * (B - A) + h.v * (C - A);
+	const Vec3f hitWorld = is.objectToWorld * hitObject;
+	const double dx = static_cast<double>(hitWorld.x) - so.x;
+	const double dy = static_cast<double>(hitWorld.y) - so.y;
+	const double dz = static_cast<double>(hitWorld.z) - so.z;
+	const double distance = sqrt(dx * dx + dy * dy + dz * dz);
+	if (distance < minRange) {
+		s.isHit = 0;
+		s.distance = p.nearNonHitDistance;
+		s.intensity = 0.0f;
+		return s;
+	}
+	s.isHit = 1;
+	s.distance = static_cast<float>(distance);
+	const Vec3f wA = is.objectToWorld * A, wB = is.objectToWorld * B, wC = is.objectToWorld * C;
+	const Vec3f n = normalized(cross(wB - wA, wC - wA));
+	const float cosI = fabsf(dot(n, sd));
+	s.normal = n;
+	s.incident = acosf(cosI);
+	float intensity = p.defaultIntensity;
+	if (is.uvs != nullptr && is.texels != nullptr) {
+		const Vec2f ua = is.uvs[tri.x], ub = is.uvs[tri.y], uc = is.uvs[tri.z];
+		const float w = 1.0f - h.u - h.v;
+		const float tu = w * ua.x + h.u * ub.x + h.v * uc.x;
+		const float tv = w * ua.y + h.u * ub.y + h.v * uc.y;
+		// Point sampling, normalized coordinates, wrap addressing on both axes (src/scene/Texture.cpp:52-56).
+		const float fu = tu - floorf(tu), fv = tv - floorf(tv);
+		int ix = static_cast<int>(floorf(fu * is.texWidth));
+		int iy = static_cast<int>(floorf(fv * is.texHeight));
+		ix = min(max(ix, 0), is.texWidth - 1);
+		iy = min(max(iy, 0), is.texHeight - 1);
+		intensity = static_cast<float>(is.texels[static_cast<size_t>(iy) * is.texWidth + ix]);
+	}
+	s.intensity = intensity * cosI;
+	s.entityId = is.entityId;
+	s.retro = is.laserRetro;
+	const float qnan = __int_as_float(0x7fc00000);
+	s.absVel = Vec3f{qnan, qnan, qnan};
+	s.relVel = Vec3f{qnan, qnan, qnan};
+	s.radial = qnan;
+	if (full && p.velocityRequested && p.sceneDeltaTime > 0.0f) {
+		Vec3f disp{0.0f, 0.0f, 0.0f};
+		if (is.hasPrev) {
+			disp = hitWorld - is.prevObjectToWorld * hitObject;
+		}
+		const Vec3f absVel = disp / p.sceneDeltaTime;
+		const Mat3x4f rotInv = p.rayOriginToWorld.rotation().inverse();
+		const Vec3f absInSensor = rotInv * absVel;
+		const Vec3f relLin = absInSensor - p.sensorLinearVelocity;
+		const Vec3f hitRays = p.worldToRayOrigin * hitWorld;
+		const Vec3f relAng = Vec3f{0.0f, 0.0f, 0.0f} - cross(p.sensorAngularVelocity, hitRays);
+		s.absVel = absVel;
+		s.relVel = relLin + relAng;
+		s.radial = dot(normalized(hitRays), s.relVel);
+	}
+	return s;
+}
+
+__device__ void writeReturnHit(const RaytraceOutputs& out, const RaytraceParams& p, int beam, int rp, uint8_t type,
+                               const Shade& s)
+{
+	if (out.xyz) {
+		Vec3f o, d;
+		rayOriginDir(p.rays[beam], o, d);
+		out.xyz[rp] = o + normalized(d) * s.distance;
+	}
+	if (out.isHit) out.isHit[rp] = 1;
+	if (out.returnType) out.returnType[rp] = type;
+	if (out.distance) out.distance[rp] = s.distance;
+	if (out.intensityF32) out.intensityF32[rp] = s.intensity;
+	if (out.intensityU8) {
+		out.intensityU8[rp] = s.intensity < 255.0f ? static_cast<uint8_t>(roundf(s.intensity)) : static_cast<uint8_t>(255);
+	}
+	if (out.entityId) out.entityId[rp] = s.entityId;
+	if (out.absVelocity) out.absVelocity[rp] = s.absVel;
+	if (out.relVelocity) out.relVelocity[rp] = s.relVel;
+	if (out.radialSpeed) out.radialSpeed[rp] = s.radial;
+	if (out.normal) out.normal[rp] = s.normal;
+	if (out.incidentAngle) out.incidentAngle[rp] = s.incident;
+	if (out.laserRetro) out.laserRetro[rp] = s.retro;
+}
+
+__device__ void writeReturnNonHit(const RaytraceOutputs& out, const RaytraceParams& p, int beam, int rp, uint8_t type,
+                                  float d0)
+{
+	const float qnan = __int_as_float(0x7fc00000);
+	if (out.xyz) {
+		Vec3f o, d;
+		rayOriginDir(p.rays[beam], o, d);
+		Vec3f disp = normalized(d) * d0;
+		disp.x = isnan(disp.x) ? 0.0f : disp.x;
+		disp.y = isnan(disp.y) ? 0.0f : disp.y;
+		disp.z = isnan(disp.z) ? 0.0f : disp.z;
+		out.xyz[rp] = o + disp;
+	}
+	if (out.isHit) out.isHit[rp] = 0;
+	if (out.returnType) out.returnType[rp] = type;
+	if (out.distance) out.distance[rp] = d0;
+	if (out.intensityF32) out.intensityF32[rp] = 0.0f;
+	if (out.intensityU8) out.intensityU8[rp] = 0;
+	if (out.entityId) out.entityId[rp] = 0;
+	if (out.absVelocity) out.absVelocity[rp] = Vec3f{qnan, qnan, qnan};
+	if (out.relVelocity) out.relVelocity[rp] = Vec3f{qnan, qnan, qnan};
+	if (out.radialSpeed) out.radialSpeed[rp] = 0.0f;
+	if (out.normal) out.normal[rp] = Vec3f{qnan, qnan, qnan};
+	if (out.incidentAngle) out.incidentAngle[rp] = qnan;
+	if (out.laserRetro) out.laserRetro[rp] = 0.0f;
+}
+
+__device__ __forceinline__ uint8_t returnTypeOf(int mode, int r) { return static_cast<uint8_t>((mode >> (8 * r)) & 0xff); }
+
+struct Tracker
+{
+	int sample;
+	float distance;
+	float intensity;
+	HitRec hit;
+};
+
+__global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+{
+	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
+	if (beam >= p.rayCount) {
+		return;
+	}
+	const Mat3x4f rayWorld = p.rays[beam];
+	const int R = p.returnCount;
+
+	// Beam-shared fields (src/gpu/optixPrograms.cu:295-326).
+	if (out.rayIdx || out.azimuth || out.elevation || out.timeStampF64 || out.timeStampU32 || out.ringId) {
+		const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
+		for (int r = 0; r < R; ++r) {
+			const int rp = beam * R + r;
+			if (out.rayIdx) out.rayIdx[rp] = static_cast<uint32_t>(beam);
+			if (out.azimuth) out.azimuth[rp] = rayLocal.toRotationYOrderZXYLeftHandRad();
+			if (out.elevation) out.elevation[rp] = rayLocal.toRotationXOrderZXYLeftHandRad();
+			if (out.timeStampF64) {
+				out.timeStampF64[rp] = p.doDistortion ? static_cast<double>(p.timeOffsetsMs[beam] * 1e-3f) : 0.0;
+			}
+			if (out.timeStampU32) {
+				out.timeStampU32[rp] = p.doDistortion ? static_cast<uint32_t>(p.timeOffsetsMs[beam] * 1e6f) : 0u;
+			}
+			if (out.ringId && p.ringIds) out.ringId[rp] = static_cast<uint16_t>(p.ringIds[beam % p.ringIdsCount]);
+		}
+	}
+
+	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
+	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
+	if (masked) {
+		for (int r = 0; r < R; ++r) {
+			writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
+		}
+		return;
+	}
+
+	Mat3x4f ray = rayWorld;
+	if (p.doDistortion) {
+		const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
+		const float toDeg = static_cast<float>(180.0 / M_PI);
+		const float off = p.timeOffsetsMs[beam];
+		const Vec3f t = (off * p.sensorLinearVelocity) * 0.001f;
+		const Vec3f rdeg = (off * (p.sensorAngularVelocity * toDeg)) * 0.001f;
+		ray = p.rayOriginToWorld * (TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f}) * rayLocal);
+	}
+
+	const bool divergent = p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f;
+	const int samples = divergent ? kBeamSamples : 1;
+	const bool needVel = p.velocityRequested != 0;
+
+	Tracker first, second, last, strongest, secondStrongest;
+	first.sample = -1;
+	float sample0Distance = p.farNonHitDistance;
+	Shade shade0;  // single-sample beams keep their fully shaded sample instead of re-shading it
+	for (int s = 0; s < samples; ++s) {
+		Mat3x4f sr = ray;
+		if (s > 0) {
+			const int layer = (s - 1) / kBeamVertices;
+			const int vertex = (s - 1) % kBeamVertices;
+			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
+		}
+		Vec3f so, sd;
+		rayOriginDir(sr, so, sd);
+		HitRec h;
+		const bool any = traceClosest<false>(sc, so, sd, range.y, h, nullptr);
+		Tracker cand;
+		cand.sample = s;
+		cand.hit = h;
+		int isHit = 0;
+		if (any) {
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, samples == 1 && needVel);
+			isHit = sh.isHit;
+			cand.distance = sh.distance;
+			cand.intensity = sh.intensity;
+			if (samples == 1) {
+				shade0 = sh;
+			}
+		} else {
+			cand.distance = p.farNonHitDistance;
+			cand.intensity = 0.0f;
+		}
+		if (s == 0) {
+			sample0Distance = cand.distance;
+		}
+		if (!isHit) {
+			continue;
+		}
+		if (first.sample < 0) {
+			first = second = last = strongest = secondStrongest = cand;
+			continue;
+		}
+		const float d = cand.distance;
+		if (d < first.distance) {
+			second = first;
+			first = cand;
+		} else if (first.distance < d && d < second.distance) {
+			second = cand;
+		}
+		if (d > last.distance) {
+			last = cand;
+		}
+		const float I = cand.intensity;
+		if (I > strongest.intensity) {
+			secondStrongest = strongest;
+			strongest = cand;
+		} else if (strongest.intensity > I && I > secondStrongest.intensity) {
+			secondStrongest = cand;
+		}
+	}
+
+	for (int r = 0; r < R; ++r) {
+		const uint8_t type = returnTypeOf(p.returnMode, r);
+		const int rp = beam * R + r;
+		if (first.sample < 0) {
+			writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
+			continue;
+		}
+		const Tracker* t = nullptr;
+		switch (type) {
+			case 4: t = &first; break;           // RGL_RETURN_TYPE_FIRST
+			case 3: t = &second; break;          // RGL_RETURN_TYPE_SECOND
+			case 2: t = &last; break;            // RGL_RETURN_TYPE_LAST
+			case 1: t = &strongest; break;       // RGL_RETURN_TYPE_STRONGEST
+			case 5: t = &secondStrongest; break; // RGL_RETURN_TYPE_SECOND_STRONGEST
+			default: break;
+		}
+		if (t == nullptr) {
+			continue;
+		}
+		if (samples == 1) {
+			writeReturnHit(out, p, beam, rp, type, shade0);
+			continue;
+		}
+		Mat3x4f sr = ray;
+		if (t->sample > 0) {
+			const int layer = (t->sample - 1) / kBeamVertices;
+			const int vertex = (t->sample - 1) % kBeamVertices;
+			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
+		}
+		Vec3f so, sd;
+		rayOriginDir(sr, so, sd);
+		const Shade sh = shadeSample(sc, p, range.x, so, sd, t->hit, needVel);
+		writeReturnHit(out, p, beam, rp, type, sh);
+	}
+}
+
+__global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, RaytraceParams p, int32_t* nodesVisited,
+                                                             int32_t* trisTested, int32_t* instancesEntered)
+{
+	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
+	if (beam >= p.rayCount) {
+		return;
+	}
+	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
+	Vec3f so, sd;
+	rayOriginDir(p.rays[beam], so, sd);
+	HitRec h;
+	TravStats st{0, 0, 0};
+	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
+	if (!masked) {
+		traceClosest<true>(sc, so, sd, range.y, h, &st);
+	}
+	nodesVisited[beam] = st.nodes;
+	trisTested[beam] = st.tris;
+	instancesEntered[beam] = st.insts;
+}
+
+__global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytraceParams p, float* outT, int32_t* outInst,
+                            int32_t* outPrim)
+{
+	const int beam = blockIdx.x * blockDim.x + threadIdx.x;
+	if (beam >= p.rayCount) {
+		return;
+	}
+	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
+	Vec3f o, d;
+	rayOriginDir(p.rays[beam], o, d);
+	HitRec best;
+	best.t = range.y;
+	best.inst = INT_MAX;
+	best.prim = INT_MAX;
+	bool found = false;
+	for (int i = 0; i < sc.instanceCount; ++i) {
+		const InstanceRec& ir = sc.instances[i];
+		const TravRay r = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
+		for (int k = 0; k < triCounts[i]; ++k) {
+			const LeafTri tri = ir.blasTris[k];
+			float t, u, v;
+			if (intersectTri(r, tri, t, u, v) && t > 0.0f && t <= range.y && better(t, i, tri.prim, best)) {
+				best.t = t;
+				best.inst = i;
+				best.prim = tri.prim;
+				found = true;
+			}
+		}
+	}
+	outT[beam] = found ? best.t : -1.0f;
+	outInst[beam] = found ? best.inst : -1;
+	outPrim[beam] = found ? best.prim : -1;
+}
+
+__global__ void kTransformRays(int n, const Mat3x4f* in, Mat3x4f* out, Mat3x4f tf)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) {
+		out[i] = tf * in[i];
+	}
+}
+
+} // namespace
+
+void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, const RaytraceOutputs& out)
+{
+	if (params.rayCount <= 0) {
+		return;
+	}
+	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
+	hipLaunchKernelGGL(kRaytrace, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, out);
+}
+
+void traversalStats(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, int32_t* nodesVisited,
+                    int32_t* trisTested, int32_t* instancesEntered)
+{
+	if (params.rayCount <= 0) {
+		return;
+	}
+	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
+	hipLaunchKernelGGL(kTraversalStats, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, nodesVisited, trisTested,
+	                   instancesEntered);
+}
+
+void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
+                       float* outT, int32_t* outInstance, int32_t* outPrim)
+{
+	if (params.rayCount <= 0) {
+		return;
+	}
+	hipLaunchKernelGGL(kBruteForce, dim3((params.rayCount + 127) / 128), dim3(128), 0, stream, scene, triCounts, params,
+	                   outT, outInstance, outPrim);
+}
+
+void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kTransformRays, dim3((n + 255) / 256), dim3(256), 0, stream, n, in, out, tf);
+	}
+}
+
+int maxTraversalStack() { return kLdsStack + kOvfStack; }
+
+} // namespace rgl::gpu

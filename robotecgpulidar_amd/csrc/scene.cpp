@@ -1,0 +1,287 @@
+#include "scene.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace rgl {
+
+namespace {
+
+// FP64 inverse of an affine 3x4 matrix, rounded to FP32 once. Used for world->object instance transforms so
+// that large world coordinates do not accumulate FP32 error in the inverse.
+Mat3x4f inverseFp64(const Mat3x4f& m)
+{
+	double a[3][4];
+	for (int r = 0; r < 3; ++r) {
+		for (int c = 0; c < 4; ++c) {
+			a[r][c] = m.rc[r][c];
+		}
+	}
+	const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+	const double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+	const double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+	const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+	if (det == 0.0) {
+		return Mat3x4f::zero();
+	}
+	const double id = 1.0 / det;
+	double r[3][3];
+	r[0][0] = c00 * id;
+	r[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+	r[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+	r[1][0] = c01 * id;
+	r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+	r[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+	r[2][0] = c02 * id;
+	r[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+	r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+	Mat3x4f out;
+	for (int i = 0; i < 3; ++i) {
+		double t = 0.0;
+		for (int k = 0; k < 3; ++k) {
+			out.rc[i][k] = static_cast<float>(r[i][k]);
+			t -= r[i][k] * a[k][3];
+		}
+		out.rc[i][3] = static_cast<float>(t);
+	}
+	return out;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------------------------------ Mesh
+Mesh::Mesh(const Vec3f* v, int vc, const Vec3i* idx, int ic) : vertexCount(vc), triCount(ic)
+{
+	for (int i = 0; i < ic; ++i) {
+		const Vec3i t = idx[i];
+		if (t.x < 0 || t.y < 0 || t.z < 0 || t.x >= vc || t.y >= vc || t.z >= vc) {
+			throw std::invalid_argument("mesh index out of range of the vertex array");
+		}
+	}
+	RGL_HIP(hipMemcpy(vertices.ensure(sizeof(Vec3f) * vc), v, sizeof(Vec3f) * vc, hipMemcpyHostToDevice));
+	RGL_HIP(hipMemcpy(indices.ensure(sizeof(Vec3i) * ic), idx, sizeof(Vec3i) * ic, hipMemcpyHostToDevice));
+}
+
+void Mesh::setTexCoords(const Vec2f* uv, int count)
+{
+	if (count != vertexCount) {
+		throw std::invalid_argument("Cannot set texture coordinates because vertex count do not match with texture "
+		                            "coordinates count: vertexCount=" +
+		                            std::to_string(vertexCount) + ", textureCoords=" + std::to_string(count));
+	}
+	RGL_HIP(hipMemcpy(uvs.ensure(sizeof(Vec2f) * count), uv, sizeof(Vec2f) * count, hipMemcpyHostToDevice));
+	hasUVs = true;
+}
+
+void Mesh::ensureBlas(hipStream_t s)
+{
+	if (blasBuilt) {
+		return;
+	}
+	const int n = triCount;
+	const int inner = std::max(n - 1, 1);
+	boxes.ensure(sizeof(Aabb) * n);
+	nodes.ensure(sizeof(BvhNode) * inner);
+	tris.ensure(sizeof(LeafTri) * n);
+	leafPrim.ensure(sizeof(int32_t) * n);
+	parentSlot.ensure(sizeof(int32_t) * 2 * n);
+	counters.ensure(sizeof(int32_t) * inner);
+	height.ensure(sizeof(int32_t) * inner);
+	scratch.ensure(gpu::lbvhScratchBytes(n));
+	gpu::triangleBoxes(s, n, vertices.as<Vec3f>(), indices.as<Vec3i>(), boxes.as<Aabb>());
+	gpu::buildLbvh(s, n, boxes.as<Aabb>(), false, nodes.as<BvhNode>(), leafPrim.as<int32_t>(), parentSlot.as<int32_t>(),
+	               counters.as<int32_t>(), height.as<int32_t>(), scratch.get());
+	gpu::gatherTris(s, n, leafPrim.as<int32_t>(), vertices.as<Vec3f>(), indices.as<Vec3i>(), tris.as<LeafTri>());
+	int32_t depth = 0;
+	RGL_HIP(hipMemcpyAsync(&depth, height.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+	RGL_HIP(hipStreamSynchronize(s));
+	RGL_HIP(hipGetLastError());
+	blasDepth = depth;
+	scratch.release();
+	blasBuilt = true;
+}
+
+// ------------------------------------------------------------------------------------------------ Texture
+Texture::Texture(const void* texels_, int w, int h) : width(w), height(h)
+{
+	const size_t bytes = static_cast<size_t>(w) * h;
+	RGL_HIP(hipMemcpy(texels.ensure(bytes), texels_, bytes, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------------------------------------ Entity
+void Entity::setTransform(const Mat3x4f& tf)
+{
+	formerTransform = transform;
+	formerTransformTime = transformTime;
+	formerTransformValid = true;
+	transform = tf;
+	transformTime = Scene::instance().getTime();
+	Scene::instance().markDirty(false);
+}
+
+// src/scene/Entity.cpp:66-86
+std::optional<Mat3x4f> Entity::previousFrameTransform() const
+{
+	const bool setOnlyOnce = transformTime.has_value() && !formerTransformTime.has_value();
+	if (setOnlyOnce) {
+		return transform;
+	}
+	const bool setInPrecedingFrame = formerTransformTime.has_value() &&
+	                                 formerTransformTime == Scene::instance().getPrevTime();
+	if (!setInPrecedingFrame) {
+		return std::nullopt;
+	}
+	return formerTransform;
+}
+
+// ------------------------------------------------------------------------------------------------ Scene
+Scene& Scene::instance()
+{
+	static Scene scene;
+	return scene;
+}
+
+Scene::Scene() {}
+
+void Scene::addEntity(const std::shared_ptr<Entity>& e)
+{
+	e->order = nextOrder++;
+	entities.push_back(e);
+	markDirty(true);
+}
+
+void Scene::removeEntity(const std::shared_ptr<Entity>& e)
+{
+	entities.erase(std::remove(entities.begin(), entities.end(), e), entities.end());
+	markDirty(true);
+}
+
+void Scene::clear()
+{
+	entities.clear();
+	time.reset();
+	prevTime.reset();
+	markDirty(true);
+}
+
+void Scene::markDirty(bool topologyChanged)
+{
+	dirty = true;
+	topologyDirty = topologyDirty || topologyChanged;
+}
+
+float Scene::deltaTimeSeconds() const
+{
+	if (!prevTime.has_value() || !time.has_value()) {
+		return 0.0f;
+	}
+	const double dt = (static_cast<double>(*time) - static_cast<double>(*prevTime)) * 1e-9;
+	return static_cast<float>(dt);
+}
+
+int32_t* Scene::overflowFlag()
+{
+	if (dOverflow.get() == nullptr) {
+		dOverflow.ensure(sizeof(int32_t));
+		RGL_HIP(hipMemset(dOverflow.get(), 0, sizeof(int32_t)));
+	}
+	return dOverflow.as<int32_t>();
+}
+
+std::vector<int32_t> Scene::instanceTriCounts() const
+{
+	std::vector<int32_t> c;
+	for (auto& e : entities) {
+		c.push_back(e->mesh->triCount);
+	}
+	return c;
+}
+
+void Scene::rebuild(hipStream_t s)
+{
+	// The previous rebuild's uploads read the pinned staging arrays reused below.
+	RGL_HIP(hipStreamSynchronize(s));
+	const int n = static_cast<int>(entities.size());
+	maxBlasDepth = 0;
+	for (auto& e : entities) {
+		e->mesh->ensureBlas(s);
+		maxBlasDepth = std::max(maxBlasDepth, e->mesh->blasDepth);
+	}
+	if (n == 0) {
+		view = SceneView{};
+		view.overflow = overflowFlag();
+		return;
+	}
+	auto* inst = static_cast<InstanceRec*>(hostInst.ensure(sizeof(InstanceRec) * n));
+	auto* shade = static_cast<InstanceShade*>(hostShade.ensure(sizeof(InstanceShade) * n));
+	for (int i = 0; i < n; ++i) {
+		const Entity& e = *entities[i];
+		inst[i].worldToObject = inverseFp64(e.transform);
+		inst[i].blasNodes = e.mesh->nodes.as<BvhNode>();
+		inst[i].blasTris = e.mesh->tris.as<LeafTri>();
+		InstanceShade& sh = shade[i];
+		std::memset(&sh, 0, sizeof(sh));
+		sh.objectToWorld = e.transform;
+		const auto prev = e.previousFrameTransform();
+		sh.prevObjectToWorld = prev.value_or(Mat3x4f::identity());
+		sh.hasPrev = prev.has_value() ? 1 : 0;
+		sh.vertices = e.mesh->vertices.as<Vec3f>();
+		sh.indices = e.mesh->indices.as<Vec3i>();
+		sh.uvs = e.mesh->hasUVs ? e.mesh->uvs.as<Vec2f>() : nullptr;
+		sh.texels = e.texture ? e.texture->texels.as<uint8_t>() : nullptr;
+		sh.texWidth = e.texture ? e.texture->width : 0;
+		sh.texHeight = e.texture ? e.texture->height : 0;
+		sh.entityId = e.id;
+		sh.laserRetro = e.laserRetro;
+	}
+	const bool refit = !topologyDirty && tlasUpdate == TlasUpdate::Refit && tlasNodes.get() != nullptr;
+	RGL_HIP(hipMemcpyAsync(dInst.ensure(sizeof(InstanceRec) * n), inst, sizeof(InstanceRec) * n, hipMemcpyHostToDevice, s));
+	RGL_HIP(hipMemcpyAsync(dShade.ensure(sizeof(InstanceShade) * n), shade, sizeof(InstanceShade) * n,
+	                       hipMemcpyHostToDevice, s));
+	const int inner = std::max(n - 1, 1);
+	dBoxes.ensure(sizeof(Aabb) * n);
+	gpu::instanceBoxes(s, n, dShade.as<InstanceShade>(), dInst.as<InstanceRec>(), dBoxes.as<Aabb>());
+	if (refit) {
+		gpu::refitLbvh(s, n, dBoxes.as<Aabb>(), tlasLeafPrim.as<int32_t>(), tlasParent.as<int32_t>(),
+		               tlasNodes.as<BvhNode>(), tlasCounters.as<int32_t>(), tlasHeight.as<int32_t>());
+	} else {
+		tlasNodes.ensure(sizeof(BvhNode) * inner);
+		tlasLeafPrim.ensure(sizeof(int32_t) * n);
+		tlasParent.ensure(sizeof(int32_t) * 2 * n);
+		tlasCounters.ensure(sizeof(int32_t) * inner);
+		tlasHeight.ensure(sizeof(int32_t) * inner);
+		tlasScratch.ensure(gpu::lbvhScratchBytes(n));
+		gpu::buildLbvh(s, n, dBoxes.as<Aabb>(), true, tlasNodes.as<BvhNode>(), tlasLeafPrim.as<int32_t>(),
+		               tlasParent.as<int32_t>(), tlasCounters.as<int32_t>(), tlasHeight.as<int32_t>(), tlasScratch.get());
+	}
+	RGL_HIP(hipGetLastError());
+	view.tlasNodes = tlasNodes.as<BvhNode>();
+	view.instances = dInst.as<InstanceRec>();
+	view.shades = dShade.as<InstanceShade>();
+	view.instanceCount = n;
+	view.overflow = overflowFlag();
+}
+
+SceneView Scene::prepare(hipStream_t graphStream)
+{
+	if (!stream) {
+		stream = std::make_unique<Stream>();
+		ready = std::make_unique<Event>();
+	}
+	if (dirty) {
+		rebuild(stream->get());
+		ready->record(stream->get());
+		dirty = false;
+		topologyDirty = false;
+	}
+	if (ready->wasRecorded()) {
+		RGL_HIP(hipStreamWaitEvent(graphStream, ready->get(), 0));
+	}
+	if (view.overflow == nullptr) {
+		view.overflow = overflowFlag();
+	}
+	return view;
+}
+
+} // namespace rgl

@@ -1,0 +1,124 @@
+// Scene model: meshes (with their BLAS), textures, entities (instances) and the default scene (TLAS).
+//
+// Mirrors the reference's object model (src/scene/Mesh.cpp:20-41, src/scene/Entity.cpp:26-114,
+// src/scene/Texture.cpp:17-76, src/scene/Scene.cpp:41-233). Acceleration structures are this library's own
+// LBVHs: one BLAS per mesh, shared by every entity of that mesh, and a TLAS over entity world boxes that is
+// rebuilt (or refitted, when only transforms moved) on the scene stream when the scene changed.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <optional>
+#include <vector>
+
+#include "device_types.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rgl {
+
+struct Mesh
+{
+	Mesh(const Vec3f* vertices, int vertexCount, const Vec3i* indices, int indexCount);
+	void setTexCoords(const Vec2f* uvs, int count);
+	// Builds the BLAS on `stream` if needed (first use); waits for it to finish.
+	void ensureBlas(hipStream_t stream);
+
+	int vertexCount = 0;
+	int triCount = 0;
+	bool hasUVs = false;
+	DeviceBuffer vertices, indices, uvs;
+	// BLAS
+	bool blasBuilt = false;
+	int blasDepth = 0;
+	DeviceBuffer nodes, tris, leafPrim, parentSlot, counters, height, boxes, scratch;
+};
+
+struct Texture
+{
+	Texture(const void* texels, int width, int height);
+	int width = 0, height = 0;
+	DeviceBuffer texels;
+};
+
+struct Time
+{
+	uint64_t ns = 0;
+};
+
+struct Entity
+{
+	explicit Entity(std::shared_ptr<Mesh> m) : mesh(std::move(m)) {}
+	void setTransform(const Mat3x4f& tf);
+	std::optional<Mat3x4f> previousFrameTransform() const;
+
+	std::shared_ptr<Mesh> mesh;
+	std::shared_ptr<Texture> texture;
+	Mat3x4f transform = Mat3x4f::identity();
+	std::optional<uint64_t> transformTime;       // scene time when the transform was set
+	Mat3x4f formerTransform = Mat3x4f::identity();
+	std::optional<uint64_t> formerTransformTime;
+	bool formerTransformValid = false;
+	int32_t id = RGL_DEFAULT_ENTITY_ID;
+	float laserRetro = 0.0f;
+	uint64_t order = 0;                          // creation order: instance index order in the TLAS
+};
+
+// How the TLAS follows entity motion when no entity was added or removed.
+enum class TlasUpdate
+{
+	Rebuild,
+	Refit,
+};
+
+class Scene
+{
+public:
+	static Scene& instance();
+
+	void addEntity(const std::shared_ptr<Entity>& e);
+	void removeEntity(const std::shared_ptr<Entity>& e);
+	void clear();
+	void markDirty(bool topologyChanged);
+
+	void setTime(uint64_t ns)
+	{
+		prevTime = time;
+		time = ns;
+	}
+	std::optional<uint64_t> getTime() const { return time; }
+	std::optional<uint64_t> getPrevTime() const { return prevTime; }
+	// Seconds since the previous scene time, 0 if unknown.
+	float deltaTimeSeconds() const;
+
+	// Builds whatever is stale on the scene stream and makes `graphStream` wait for it.
+	SceneView prepare(hipStream_t graphStream);
+	// Traversal-stack overflow flag written by the raytrace kernel (never expected to be set).
+	int32_t* overflowFlag();
+	size_t entityCount() const { return entities.size(); }
+	const std::vector<std::shared_ptr<Entity>>& getEntities() const { return entities; }
+	std::vector<int32_t> instanceTriCounts() const;
+
+	TlasUpdate tlasUpdate = TlasUpdate::Rebuild;
+	int lastTlasDepth = 0;
+
+private:
+	Scene();
+	void rebuild(hipStream_t s);
+
+	std::vector<std::shared_ptr<Entity>> entities;
+	std::optional<uint64_t> time, prevTime;
+	bool dirty = true;
+	bool topologyDirty = true;
+	int maxBlasDepth = 0;
+	std::unique_ptr<Stream> stream;
+	std::unique_ptr<Event> ready;
+	PinnedBuffer hostInst, hostShade;
+	DeviceBuffer dInst, dShade, dBoxes, tlasNodes, tlasLeafPrim, tlasParent, tlasCounters, tlasHeight, tlasScratch;
+	DeviceBuffer dOverflow;
+	SceneView view{};
+	uint64_t nextOrder = 0;
+	friend struct Entity;
+};
+
+} // namespace rgl
