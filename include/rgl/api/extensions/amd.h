@@ -29,5 +29,11 @@ RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t*
 RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t, int32_t* out_instance,
                                                    int32_t* out_prim, int32_t ray_count);
 
+// Enables (and resets) HIP-event timing of a raytrace node's kernel launches on its graph stream.
+RGL_API rgl_status_t rgl_node_raytrace_set_kernel_timing(rgl_node_t node, bool enable);
+
+// Total device time (ms) and number of raytrace kernel launches timed since timing was enabled; waits for them.
+RGL_API rgl_status_t rgl_node_raytrace_get_kernel_timing(rgl_node_t node, double* out_total_ms, int32_t* out_launches);
+
 // Standard normal variates normal(seed, run, tag, i) for i in [0, count), computed on the GPU.
 RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);

@@ -1,0 +1,112 @@
+/* FP64 CPU oracle of RGL's raytrace-and-point-cloud semantics -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and only as the
+ * checker (or the timed CPU baseline). The product path never calls it.
+ */
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct
+{
+	int nMeshes;
+	const int32_t* meshVertOffset; /* nMeshes + 1 prefix offsets into vertices (in vertices) */
+	const int32_t* meshTriOffset;  /* nMeshes + 1 prefix offsets into indices (in triangles) */
+	const float* vertices;         /* xyz per vertex, all meshes concatenated */
+	const int32_t* indices;        /* 3 mesh-local vertex indices per triangle */
+	const float* uvs;              /* uv per vertex (same indexing as vertices) or NULL */
+	const int32_t* meshHasUV;      /* per mesh */
+	int nInstances;
+	const int32_t* instMesh;
+	const float* instTransform;     /* 12 floats (row-major 3x4) per instance */
+	const float* instPrevTransform; /* 12 floats per instance (used when instHasPrev) */
+	const int32_t* instHasPrev;
+	const int32_t* instId;
+	const float* instRetro;
+	const int32_t* instTexture; /* texture index or -1 */
+	int nTextures;
+	const int32_t* texWidth;
+	const int32_t* texHeight;
+	const int64_t* texOffset; /* byte offset of each texture in texels */
+	const uint8_t* texels;
+} OrcScene;
+
+typedef struct
+{
+	int nRays;
+	const float* rays;          /* 12 floats per ray: world-frame rays after the rays chain */
+	float rayOriginToWorld[12]; /* cumulative transform of the rays chain */
+	const float* ranges;        /* 2 floats per entry */
+	int nRanges;
+	const int32_t* ringIds;
+	int nRingIds;
+	const float* timeOffsetsMs;
+	const int8_t* mask;
+	int distortion;
+	float linVel[3];
+	float angVel[3];
+	float nearDistance;
+	float farDistance;
+	float defaultIntensity;
+	float sceneDeltaTime;
+	float hDivergence; /* full angles, radians (the API's values) */
+	float vDivergence;
+	int32_t returnMode;
+	int bruteForce;
+	int threads;
+	double ambiguityEps; /* relative tolerance of the grazing / tie diagnostic */
+} OrcRequest;
+
+typedef struct
+{
+	float* xyz;
+	int32_t* isHit;
+	uint32_t* rayIdx;
+	uint16_t* ringId;
+	uint8_t* returnType;
+	float* distance;
+	float* intensityF32;
+	uint8_t* intensityU8;
+	float* laserRetro;
+	double* timeStampF64;
+	uint32_t* timeStampU32;
+	int32_t* entityId;
+	float* absVelocity;
+	float* relVelocity;
+	float* radialSpeed;
+	float* azimuth;
+	float* elevation;
+	float* normal;
+	float* incidentAngle;
+	/* per beam diagnostics */
+	int8_t* ambiguous;     /* 1 if any sample of the beam is a grazing hit, a near tie or at a range boundary */
+	double* primaryT;      /* FP64 ray parameter of the primary sample's closest hit, -1 if none */
+	int32_t* primaryInst;  /* instance of that hit, -1 if none */
+	int32_t* primaryPrim;  /* triangle of that hit, -1 if none */
+} OrcOutputs;
+
+/* Runs the whole raytrace node (raygen, closest hit, miss, beam reduction) for every beam. Returns 0 on success. */
+int orc_raytrace(const OrcScene* scene, const OrcRequest* request, OrcOutputs* out);
+
+/* Persistent variant: builds the oracle's BVHs once; the scene arrays must stay alive until destroy. */
+void* orc_world_create(const OrcScene* scene, int bruteForce);
+int orc_world_raytrace(void* world, const OrcRequest* request, OrcOutputs* out);
+void orc_world_destroy(void* world);
+
+/* Philox4x32-10 block (Salmon et al., SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Standard normal variates of the library's seeded stream: normal(seed, run, tag, i), i in [0, count). */
+void orc_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int count);
+
+/* Natural log and sin/cos(2 pi u) with the single-precision methods the noise stream is defined with. */
+float orc_logf(float x);
+void orc_sincos2pi(float u, float* s, float* c);
+
+#ifdef __cplusplus
+}
+#endif

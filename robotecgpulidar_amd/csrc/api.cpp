@@ -1039,6 +1039,25 @@ RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t
 	});
 }
 
+RGL_API rgl_status_t rgl_node_raytrace_set_kernel_timing(rgl_node_t node, bool enable)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
+		syncNodeGraph(n);
+		n->setTiming(enable);
+	});
+}
+
+RGL_API rgl_status_t rgl_node_raytrace_get_kernel_timing(rgl_node_t node, double* out_total_ms, int32_t* out_launches)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(out_total_ms != nullptr && out_launches != nullptr);
+		nodeAs<RaytraceNode>(node, "RaytraceNode")->getTiming(out_total_ms, out_launches);
+	});
+}
+
 RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count)
 {
 	return safeCall([&]() {

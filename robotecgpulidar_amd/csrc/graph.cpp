@@ -552,10 +552,50 @@ void RaytraceNode::enqueueImpl()
 	o.elevation = static_cast<float*>(ptr(RGL_FIELD_ELEVATION_F32));
 	o.normal = static_cast<Vec3f*>(ptr(RGL_FIELD_NORMAL_VEC3_F32));
 	o.incidentAngle = static_cast<float*>(ptr(RGL_FIELD_INCIDENT_ANGLE_F32));
+	if (timing) {
+		if (timingUsed == timingEvents.size()) {
+			hipEvent_t a, b;
+			RGL_HIP(hipEventCreate(&a));
+			RGL_HIP(hipEventCreate(&b));
+			timingEvents.emplace_back(a, b);
+		}
+		RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, stream()));
+	}
 	gpu::raytrace(stream(), view, p, o);
 	RGL_HIP(hipGetLastError());
+	if (timing) {
+		RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, stream()));
+		++timingUsed;
+	}
 	RGL_HIP(hipMemcpyAsync(overflowHost.ensure(sizeof(int32_t)), view.overflow, sizeof(int32_t), hipMemcpyDeviceToHost,
 	                       stream()));
+}
+
+void RaytraceNode::setTiming(bool enable)
+{
+	timing = enable;
+	timingUsed = 0;
+}
+
+void RaytraceNode::getTiming(double* totalMs, int32_t* launches)
+{
+	double total = 0.0;
+	for (size_t i = 0; i < timingUsed; ++i) {
+		RGL_HIP(hipEventSynchronize(timingEvents[i].second));
+		float ms = 0.0f;
+		RGL_HIP(hipEventElapsedTime(&ms, timingEvents[i].first, timingEvents[i].second));
+		total += ms;
+	}
+	*totalMs = total;
+	*launches = static_cast<int32_t>(timingUsed);
+}
+
+RaytraceNode::~RaytraceNode()
+{
+	for (auto& [a, b] : timingEvents) {
+		(void) hipEventDestroy(a);
+		(void) hipEventDestroy(b);
+	}
 }
 
 bool RaytraceNode::overflowed()

@@ -315,6 +315,10 @@ struct RaytraceNode : IPointsNode
 	void bruteForce(float* t, int32_t* inst, int32_t* prim);
 	RaytraceParams buildParams(const SceneView& view) const;
 	bool overflowed();
+	// HIP-event timing of the raytrace kernel launches (on the graph stream) while enabled.
+	void setTiming(bool enable);
+	void getTiming(double* totalMs, int32_t* launches);
+	~RaytraceNode() override;
 
 protected:
 	void validateImpl() override;
@@ -337,6 +341,9 @@ private:
 	rgl_return_mode_t mode = RGL_RETURN_FIRST;
 	float hHalf = 0.0f, vHalf = 0.0f;
 	PinnedBuffer overflowHost;
+	bool timing = false;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
+	size_t timingUsed = 0;
 };
 
 struct TransformPointsNode : IPointsNodeSingleInput
