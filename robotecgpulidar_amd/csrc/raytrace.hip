@@ -545,31 +545,28 @@ __global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytracePara
 			writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
 			continue;
 		}
-		const Tracker* t = nullptr;
-		switch (type) {
-			case 4: t = &first; break;           // RGL_RETURN_TYPE_FIRST
-			case 3: t = &second; break;          // RGL_RETURN_TYPE_SECOND
-			case 2: t = &last; break;            // RGL_RETURN_TYPE_LAST
-			case 1: t = &strongest; break;       // RGL_RETURN_TYPE_STRONGEST
-			case 5: t = &secondStrongest; break; // RGL_RETURN_TYPE_SECOND_STRONGEST
-			default: break;
-		}
-		if (t == nullptr) {
+		// Value selects (not a pointer) keep the trackers in registers.
+		if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
 			continue;
 		}
 		if (samples == 1) {
 			writeReturnHit(out, p, beam, rp, type, shade0);
 			continue;
 		}
+		const Tracker t = type == 4 ? first                      // RGL_RETURN_TYPE_FIRST
+		                  : type == 3 ? second                   // RGL_RETURN_TYPE_SECOND
+		                  : type == 2 ? last                     // RGL_RETURN_TYPE_LAST
+		                  : type == 1 ? strongest                // RGL_RETURN_TYPE_STRONGEST
+		                              : secondStrongest;         // RGL_RETURN_TYPE_SECOND_STRONGEST
 		Mat3x4f sr = ray;
-		if (t->sample > 0) {
-			const int layer = (t->sample - 1) / kBeamVertices;
-			const int vertex = (t->sample - 1) % kBeamVertices;
+		if (t.sample > 0) {
+			const int layer = (t.sample - 1) / kBeamVertices;
+			const int vertex = (t.sample - 1) % kBeamVertices;
 			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
 		}
 		Vec3f so, sd;
 		rayOriginDir(sr, so, sd);
-		const Shade sh = shadeSample(sc, p, range.x, so, sd, t->hit, needVel);
+		const Shade sh = shadeSample(sc, p, range.x, so, sd, t.hit, needVel);
 		writeReturnHit(out, p, beam, rp, type, sh);
 	}
 }
