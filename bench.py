@@ -105,7 +105,19 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for cpu_baseline (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copy kernels of MIB MiB (reference for rocprof byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch  # device memory plumbing for the calibration copy only
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this library runs every sensor of a frame on one MI355X and has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)

@@ -107,7 +107,7 @@ void lazyInit()
 		RGL_HIP(hipSetDevice(0));
 		hipDeviceProp_t prop{};
 		RGL_HIP(hipGetDeviceProperties(&prop, 0));
-		log(RGL_LOG_LEVEL_INFO, std::string("RGL-MI355X on ") + prop.name + " (" + prop.gcnArchName + ")");
+		log(RGL_LOG_LEVEL_DEBUG, std::string("RGL-MI355X on ") + prop.name + " (" + prop.gcnArchName + ")");
 	});
 	if (st != RGL_SUCCESS) {
 		updateState(RGL_INITIALIZATION_ERROR, lastMessage);

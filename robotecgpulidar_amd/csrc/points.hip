@@ -73,13 +73,14 @@ __device__ __forceinline__ void copyElem(const FieldCopy& f, int src, int dst)
 	}
 }
 
-// Offset of this tile = sum of the counts of all previous tiles; then a block-wide exclusive scan of the tile's
-// flags (each thread owns kItems consecutive elements) and the scatter of every field.
+// Offset of this tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so
+// each of the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave)
+// counts in LDS, which preserves input order.
 __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int32_t* flags, FieldCopyList fields,
                                 const int32_t* tileCounts, int numTiles, int32_t* countOut)
 {
 	__shared__ int32_t red[kBlock];
-	__shared__ int32_t scan[kBlock];
+	__shared__ int32_t waveBase[kItems][kBlock / 64];
 	const int n = liveCount(capacity, countIn);
 	// 1. tile offset
 	int s = 0;
@@ -98,31 +99,38 @@ __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int3
 	if (blockIdx.x == numTiles - 1 && threadIdx.x == 0) {
 		*countOut = tileOffset + tileCounts[blockIdx.x];
 	}
-	// 2. per-thread flags over consecutive elements (stable order inside the tile)
-	const int first = blockIdx.x * kTile + threadIdx.x * kItems;
-	int keep[kItems];
-	int local = 0;
+	// 2. flags, ballots and per-(round, wave) counts
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	const int base = blockIdx.x * kTile + threadIdx.x;
+	uint64_t masks[kItems];
 	for (int k = 0; k < kItems; ++k) {
-		const int i = first + k;
-		keep[k] = (i < n && flags[i] != 0) ? 1 : 0;
-		local += keep[k];
+		const int i = base + k * kBlock;
+		masks[k] = __ballot(i < n && flags[i] != 0);
+		if (lane == 0) {
+			waveBase[k][wave] = static_cast<int32_t>(__popcll(masks[k]));
+		}
 	}
-	scan[threadIdx.x] = local;
 	__syncthreads();
-	for (int off = 1; off < kBlock; off <<= 1) {
-		const int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
-		__syncthreads();
-		scan[threadIdx.x] += v;
-		__syncthreads();
-	}
-	int pos = tileOffset + scan[threadIdx.x] - local;
-	// 3. scatter
-	for (int k = 0; k < kItems; ++k) {
-		if (keep[k]) {
-			for (int f = 0; f < fields.count; ++f) {
-				copyElem(fields.f[f], first + k, pos);
+	if (threadIdx.x == 0) {
+		int run = tileOffset;
+		for (int k = 0; k < kItems; ++k) {
+			for (int w = 0; w < kBlock / 64; ++w) {
+				const int c = waveBase[k][w];
+				waveBase[k][w] = run;
+				run += c;
 			}
-			++pos;
+		}
+	}
+	__syncthreads();
+	// 3. scatter every field
+	for (int k = 0; k < kItems; ++k) {
+		if ((masks[k] >> lane) & 1ull) {
+			const int pos = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
+			for (int f = 0; f < fields.count; ++f) {
+				copyElem(fields.f[f], base + k * kBlock, pos);
+			}
 		}
 	}
 }

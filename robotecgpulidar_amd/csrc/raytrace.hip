@@ -161,7 +161,9 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		return false;
 	}
 	const int tid = threadIdx.x;
-	int32_t ovf[kOvfStack];
+	// The overflow part holds the same int32 entries as float bit patterns: with a different element type the
+	// compiler cannot fold "LDS load or scratch load" into one load through a generic pointer.
+	float ovf[kOvfStack];
 	int sp = 0;
 	bool overflow = false;
 #define RGL_PUSH(v)                                                                                                    \
@@ -170,13 +172,13 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			ldsStack[sp * kRtBlock + tid] = (v);                                                                       \
 			++sp;                                                                                                      \
 		} else if (sp < kLdsStack + kOvfStack) {                                                                       \
-			ovf[sp - kLdsStack] = (v);                                                                                 \
+			ovf[sp - kLdsStack] = __int_as_float(v);                                                                   \
 			++sp;                                                                                                      \
 		} else {                                                                                                       \
 			overflow = true;                                                                                           \
 		}                                                                                                              \
 	} while (0)
-#define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kRtBlock + tid] : ovf[sp - kLdsStack])
+#define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kRtBlock + tid] : __float_as_int(ovf[sp - kLdsStack]))
 
 	const TravRay world = makeTravRay(o, d);
 	TravRay cur = world;
@@ -474,14 +476,37 @@ __global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytracePara
 		ray = p.rayOriginToWorld * (TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f}) * rayLocal);
 	}
 
-	const bool divergent = p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f;
-	const int samples = divergent ? kBeamSamples : 1;
 	const bool needVel = p.velocityRequested != 0;
+	if (!(p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f)) {
+		// One sample per beam: the beam reduction degenerates to "every return is the primary sample"
+		// (src/gpu/nodeKernels.cu:345-354,399-421 with a single sample), so no trackers are kept.
+		Vec3f so, sd;
+		rayOriginDir(ray, so, sd);
+		HitRec h;
+		float d0 = p.farNonHitDistance;
+		if (traceClosest<false>(sc, so, sd, range.y, h, nullptr)) {
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
+			if (sh.isHit) {
+				for (int r = 0; r < R; ++r) {
+					const uint8_t type = returnTypeOf(p.returnMode, r);
+					if (type >= 1 && type <= 5) {
+						writeReturnHit(out, p, beam, beam * R + r, type, sh);
+					}
+				}
+				return;
+			}
+			d0 = sh.distance;
+		}
+		for (int r = 0; r < R; ++r) {
+			writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), d0);
+		}
+		return;
+	}
 
+	const int samples = kBeamSamples;
 	Tracker first, second, last, strongest, secondStrongest;
 	first.sample = -1;
 	float sample0Distance = p.farNonHitDistance;
-	Shade shade0;  // single-sample beams keep their fully shaded sample instead of re-shading it
 	for (int s = 0; s < samples; ++s) {
 		Mat3x4f sr = ray;
 		if (s > 0) {
@@ -498,13 +523,10 @@ __global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytracePara
 		cand.hit = h;
 		int isHit = 0;
 		if (any) {
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, samples == 1 && needVel);
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
 			isHit = sh.isHit;
 			cand.distance = sh.distance;
 			cand.intensity = sh.intensity;
-			if (samples == 1) {
-				shade0 = sh;
-			}
 		} else {
 			cand.distance = p.farNonHitDistance;
 			cand.intensity = 0.0f;
@@ -545,19 +567,20 @@ __global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytracePara
 			writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
 			continue;
 		}
-		// Value selects (not a pointer) keep the trackers in registers.
 		if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
 			continue;
 		}
-		if (samples == 1) {
-			writeReturnHit(out, p, beam, rp, type, shade0);
-			continue;
+		// Branch-wise copies (not an aggregate select of addresses) let the trackers stay in registers.
+		Tracker t = first; // RGL_RETURN_TYPE_FIRST
+		if (type == 3) {
+			t = second;
+		} else if (type == 2) {
+			t = last;
+		} else if (type == 1) {
+			t = strongest;
+		} else if (type == 5) {
+			t = secondStrongest;
 		}
-		const Tracker t = type == 4 ? first                      // RGL_RETURN_TYPE_FIRST
-		                  : type == 3 ? second                   // RGL_RETURN_TYPE_SECOND
-		                  : type == 2 ? last                     // RGL_RETURN_TYPE_LAST
-		                  : type == 1 ? strongest                // RGL_RETURN_TYPE_STRONGEST
-		                              : secondStrongest;         // RGL_RETURN_TYPE_SECOND_STRONGEST
 		Mat3x4f sr = ray;
 		if (t.sample > 0) {
 			const int layer = (t.sample - 1) / kBeamVertices;

@@ -198,7 +198,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("RGL_LIBRARY", LIB_PATH))
     if not p.exists():
         raise FileNotFoundError(f"{p} is not built; run __graft_entry__.build() or make -C robotecgpulidar_amd/csrc")
     lib = C.CDLL(str(p))
