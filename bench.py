@@ -1,14 +1,19 @@
-"""Benchmark: one lidar frame of the urban workload through the RGL graph on one MI355X.
+"""Benchmark: served-batched vs plain lidar frames through the RGL graph on one MI355X.
 
-Workload (one step = one frame, all inputs resident in HBM): ~2M-triangle urban scene (500 instanced icospheres,
-boxes and 4-12 sided prisms on a ground grid), 128 x 2048 = 262,144-ray lidar, range 200 m, Gaussian angular ray
-noise (sigma 0.05 deg) and distance noise (0.02 m + 0.001 d), velocity distortion at 10 m/s, non-hit compaction.
-Graph: rays -> pose -> range -> time offsets -> angular noise -> raytrace -> distance noise -> compact -> yield.
+Default workload (`--workload batch64`), one step = one frame of 64 lidars, all inputs resident in HBM:
+64 lidars of uneven size (16-128 channels x 900-4096 azimuths, 10.1M rays in total) over 8 device-resident urban
+scenes of 76k-3.6M triangles (8.0M in total). Each sensor's graph is
+rays -> pose -> range -> raytrace -> points transform (to the sensor frame) -> distance noise -> compact -> yield.
+`value` is Mrays/s (API beams completed through compaction, results readable on the host) with every frame run as
+ONE batch (rgl_graph_run_batch); `plain` is the same frames with one rgl_graph_run per sensor (each graph on its
+own stream). p50/p99 are frame latencies (submission until every sensor's kept count is readable).
 
-Prints one JSON line. `value` = Mrays/s of API beams completed through compaction with a host wait per frame
-(each frame's latency is measured from rgl_graph_run until the compacted count is readable on the host).
-`roofline` is computed for the raytrace kernel from its HIP-event durations on the graph stream;
-`cpu_baseline` times the FP64 CPU oracle (multithreaded) on the same frame for a bounded ~10 s sample.
+`--workload urban`: one 128 x 2048 lidar frame on the ~2M-triangle urban scene with angular and distance noise and
+velocity distortion (rays -> pose -> range -> time offsets -> angular noise -> raytrace -> distance noise ->
+compact -> yield), run plain.
+
+`roofline` is computed for the (batched) raytrace kernel from HIP-event durations on the stream it runs on;
+`cpu_baseline` times the FP64 CPU oracle (multithreaded) on one sensor of the same workload for a bounded sample.
 """
 from __future__ import annotations
 
@@ -34,7 +39,7 @@ FIELDS = [rgl.XYZ_VEC3_F32, rgl.DISTANCE_F32, rgl.ENTITY_ID_I32, rgl.NORMAL_VEC3
 ANGULAR_SIGMA = float(np.float32(0.05 * np.pi / 180.0))
 
 
-def build_graph(lidar: S.Lidar):
+def build_urban_graph(lidar: S.Lidar):
     n = rgl.Node
     rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
     pose = rgl.node_rays_transform(n(), lidar.pose)
@@ -57,11 +62,34 @@ def build_graph(lidar: S.Lidar):
     return rays, rt, out
 
 
-def frame(entry, out) -> int:
-    rgl.check("rgl_graph_run", entry.h)
+def build_sensor_graph(lidar: S.Lidar, scene, seed: int):
+    """rays -> pose -> range -> raytrace(scene) -> points transform (world -> sensor) -> distance noise -> compact
+    -> yield."""
+    n = rgl.Node
+    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
+    pose = rgl.node_rays_transform(n(), lidar.pose)
+    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
+    rt = rgl.node_raytrace(n(), scene)
+    to_sensor = S.f32(np.linalg.inv(np.vstack([lidar.pose.astype(np.float64), [0, 0, 0, 1]]))[:3])
+    tf = rgl.node_points_transform(n(), to_sensor)
+    dist = n()
+    rgl.check("rgl_node_gaussian_noise_distance", dist.ref, 0.0, 0.02, 0.001)
+    comp = rgl.node_points_compact_by_field(n(), rgl.IS_HIT_I32)
+    out = rgl.node_points_yield(n(), FIELDS)
+    rgl.chain(rays, pose, rng, rt, tf, dist, comp, out)
+    rgl.check("rgl_node_gaussian_noise_set_seed", dist.h, C.c_uint64(seed))
+    return rays, rt, out
+
+
+def kept_count(out) -> int:
     count = C.c_int32()
     rgl.check("rgl_graph_get_result_size", out.h, rgl.XYZ_VEC3_F32, C.byref(count), None)
     return count.value
+
+
+def frame(entry, out) -> int:
+    rgl.check("rgl_graph_run", entry.h)
+    return kept_count(out)
 
 
 def algorithmic_bytes(rt, n_rays: int, return_count: int, point_bytes: int, n_instances: int):
@@ -73,18 +101,20 @@ def algorithmic_bytes(rt, n_rays: int, return_count: int, point_bytes: int, n_in
     p = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
     rgl.check("rgl_node_raytrace_traversal_stats", rt.h, p(nodes), p(tris), p(insts), n_rays)
     b_io = n_rays * (48 + return_count * point_bytes) + 48 * n_instances
-    b_trav = 64 * int(nodes.sum(dtype=np.int64)) + 36 * int(tris.sum(dtype=np.int64)) + 64 * int(insts.sum(dtype=np.int64))
-    return b_io, b_trav, float(nodes.mean()), float(tris.mean()), float(insts.mean())
+    sums = (int(nodes.sum(dtype=np.int64)), int(tris.sum(dtype=np.int64)), int(insts.sum(dtype=np.int64)))
+    b_trav = 64 * sums[0] + 36 * sums[1] + 64 * sums[2]
+    return b_io, b_trav, sums
 
 
-def cpu_baseline(spec, lidar, budget_s: float):
+def cpu_baseline(spec, lidar, budget_s: float, distortion: bool, label: str):
     from oracle import oracle  # the oracle is the timed CPU baseline only here
     threads = min(os.cpu_count() or 1, 16)
     world_rays = np.einsum("ij,njk->nik", lidar.pose[:, :3], lidar.rays).astype(np.float32)
     world_rays[:, :, 3] += lidar.pose[:, 3]
     w = oracle.World(spec)
-    kw = dict(ray_origin_to_world=lidar.pose, ranges=[lidar.range], time_offsets=lidar.time_offsets, distortion=True,
-              lin_vel=(10, 0, 0), eps=0.0, threads=threads, world=w, fields=FIELDS)
+    kw = dict(ray_origin_to_world=lidar.pose, ranges=[lidar.range], eps=0.0, threads=threads, world=w, fields=FIELDS)
+    if distortion:
+        kw.update(time_offsets=lidar.time_offsets, distortion=True, lin_vel=(10, 0, 0))
     frames, t0 = 0, time.perf_counter()
     while True:
         oracle.raytrace(spec, world_rays, **kw)
@@ -95,15 +125,193 @@ def cpu_baseline(spec, lidar, budget_s: float):
     w.close()
     return {"value": round(frames * len(world_rays) / el / 1e6, 4), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"{frames} full frames ({len(world_rays)} rays each) of the same scene and lidar, FP64 oracle "
-                      f"BVH traversal with distortion, BVH build excluded, {el:.1f} s"}
+            "sample": f"{frames} frames of {label} ({len(world_rays)} rays each), FP64 oracle BVH traversal, "
+                      f"BVH build excluded, {el:.1f} s"}
+
+
+def latency_stats(lat_s):
+    ms = np.array(lat_s) * 1e3
+    return round(float(np.percentile(ms, 50)), 4), round(float(np.percentile(ms, 99)), 4)
+
+
+def roofline(b_alg: int, b_io: int, kernel_ms: float, traffic):
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": b_alg,
+            "io_bytes_per_launch": b_io, "io_only_frac": round(b_io / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+
+
+def traffic_bytes(name: str):
+    tf = ROOT / "profiles" / name
+    return json.loads(tf.read_text()).get("bytes_per_launch") if tf.exists() else None
+
+
+def run_batch64(args):
+    specs, sensors = S.config4_workload()
+    scenes = []
+    for spec in specs:
+        sc = rgl.scene_create()
+        S.load_into_rgl(spec, scene=sc)
+        scenes.append(sc)
+    graphs = [build_sensor_graph(x.lidar, scenes[x.scene], 0xC0F40000 + k) for k, x in enumerate(sensors)]
+    entries = [g[0] for g in graphs]
+    outs = [g[2] for g in graphs]
+    rays_per_frame = sum(len(x.lidar.rays) for x in sensors)
+
+    def plain_frame():
+        for e in entries:
+            rgl.check("rgl_graph_run", e.h)
+        return sum(kept_count(o) for o in outs)
+
+    def batched_frame():
+        rgl.graph_run_batch(entries)
+        return sum(kept_count(o) for o in outs)
+
+    def timed(fn, steps):
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f0 = time.perf_counter()
+            kept = fn()
+            lat.append(time.perf_counter() - f0)
+        return time.perf_counter() - t0, lat, kept
+
+    for _ in range(args.warmup):
+        plain_frame()
+        batched_frame()
+    p_total, p_lat, p_kept = timed(plain_frame, args.steps)
+    b_total, b_lat, b_kept = timed(batched_frame, args.steps)
+    # Kernel durations from HIP events, in separate frames so the timed frames carry no extra events.
+    kframes = 3
+    for g in graphs:
+        rgl.check("rgl_node_raytrace_set_kernel_timing", g[1].h, True)
+    timed(plain_frame, kframes)
+    p_kernel = 0.0
+    for g in graphs:
+        ms, n = C.c_double(), C.c_int32()
+        rgl.check("rgl_node_raytrace_get_kernel_timing", g[1].h, C.byref(ms), C.byref(n))
+        rgl.check("rgl_node_raytrace_set_kernel_timing", g[1].h, False)
+        p_kernel += ms.value / max(n.value, 1)
+    rgl.batch_kernel_timing(True)
+    timed(batched_frame, kframes)
+    kms, launches = rgl.batch_kernel_timing()
+    rgl.batch_kernel_timing(False)
+    kernel_ms = kms / kframes  # batched raytrace time per frame
+    launches_per_frame = launches / kframes
+
+    point_bytes = sum(rgl.field_size(f) for f in FIELDS)
+    b_io = b_trav = 0
+    sums = np.zeros(3, np.int64)
+    for (e, rt, o), x in zip(graphs, sensors):
+        io, trav, s3 = algorithmic_bytes(rt, len(x.lidar.rays), 1, point_bytes, len(specs[x.scene].instances))
+        b_io += io
+        b_trav += trav
+        sums += np.array(s3, np.int64)
+    value = args.steps * rays_per_frame / b_total / 1e6
+    plain_value = args.steps * rays_per_frame / p_total / 1e6
+    p50, p99 = latency_stats(b_lat)
+    pp50, pp99 = latency_stats(p_lat)
+    cpu = None
+    if args.cpu_budget > 0:
+        x = sensors[0]
+        cpu = cpu_baseline(specs[x.scene], x.lidar, args.cpu_budget, False,
+                           f"sensor 0 of the batch ({x.lidar.channels} ch x {x.lidar.azimuths} az on its "
+                           f"{specs[x.scene].triangle_count}-triangle scene)")
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b_total / args.steps * 1e3, 4),
+        "p50_ms": p50,
+        "p99_ms": p99,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "64-lidar mixed batch over 8 urban scenes, every sensor frame in one rgl_graph_run_batch "
+                        "(rays->pose->range->raytrace->points transform->distance noise->compact->yield)",
+            "sensors": len(sensors), "scenes": len(specs), "triangles": sum(s.triangle_count for s in specs),
+            "scene_triangles": [s.triangle_count for s in specs], "rays_per_step": rays_per_frame,
+            "kept_points_per_step": b_kept, "mode": "batched",
+        },
+        "plain": {"value": round(plain_value, 3), "ms_per_step": round(p_total / args.steps * 1e3, 4),
+                  "p50_ms": pp50, "p99_ms": pp99, "raytrace_kernel_ms_per_step": round(p_kernel, 4),
+                  "kept_points_per_step": p_kept, "mode": "one rgl_graph_run per sensor"},
+        "batched_vs_plain": round(value / plain_value, 3),
+        "raytrace_kernel_ms": round(kernel_ms, 4),
+        "raytrace_launches_per_step": launches_per_frame,
+        "traversal_per_ray": {"nodes": round(sums[0] / rays_per_frame, 2), "triangles": round(sums[1] / rays_per_frame, 2),
+                              "instances": round(sums[2] / rays_per_frame, 2)},
+        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, traffic_bytes("raytrace_batch64_traffic.json")),
+        "cpu_baseline": cpu,
+    }
+
+
+def run_urban(args):
+    spec, lidar = S.config2_scene(), S.config2_lidar()
+    S.load_into_rgl(spec)
+    entry, rt, out = build_urban_graph(lidar)
+    n_rays = len(lidar.rays)
+    for _ in range(args.warmup):
+        frame(entry, out)
+    rgl.check("rgl_node_raytrace_set_kernel_timing", rt.h, True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f0 = time.perf_counter()
+        hits = frame(entry, out)
+        lat.append(time.perf_counter() - f0)
+    total = time.perf_counter() - t0
+    kms = C.c_double()
+    launches = C.c_int32()
+    rgl.check("rgl_node_raytrace_get_kernel_timing", rt.h, C.byref(kms), C.byref(launches))
+    rgl.check("rgl_node_raytrace_set_kernel_timing", rt.h, False)
+    kernel_ms = kms.value / max(launches.value, 1)
+    point_bytes = sum(rgl.field_size(f) for f in FIELDS)
+    b_io, b_trav, sums = algorithmic_bytes(rt, n_rays, 1, point_bytes, len(spec.instances))
+    p50, p99 = latency_stats(lat)
+    return {
+        "metric": METRIC,
+        "value": round(args.steps * n_rays / total / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 4),
+        "p50_ms": p50,
+        "p99_ms": p99,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "urban scene, plain single-sensor graph (rays->pose->range->offsets->angular noise->raytrace->"
+                        "distance noise->compact->yield)",
+            "triangles": spec.triangle_count, "instances": len(spec.instances), "rays": n_rays,
+            "lidar": "128 ch x 2048 az, el [-25,15] deg, 200 m", "noise": "angular 0.05 deg, distance 0.02+0.001d",
+            "distortion": "10 m/s", "hits_per_frame": hits, "mode": "plain",
+        },
+        "raytrace_kernel_ms": round(kernel_ms, 4),
+        "traversal_per_ray": {"nodes": round(sums[0] / n_rays, 2), "triangles": round(sums[1] / n_rays, 2),
+                              "instances": round(sums[2] / n_rays, 2)},
+        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, traffic_bytes("raytrace_urban_traffic.json")),
+        "cpu_baseline": cpu_baseline(spec, lidar, args.cpu_budget, True, "the same scene and lidar")
+        if args.cpu_budget > 0 else None,
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["batch64", "urban"], default="batch64")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for cpu_baseline (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copy kernels of MIB MiB (reference for rocprof byte counters)")
@@ -122,70 +330,9 @@ def main():
         print("bench.py measures one GPU: this library runs every sensor of a frame on one MI355X and has no "
               "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
     rgl.load()
     rgl.check("rgl_configure_logging", 6, None, False)
-    spec, lidar = S.config2_scene(), S.config2_lidar()
-    S.load_into_rgl(spec)
-    entry, rt, out = build_graph(lidar)
-    n_rays = len(lidar.rays)
-
-    for _ in range(args.warmup):
-        frame(entry, out)
-    rgl.check("rgl_node_raytrace_set_kernel_timing", rt.h, True)
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f0 = time.perf_counter()
-        hits = frame(entry, out)
-        lat.append(time.perf_counter() - f0)
-    total = time.perf_counter() - t0
-    kms = C.c_double()
-    launches = C.c_int32()
-    rgl.check("rgl_node_raytrace_get_kernel_timing", rt.h, C.byref(kms), C.byref(launches))
-    rgl.check("rgl_node_raytrace_set_kernel_timing", rt.h, False)
-    kernel_ms = kms.value / max(launches.value, 1)
-
-    point_bytes = sum(rgl.field_size(f) for f in FIELDS)
-    b_io, b_trav, nodes_pr, tris_pr, insts_pr = algorithmic_bytes(rt, n_rays, 1, point_bytes, len(spec.instances))
-    achieved = (b_io + b_trav) / (kernel_ms * 1e-3) / 1e9
-    traffic = None
-    tf = ROOT / "profiles" / "raytrace_config2_traffic.json"
-    if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
-
-    lat_ms = np.array(lat) * 1e3
-    value = args.steps * n_rays / total / 1e6
-    result = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(total / args.steps * 1e3, 4),
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "urban scene, plain per-sensor graph (rays->pose->range->offsets->angular noise->raytrace->"
-                        "distance noise->compact->yield)",
-            "triangles": spec.triangle_count, "instances": len(spec.instances), "rays": n_rays,
-            "lidar": "128 ch x 2048 az, el [-25,15] deg, 200 m", "noise": "angular 0.05 deg, distance 0.02+0.001d",
-            "distortion": "10 m/s", "hits_per_frame": hits, "mode": "plain",
-        },
-        "raytrace_kernel_ms": round(kernel_ms, 4),
-        "traversal_per_ray": {"nodes": round(nodes_pr, 2), "triangles": round(tris_pr, 2), "instances": round(insts_pr, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": b_io + b_trav, "io_bytes_per_launch": b_io,
-                     "io_only_frac": round(b_io / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
-        "cpu_baseline": cpu_baseline(spec, lidar, args.cpu_budget) if args.cpu_budget > 0 else None,
-    }
+    result = run_batch64(args) if args.workload == "batch64" else run_urban(args)
     rgl.cleanup()
     print(json.dumps(result))
 

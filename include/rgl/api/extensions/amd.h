@@ -35,5 +35,25 @@ RGL_API rgl_status_t rgl_node_raytrace_set_kernel_timing(rgl_node_t node, bool e
 // Total device time (ms) and number of raytrace kernel launches timed since timing was enabled; waits for them.
 RGL_API rgl_status_t rgl_node_raytrace_get_kernel_timing(rgl_node_t node, double* out_total_ms, int32_t* out_launches);
 
+// Creates an empty scene. Entities created with it raytrace only against it; rgl_scene_t NULL stays the default
+// scene of the reference API (rgl_entity_create, include/rgl/api/core.h:631, and rgl_node_raytrace take a scene
+// argument, but the reference accepts only NULL there: src/api/apiCore.cpp:297,922).
+RGL_API rgl_status_t rgl_scene_create(rgl_scene_t* out_scene);
+
+// Destroys a scene created with rgl_scene_create together with its entities; waits for graphs in flight.
+RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene);
+
+// Runs the graphs containing `nodes[0..node_count)` (any node of each graph; a graph named twice runs once) as one
+// batch: nodes of the same kind at the same depth of every graph execute in one launch. Each graph's results are
+// identical to those of rgl_graph_run on the same graph and state; results are read with
+// rgl_graph_get_result_size / rgl_graph_get_result_data as usual. Graphs may raytrace different scenes.
+RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* nodes, int32_t node_count);
+
+// Enables (and resets) HIP-event timing of the batched raytrace launches.
+RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable);
+
+// Total device time (ms) and number of batched raytrace launches timed since timing was enabled; waits for them.
+RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int32_t* out_launches);
+
 // Standard normal variates normal(seed, run, tag, i) for i in [0, count), computed on the GPU.
 RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);

@@ -403,6 +403,7 @@ struct Shade
 	bool isHit;
 	float distance;
 	float intensity;
+	double base; // intensity before the incidence factor (texel or default intensity)
 	V3 normal;
 	float incident;
 	V3 absVel, relVel;
@@ -412,7 +413,7 @@ struct Shade
 };
 
 Shade shade(const World& w, const OrcRequest& q, const M34& rayToWorld, double minRange, V3 o, V3 d, const Hit& h,
-            bool* ambiguous, double eps)
+            bool* ambiguous, double eps, bool* texelEdge)
 {
 	Shade s{};
 	const Inst& in = w.insts[h.c.inst];
@@ -449,10 +450,19 @@ Shade shade(const World& w, const OrcRequest& q, const M34& rayToWorld, double m
 		const double tv = wA0 * uvA[1] + h.c.u * uvB[1] + h.c.v * uvC[1];
 		const double fu = tu - std::floor(tu), fv = tv - std::floor(tv);
 		const int W = sc.texWidth[in.texture], H = sc.texHeight[in.texture];
+		// Point sampling jumps at texel edges: a coordinate within FP32's reach of one may pick the neighbour.
+		if (texelEdge != nullptr && eps > 0.0) {
+			const double gu = fu * W, gv = fv * H;
+			if (std::fabs(gu - std::round(gu)) <= eps * W * std::max(1.0, std::fabs(tu)) ||
+			    std::fabs(gv - std::round(gv)) <= eps * H * std::max(1.0, std::fabs(tv))) {
+				*texelEdge = true;
+			}
+		}
 		int x = std::min(std::max(static_cast<int>(std::floor(fu * W)), 0), W - 1);
 		int y = std::min(std::max(static_cast<int>(std::floor(fv * H)), 0), H - 1);
 		intensity = sc.texels[sc.texOffset[in.texture] + static_cast<int64_t>(y) * W + x];
 	}
+	s.base = intensity;
 	s.intensity = static_cast<float>(intensity * cosI);
 	s.entityId = in.id;
 	s.retro = in.retro;
@@ -512,8 +522,9 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		if (out.rayIdx) out.rayIdx[rp] = static_cast<uint32_t>(b);
 		if (out.azimuth) out.azimuth[rp] = azimuthOf(rayLocal);
 		if (out.elevation) out.elevation[rp] = elevationOf(rayLocal);
-		if (out.timeStampF64) out.timeStampF64[rp] = q.distortion ? q.timeOffsetsMs[b] * 1e-3 : 0.0;
-		if (out.timeStampU32) out.timeStampU32[rp] = q.distortion ? static_cast<uint32_t>(q.timeOffsetsMs[b] * 1e6) : 0u;
+		// Time stamps are computed in float by the reference (src/gpu/optixPrograms.cu:313-321).
+		if (out.timeStampF64) out.timeStampF64[rp] = q.distortion ? static_cast<double>(q.timeOffsetsMs[b] * 1e-3f) : 0.0;
+		if (out.timeStampU32) out.timeStampU32[rp] = q.distortion ? static_cast<uint32_t>(q.timeOffsetsMs[b] * 1e6f) : 0u;
 		if (out.ringId && q.ringIds) out.ringId[rp] = static_cast<uint16_t>(q.ringIds[b % q.nRingIds]);
 	}
 	const float* range = q.ranges + 2 * (q.nRanges == 1 ? 0 : b);
@@ -568,6 +579,15 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	};
 	Track first, second, last, strongest, secondStrongest;
 	float sample0Distance = q.farDistance;
+	// Beam reduction compares samples strictly and in sample order (nodeKernels.cu:369-397): a comparison whose
+	// operands are within FP32's reach of each other may go the other way on the GPU and change the selection.
+	// Distances carry ~eps relative error; intensities texel*cos(incidence) carry ~eps relative plus about
+	// 4e-6 of the texel value from the FP32 normal (dominant at grazing incidence).
+	bool distTie = false, intTie = false, texelEdge = false;
+	auto nearD = [&](double a, double b) { return std::fabs(a - b) <= eps * std::max(std::fabs(a), std::fabs(b)); };
+	auto nearI = [&](double a, double b, double baseA, double baseB) {
+		return std::fabs(a - b) <= eps * std::max(std::fabs(a), std::fabs(b)) + 4e-6 * std::max(baseA, baseB);
+	};
 	for (int s = 0; s < samples; ++s) {
 		M34 sr = ray;
 		if (s > 0) {
@@ -588,7 +608,8 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		c.sample = s;
 		bool isHit = false;
 		if (h.found) {
-			c.sh = shade(w, q, toWorld, minRange, so, sd, h, out.ambiguous ? &ambiguous : nullptr, eps);
+			c.sh = shade(w, q, toWorld, minRange, so, sd, h, out.ambiguous ? &ambiguous : nullptr, eps,
+			             out.ambiguous ? &texelEdge : nullptr);
 			isHit = c.sh.isHit;
 			c.distance = c.sh.distance;
 			c.intensity = c.sh.intensity;
@@ -602,6 +623,14 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		}
 		if (s == 0) sample0Distance = c.distance;
 		if (!isHit) continue;
+		if (samples > 1 && eps > 0.0 && first.sample >= 0) {
+			const double dd = c.distance, ii = c.intensity, bb = c.sh.base;
+			if (nearD(dd, first.distance) || nearD(dd, second.distance) || nearD(dd, last.distance)) distTie = true;
+			if (nearI(ii, strongest.intensity, bb, strongest.sh.base) ||
+			    nearI(ii, secondStrongest.intensity, bb, secondStrongest.sh.base)) {
+				intTie = true;
+			}
+		}
 		if (first.sample < 0) {
 			first = second = last = strongest = secondStrongest = c;
 			continue;
@@ -622,7 +651,10 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 			secondStrongest = c;
 		}
 	}
-	if (out.ambiguous) out.ambiguous[b] = ambiguous ? 1 : 0;
+	if (out.ambiguous) {
+		out.ambiguous[b] =
+		    static_cast<int8_t>((ambiguous ? 1 : 0) | (distTie ? 2 : 0) | (intTie ? 4 : 0) | (texelEdge ? 8 : 0));
+	}
 	for (int r = 0; r < R; ++r) {
 		const int rp = b * R + r;
 		const uint8_t type = typeOf(r);

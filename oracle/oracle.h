@@ -83,7 +83,10 @@ typedef struct
 	float* normal;
 	float* incidentAngle;
 	/* per beam diagnostics */
-	int8_t* ambiguous;     /* 1 if any sample of the beam is a grazing hit, a near tie or at a range boundary */
+	int8_t* ambiguous;     /* bit 0: a sample grazes an edge, ties another instance or sits at a range boundary;
+	                          bit 1: a distance comparison of the beam reduction is a near tie;
+	                          bit 2: an intensity comparison of the beam reduction is a near tie;
+	                          bit 3: a sample's texture coordinate lies at a texel edge */
 	double* primaryT;      /* FP64 ray parameter of the primary sample's closest hit, -1 if none */
 	int32_t* primaryInst;  /* instance of that hit, -1 if none */
 	int32_t* primaryPrim;  /* triangle of that hit, -1 if none */

@@ -205,6 +205,11 @@ void checkOverflow(const rgl::Node::Ptr& n)
 	}
 }
 
+std::shared_ptr<rgl::Scene> sceneOf(rgl_scene_t scene)
+{
+	return scene == nullptr ? rgl::Scene::defaultShared() : Registry<rgl::Scene>::get(scene, "Scene");
+}
+
 Mat3x4f toMat(const rgl_mat3x4f* m)
 {
 	Mat3x4f r;
@@ -308,6 +313,7 @@ RGL_API void rgl_get_last_error_string(const char** out_error_string)
 RGL_API rgl_status_t rgl_cleanup(void)
 {
 	return safeCall([&]() {
+		GraphBatch::instance().synchronize();
 		for (auto& n : allNodes()) {
 			if (n->hasCtx()) {
 				n->getCtx()->detachAndDestroy();
@@ -320,6 +326,10 @@ RGL_API rgl_status_t rgl_cleanup(void)
 		Registry<rgl::Entity>::map().clear();
 		Registry<rgl::Mesh>::map().clear();
 		Registry<rgl::Texture>::map().clear();
+		for (auto& [h, sc] : Registry<rgl::Scene>::map()) {
+			sc->clear();
+		}
+		Registry<rgl::Scene>::map().clear();
 		rgl::Scene::instance().clear();
 	});
 }
@@ -350,7 +360,7 @@ RGL_API rgl_status_t rgl_mesh_set_texture_coords(rgl_mesh_t mesh, const rgl_vec2
 		CHECK_ARG(uv_count > 0);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Mesh>::get(mesh, "Mesh")->setTexCoords(reinterpret_cast<const Vec2f*>(uvs), uv_count);
-		rgl::Scene::instance().markDirty(false);
+		rgl::Scene::markAllDirty(false);
 	});
 }
 
@@ -398,11 +408,11 @@ RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t sce
 	return safeCall([&]() {
 		CHECK_ARG(out_entity != nullptr);
 		CHECK_ARG(mesh != nullptr);
-		CHECK_ARG(scene == nullptr);
 		GraphRunCtx::synchronizeAll();
-		auto e = std::make_shared<rgl::Entity>(Registry<rgl::Mesh>::get(mesh, "Mesh"));
+		auto sc = sceneOf(scene);
+		auto e = std::make_shared<rgl::Entity>(Registry<rgl::Mesh>::get(mesh, "Mesh"), sc.get());
 		Registry<rgl::Entity>::add(e);
-		rgl::Scene::instance().addEntity(e);
+		sc->addEntity(e);
 		*out_entity = reinterpret_cast<rgl_entity_t>(e.get());
 	});
 }
@@ -413,7 +423,7 @@ RGL_API rgl_status_t rgl_entity_destroy(rgl_entity_t entity)
 		CHECK_ARG(entity != nullptr);
 		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
-		rgl::Scene::instance().removeEntity(e);
+		e->scene->removeEntity(e);
 		Registry<rgl::Entity>::release(entity, "Entity");
 	});
 }
@@ -450,7 +460,7 @@ RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id)
 			throw std::invalid_argument("Entity ID (" + std::to_string(id) + ") must be less than " + std::to_string(maxId));
 		}
 		e->id = id;
-		rgl::Scene::instance().markDirty(false);
+		e->scene->markDirty(false);
 	});
 }
 
@@ -462,7 +472,7 @@ RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_t
 		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		e->texture = Registry<rgl::Texture>::get(texture, "Texture");
-		rgl::Scene::instance().markDirty(false);
+		e->scene->markDirty(false);
 	});
 }
 
@@ -471,8 +481,9 @@ RGL_API rgl_status_t rgl_entity_set_laser_retro(rgl_entity_t entity, float retro
 	return safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		GraphRunCtx::synchronizeAll();
-		Registry<rgl::Entity>::get(entity, "Entity")->laserRetro = retro;
-		rgl::Scene::instance().markDirty(false);
+		auto e = Registry<rgl::Entity>::get(entity, "Entity");
+		e->laserRetro = retro;
+		e->scene->markDirty(false);
 	});
 }
 
@@ -531,10 +542,10 @@ RGL_API rgl_status_t rgl_texture_is_alive(rgl_texture_t texture, bool* out_alive
 RGL_API rgl_status_t rgl_scene_set_time(rgl_scene_t scene, uint64_t nanoseconds)
 {
 	return safeCall([&]() {
-		CHECK_ARG(scene == nullptr);
 		GraphRunCtx::synchronizeAll();
-		rgl::Scene::instance().setTime(nanoseconds);
-		rgl::Scene::instance().markDirty(false);
+		auto sc = sceneOf(scene);
+		sc->setTime(nanoseconds);
+		sc->markDirty(false);
 	});
 }
 
@@ -610,8 +621,8 @@ RGL_API rgl_status_t rgl_node_raytrace(rgl_node_t* node, rgl_scene_t scene)
 {
 	return safeCall([&]() {
 		CHECK_ARG(node != nullptr);
-		CHECK_ARG(scene == nullptr);
-		createOrUpdateNode<RaytraceNode>(node, "RaytraceNode", false, [&](RaytraceNode& n) { n.setParameters(); });
+		auto sc = sceneOf(scene);
+		createOrUpdateNode<RaytraceNode>(node, "RaytraceNode", false, [&](RaytraceNode& n) { n.setParameters(sc); });
 	});
 }
 
@@ -860,6 +871,34 @@ RGL_API rgl_status_t rgl_graph_run(rgl_node_t raw_node)
 	});
 }
 
+RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* raw_nodes, int32_t node_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node_count >= 0);
+		CHECK_ARG(raw_nodes != nullptr || node_count == 0);
+		std::vector<rgl::Node::Ptr> entries;
+		entries.reserve(static_cast<size_t>(node_count));
+		for (int32_t i = 0; i < node_count; ++i) {
+			CHECK_ARG(raw_nodes[i] != nullptr);
+			entries.push_back(lookupNode(raw_nodes[i]));
+		}
+		GraphBatch::instance().run(entries);
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable)
+{
+	return safeCall([&]() { GraphBatch::instance().setTiming(enable); });
+}
+
+RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int32_t* out_launches)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_total_ms != nullptr && out_launches != nullptr);
+		GraphBatch::instance().getTiming(out_total_ms, out_launches);
+	});
+}
+
 RGL_API rgl_status_t rgl_graph_destroy(rgl_node_t raw_node)
 {
 	return safeCall([&]() {
@@ -998,9 +1037,32 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_get_stream(rgl_node_t node, uint64_
 RGL_API rgl_status_t rgl_scene_set_tlas_refit(rgl_scene_t scene, bool refit)
 {
 	return safeCall([&]() {
-		CHECK_ARG(scene == nullptr);
 		GraphRunCtx::synchronizeAll();
-		rgl::Scene::instance().tlasUpdate = refit ? rgl::TlasUpdate::Refit : rgl::TlasUpdate::Rebuild;
+		sceneOf(scene)->tlasUpdate = refit ? rgl::TlasUpdate::Refit : rgl::TlasUpdate::Rebuild;
+	});
+}
+
+RGL_API rgl_status_t rgl_scene_create(rgl_scene_t* out_scene)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_scene != nullptr);
+		auto sc = std::make_shared<rgl::Scene>();
+		Registry<rgl::Scene>::add(sc);
+		*out_scene = reinterpret_cast<rgl_scene_t>(sc.get());
+	});
+}
+
+RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene)
+{
+	return safeCall([&]() {
+		CHECK_ARG(scene != nullptr);
+		GraphRunCtx::synchronizeAll();
+		auto sc = Registry<rgl::Scene>::get(scene, "Scene");
+		for (auto it = Registry<rgl::Entity>::map().begin(); it != Registry<rgl::Entity>::map().end();) {
+			it = it->second->scene == sc.get() ? Registry<rgl::Entity>::map().erase(it) : std::next(it);
+		}
+		sc->clear();
+		Registry<rgl::Scene>::release(scene, "Scene");
 	});
 }
 

@@ -1,0 +1,420 @@
+// Batched execution of many independent graphs (one per sensor) on one stream.
+//
+// The reference runs each graph on its own stream and launches every node's kernels separately
+// (src/graph/GraphRunCtx.cpp:144-211, src/gpu/RaytraceNode.cpp + OptiX launch per graph). A scan of a 16-128
+// channel lidar is only a few thousand to a few hundred thousand rays, far too few to fill 256 CUs, so on MI355X
+// a frame of many sensors is latency-bound when the graphs run one after the other. Here the nodes of all graphs
+// are list-scheduled into steps: each step launches every ready node of one stage kind in ONE launch (a raytrace
+// launch over every sensor's beams, possibly on different scenes; one elementwise launch for all transform /
+// noise / format nodes; one pair of compaction launches). Cheap kinds are drained first and raytracing last, so
+// graphs of different shapes (e.g. with and without ray noise) still share one raytrace launch. The launch
+// descriptors of the whole batch go to the device in a single copy; kept counts and overflow flags of every graph
+// come back in a single copy. Each item runs the same device code as the graph's own kernels with the same
+// parameters, so a graph's results do not depend on whether it ran alone or in a batch.
+#include <algorithm>
+#include <cstring>
+
+#include "graph.hpp"
+#include "kernels.hpp"
+
+namespace rgl {
+
+namespace {
+constexpr size_t kAlign = 256;
+
+size_t alignUp(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+} // namespace
+
+struct GraphBatch::Plan
+{
+	struct Level
+	{
+		std::vector<RaytraceBatchItem> rt;
+		std::vector<RaytraceNode*> rtNodes;
+		std::vector<PointOpItem> op;
+		std::vector<int> opLayout; // index into lists, -1 = none
+		std::vector<CompactItem> cp;
+		std::vector<int> cpList;
+		std::vector<CompactByFieldPointsNode*> cpNodes;
+		std::vector<Node*> other;
+		int rtBlocks = 0, opBlocks = 0, cpTiles = 0;
+		size_t rtOff = 0, opOff = 0, cpOff = 0;
+		void clear()
+		{
+			rt.clear();
+			rtNodes.clear();
+			op.clear();
+			opLayout.clear();
+			cp.clear();
+			cpList.clear();
+			cpNodes.clear();
+			other.clear();
+			rtBlocks = opBlocks = cpTiles = 0;
+		}
+	};
+	std::vector<Level> levels; // steps, in launch order
+	std::vector<Node*> readyOp, readyRt, readyCp, passive;
+	std::vector<FieldCopyList> lists;
+	std::vector<const int32_t*> gather;
+	std::vector<std::shared_ptr<GraphRunCtx>> ctxs;
+	std::vector<Scene*> scenes;
+};
+
+GraphBatch& GraphBatch::instance()
+{
+	// Never destroyed: its streams and buffers must not be released after the HIP runtime shuts down.
+	static GraphBatch* batch = new GraphBatch();
+	return *batch;
+}
+
+std::shared_ptr<PinnedBuffer> GraphBatch::hostSlots(size_t bytes)
+{
+	for (auto& p : slotPool) {
+		if (p.use_count() == 1 && p->capacity() >= bytes) {
+			return p;
+		}
+	}
+	auto p = std::make_shared<PinnedBuffer>();
+	p->ensure(bytes);
+	slotPool.push_back(p);
+	return p;
+}
+
+std::shared_ptr<Event> GraphBatch::completionEvent()
+{
+	for (auto& e : eventPool) {
+		if (e.use_count() == 1) {
+			return e;
+		}
+	}
+	eventPool.push_back(std::make_shared<Event>());
+	return eventPool.back();
+}
+
+void GraphBatch::synchronize()
+{
+	if (stream_) {
+		stream_->sync();
+	}
+}
+
+void GraphBatch::setTiming(bool enable)
+{
+	synchronize();
+	timing = enable;
+	timingUsed = 0;
+}
+
+void GraphBatch::getTiming(double* totalMs, int32_t* launches)
+{
+	double total = 0.0;
+	for (size_t i = 0; i < timingUsed; ++i) {
+		RGL_HIP(hipEventSynchronize(timingEvents[i].second));
+		float ms = 0.0f;
+		RGL_HIP(hipEventElapsedTime(&ms, timingEvents[i].first, timingEvents[i].second));
+		total += ms;
+	}
+	*totalMs = total;
+	*launches = static_cast<int32_t>(timingUsed);
+}
+
+void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
+{
+	if (entryNodes.empty()) {
+		return;
+	}
+	if (!stream_) {
+		stream_ = std::make_unique<Stream>();
+		uploaded = std::make_unique<Event>();
+		scratchEvent = std::make_unique<Event>();
+		plan = std::make_unique<Plan>();
+	}
+	Plan& pl = *plan;
+	pl.ctxs.clear();
+	for (auto& n : entryNodes) {
+		if (!n->hasCtx()) {
+			GraphRunCtx::createAndAttach(n);
+		}
+		auto c = n->getCtx();
+		if (std::find(pl.ctxs.begin(), pl.ctxs.end(), c) == pl.ctxs.end()) {
+			pl.ctxs.push_back(c);
+		}
+	}
+	// Every graph is validated before anything is enqueued, so an invalid graph fails the whole batch cleanly.
+	size_t nodeCount = 0;
+	for (auto& c : pl.ctxs) {
+		for (auto& n : c->validatedOrder()) {
+			n->batchLevel = static_cast<int32_t>(n->getInputs().size()); // inputs not yet scheduled
+			++nodeCount;
+		}
+	}
+	const hipStream_t s = stream_->get();
+	uploaded->sync(); // the previous batch's descriptor upload has read the staging buffer
+	for (auto& c : pl.ctxs) {
+		c->beginBatch(s, *scratchEvent);
+	}
+	try {
+		pl.lists.clear();
+		pl.gather.clear();
+		pl.scenes.clear();
+		pl.readyOp.clear();
+		pl.readyRt.clear();
+		pl.readyCp.clear();
+		pl.passive.clear();
+		int steps = 0;
+		auto newStep = [&]() -> Plan::Level& {
+			if (pl.levels.size() <= static_cast<size_t>(steps)) {
+				pl.levels.emplace_back();
+			}
+			Plan::Level& L = pl.levels[steps++];
+			L.clear();
+			return L;
+		};
+		auto makeReady = [&](Node* n) {
+			switch (n->batchStage()) {
+				case BatchStage::Passive:
+				case BatchStage::Other: pl.passive.push_back(n); break;
+				case BatchStage::PointOp: pl.readyOp.push_back(n); break;
+				case BatchStage::Compact: pl.readyCp.push_back(n); break;
+				case BatchStage::Raytrace: pl.readyRt.push_back(n); break;
+			}
+		};
+		auto release = [&](Node* n) {
+			for (auto& o : n->getOutputs()) {
+				if (--o->batchLevel == 0) {
+					makeReady(o.get());
+				}
+			}
+		};
+		for (auto& c : pl.ctxs) {
+			for (auto& n : c->validatedOrder()) {
+				if (n->batchLevel == 0) {
+					makeReady(n.get());
+				}
+			}
+		}
+		// Launch descriptors; buffers of every node are allocated here, before the first launch.
+		size_t scheduled = 0;
+		std::vector<Node*> group;
+		while (true) {
+			while (!pl.passive.empty()) {
+				Node* n = pl.passive.back();
+				pl.passive.pop_back();
+				++scheduled;
+				if (n->batchStage() == BatchStage::Other) {
+					newStep().other.push_back(n);
+				}
+				release(n);
+			}
+			if (!pl.readyOp.empty()) {
+				group.swap(pl.readyOp);
+			} else if (!pl.readyCp.empty()) {
+				group.swap(pl.readyCp);
+			} else if (!pl.readyRt.empty()) {
+				group.swap(pl.readyRt);
+			} else {
+				break;
+			}
+			Plan::Level& L = newStep();
+			for (Node* n : group) {
+				switch (n->batchStage()) {
+					case BatchStage::Raytrace: {
+						auto* rt = dynamic_cast<RaytraceNode*>(n);
+						RaytraceBatchItem it = rt->batchItem(s);
+						it.firstBlock = L.rtBlocks;
+						L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount);
+						L.rt.push_back(it);
+						L.rtNodes.push_back(rt);
+						Scene* sc = &rt->getScene();
+						if (std::find(pl.scenes.begin(), pl.scenes.end(), sc) == pl.scenes.end()) {
+							pl.scenes.push_back(sc);
+						}
+						break;
+					}
+					case BatchStage::PointOp: {
+						PointOpItem it{};
+						FieldCopyList layout;
+						n->pointOp(it, layout);
+						it.firstBlock = L.opBlocks;
+						L.opBlocks += gpu::pointOpBlocks(it.capacity);
+						L.op.push_back(it);
+						if (it.op == kOpFormat) {
+							L.opLayout.push_back(static_cast<int>(pl.lists.size()));
+							pl.lists.push_back(layout);
+						} else {
+							L.opLayout.push_back(-1);
+						}
+						break;
+					}
+					case BatchStage::Compact: {
+						auto* cn = dynamic_cast<CompactByFieldPointsNode*>(n);
+						FieldCopyList list;
+						CompactItem it = cn->compactItem(list);
+						it.firstTile = L.cpTiles;
+						L.cpTiles += it.numTiles;
+						L.cp.push_back(it);
+						L.cpList.push_back(static_cast<int>(pl.lists.size()));
+						L.cpNodes.push_back(cn);
+						pl.lists.push_back(list);
+						break;
+					}
+					default: throw std::logic_error("batch: unexpected stage kind");
+				}
+			}
+			scheduled += group.size();
+			for (Node* n : group) {
+				release(n);
+			}
+			group.clear();
+		}
+		if (scheduled != nodeCount) {
+			throw std::logic_error("batch: some nodes were never ready");
+		}
+		const int maxLevel = steps - 1;
+
+		// Host slots: kept counts of compaction nodes, then overflow flags of raytrace nodes.
+		for (int l = 0; l <= maxLevel; ++l) {
+			for (auto& it : pl.levels[l].cp) {
+				pl.gather.push_back(it.countOut);
+			}
+		}
+		for (int l = 0; l <= maxLevel; ++l) {
+			for (auto& it : pl.levels[l].rt) {
+				pl.gather.push_back(it.sc.overflow);
+			}
+		}
+
+		// One staging image: field lists, gather sources, then per-level item arrays.
+		size_t off = 0;
+		auto reserve = [&](size_t bytes) {
+			const size_t o = alignUp(off);
+			off = o + bytes;
+			return o;
+		};
+		const size_t listsOff = reserve(sizeof(FieldCopyList) * pl.lists.size());
+		const size_t gatherOff = reserve(sizeof(const int32_t*) * pl.gather.size());
+		for (int l = 0; l <= maxLevel; ++l) {
+			Plan::Level& L = pl.levels[l];
+			L.rtOff = reserve(sizeof(RaytraceBatchItem) * L.rt.size());
+			L.opOff = reserve(sizeof(PointOpItem) * L.op.size());
+			L.cpOff = reserve(sizeof(CompactItem) * L.cp.size());
+		}
+		const size_t total = std::max<size_t>(off, 1);
+		if (total > items.capacity()) {
+			synchronize(); // earlier batches may still read the descriptors being reallocated
+		}
+		char* dev = static_cast<char*>(items.ensure(total));
+		auto* devLists = reinterpret_cast<const FieldCopyList*>(dev + listsOff);
+		char* host = static_cast<char*>(staging.ensure(total));
+		if (!pl.lists.empty()) {
+			std::memcpy(host + listsOff, pl.lists.data(), sizeof(FieldCopyList) * pl.lists.size());
+		}
+		if (!pl.gather.empty()) {
+			std::memcpy(host + gatherOff, pl.gather.data(), sizeof(const int32_t*) * pl.gather.size());
+		}
+		for (int l = 0; l <= maxLevel; ++l) {
+			Plan::Level& L = pl.levels[l];
+			for (size_t i = 0; i < L.op.size(); ++i) {
+				if (L.opLayout[i] >= 0) {
+					L.op[i].format = devLists + L.opLayout[i];
+				}
+			}
+			for (size_t i = 0; i < L.cp.size(); ++i) {
+				L.cp[i].fields = devLists + L.cpList[i];
+			}
+			if (!L.rt.empty()) {
+				std::memcpy(host + L.rtOff, L.rt.data(), sizeof(RaytraceBatchItem) * L.rt.size());
+			}
+			if (!L.op.empty()) {
+				std::memcpy(host + L.opOff, L.op.data(), sizeof(PointOpItem) * L.op.size());
+			}
+			if (!L.cp.empty()) {
+				std::memcpy(host + L.cpOff, L.cp.data(), sizeof(CompactItem) * L.cp.size());
+			}
+		}
+		RGL_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
+		uploaded->record(s);
+
+		// Launches, level by level; nodes of one level never depend on each other.
+		for (int l = 0; l <= maxLevel; ++l) {
+			Plan::Level& L = pl.levels[l];
+			if (!L.rt.empty()) {
+				const bool timed = timing && L.rtBlocks > 0;
+				if (timed) {
+					if (timingUsed == timingEvents.size()) {
+						hipEvent_t a, b;
+						RGL_HIP(hipEventCreate(&a));
+						RGL_HIP(hipEventCreate(&b));
+						timingEvents.emplace_back(a, b);
+					}
+					RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
+				}
+				gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
+				                   static_cast<int>(L.rt.size()), L.rtBlocks);
+				RGL_HIP(hipGetLastError());
+				if (timed) {
+					RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, s));
+					++timingUsed;
+				}
+			}
+			if (!L.op.empty()) {
+				gpu::pointOpsBatch(s, reinterpret_cast<const PointOpItem*>(dev + L.opOff), static_cast<int>(L.op.size()),
+				                   L.opBlocks);
+				RGL_HIP(hipGetLastError());
+			}
+			if (!L.cp.empty()) {
+				gpu::compactBatch(s, reinterpret_cast<const CompactItem*>(dev + L.cpOff), static_cast<int>(L.cp.size()),
+				                  L.cpTiles);
+				RGL_HIP(hipGetLastError());
+			}
+			for (Node* n : L.other) {
+				n->enqueue();
+			}
+		}
+		for (Scene* sc : pl.scenes) {
+			sc->noteUse(s);
+		}
+
+		// Kept counts and overflow flags of every graph: one gather, one copy.
+		const int g = static_cast<int>(pl.gather.size());
+		if (g > 0) {
+			const size_t bytes = sizeof(int32_t) * g;
+			if (bytes > slotsDev.capacity()) {
+				synchronize();
+			}
+			auto* slots = slotsDev.ensureT<int32_t>(g);
+			gpu::gatherInts(s, g, reinterpret_cast<const int32_t* const*>(dev + gatherOff), slots);
+			RGL_HIP(hipGetLastError());
+			auto hostBuf = hostSlots(bytes);
+			RGL_HIP(hipMemcpyAsync(hostBuf->get(), slots, bytes, hipMemcpyDeviceToHost, s));
+			int k = 0;
+			for (int l = 0; l <= maxLevel; ++l) {
+				for (auto* cn : pl.levels[l].cpNodes) {
+					cn->setCountHost(hostBuf, k++);
+				}
+			}
+			for (int l = 0; l <= maxLevel; ++l) {
+				for (auto* rt : pl.levels[l].rtNodes) {
+					rt->setOverflowHost(hostBuf, k++);
+				}
+			}
+		}
+	} catch (...) {
+		auto ev = completionEvent();
+		ev->record(s);
+		for (auto& c : pl.ctxs) {
+			c->endBatch(ev);
+		}
+		throw;
+	}
+	auto ev = completionEvent();
+	ev->record(s);
+	for (auto& c : pl.ctxs) {
+		for (auto& n : c->validatedOrder()) {
+			n->markEnqueued(ev);
+		}
+		c->endBatch(ev);
+	}
+}
+
+} // namespace rgl

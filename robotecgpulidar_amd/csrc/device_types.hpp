@@ -142,4 +142,58 @@ struct FieldCopyList
 	int32_t count;
 };
 
+// ------------------------------------------------------------------------------------------------ batches
+// One sensor of a batched raytrace launch; blocks [firstBlock, next item's firstBlock) trace its beams.
+struct RaytraceBatchItem
+{
+	SceneView sc;
+	RaytraceParams p;
+	RaytraceOutputs out;
+	int32_t firstBlock;
+};
+
+// Elementwise point / ray stage of one graph inside a batched launch.
+enum PointOpKind : int32_t
+{
+	kOpTransformPoints = 0, // out0 = m0 * in0                                   (nodeKernels.cu:51-56)
+	kOpNoiseHitpoint = 1,   // angular hit-point noise                           (gaussianNoiseKernels.cu:31-47)
+	kOpNoiseDistance = 2,   // distance noise                                    (gaussianNoiseKernels.cu:49-66)
+	kOpTransformRays = 3,   // out0 = m0 * in0 for rays                          (nodeKernels.cu:45-49)
+	kOpNoiseRays = 4,       // angular ray noise                                 (gaussianNoiseKernels.cu:20-29)
+	kOpFormat = 5,          // SoA -> AoS                                         (nodeKernels.cu:25-33)
+};
+
+struct PointOpItem
+{
+	int32_t op;
+	int32_t capacity;
+	const int32_t* count; // live count on the device, nullptr = capacity
+	const void* in0;
+	const void* in1;
+	void* out0;
+	void* out1;
+	Mat3x4f m0; // transform, or look-at transform for noise
+	Mat3x4f m1; // inverse look-at transform for noise
+	float a, b, c; // mean, st dev (base), st dev rise
+	int32_t axis;
+	uint64_t seed;
+	uint32_t run, tag;
+	const FieldCopyList* format; // kOpFormat field layout (device memory)
+	int32_t recordSize;
+	int32_t firstBlock;
+};
+
+// Compaction of one graph inside a batched launch; tiles [firstTile, firstTile + numTiles).
+struct CompactItem
+{
+	int32_t capacity;
+	const int32_t* countIn;
+	const int32_t* flags;
+	const FieldCopyList* fields; // device memory
+	int32_t* tileCounts;
+	int32_t* countOut;
+	int32_t numTiles;
+	int32_t firstTile;
+};
+
 } // namespace rgl

@@ -189,10 +189,21 @@ void Node::enqueue()
 		throw std::logic_error(getName() + ": attempted to call enqueueExec() despite !isValid()");
 	}
 	enqueueImpl();
-	if (!done) {
-		done = std::make_unique<Event>();
+	if (!done || done.use_count() > 1) {
+		done = std::make_shared<Event>();
 	}
 	done->record(stream());
+	ranOnce = true;
+}
+
+void Node::pointOp(PointOpItem&, FieldCopyList&)
+{
+	throw std::logic_error(getName() + " has no elementwise stage");
+}
+
+void Node::markEnqueued(const std::shared_ptr<Event>& ev)
+{
+	done = ev;
 	ranOnce = true;
 }
 
@@ -260,7 +271,7 @@ std::vector<Node::Ptr> GraphRunCtx::findExecutionOrder(std::set<Node::Ptr> nodes
 	return {reverseOrder.rbegin(), reverseOrder.rend()};
 }
 
-void GraphRunCtx::executeAsync()
+const std::vector<Node::Ptr>& GraphRunCtx::validatedOrder()
 {
 	if (order.empty()) {
 		order = findExecutionOrder(nodes);
@@ -268,15 +279,49 @@ void GraphRunCtx::executeAsync()
 	for (auto& n : order) {
 		n->validate();
 	}
+	return order;
+}
+
+void GraphRunCtx::executeAsync()
+{
+	validatedOrder();
+	if (batchDone) {
+		// The previous batch wrote this graph's buffers on the batch stream.
+		RGL_HIP(hipStreamWaitEvent(stream_->get(), batchDone->get(), 0));
+		batchDone.reset();
+	}
 	running = true;
+	lastRunSolo = true;
 	for (auto& n : order) {
 		n->enqueue();
+	}
+}
+
+void GraphRunCtx::beginBatch(hipStream_t batchStream, Event& scratch)
+{
+	if (running && lastRunSolo) {
+		scratch.record(stream_->get());
+		RGL_HIP(hipStreamWaitEvent(batchStream, scratch.get(), 0));
+	}
+	streamOverride = batchStream;
+	running = true;
+	lastRunSolo = false;
+}
+
+void GraphRunCtx::endBatch(std::shared_ptr<Event> completion)
+{
+	streamOverride = nullptr;
+	if (completion) {
+		batchDone = std::move(completion);
 	}
 }
 
 void GraphRunCtx::synchronize()
 {
 	if (running) {
+		if (batchDone) {
+			batchDone->sync();
+		}
 		stream_->sync();
 		running = false;
 	}
@@ -389,10 +434,21 @@ void SetTimeOffsetsRaysNode::validateImpl()
 	}
 }
 
-void TransformRaysNode::enqueueImpl()
+void TransformRaysNode::pointOp(PointOpItem& it, FieldCopyList&)
 {
 	const int n = rayCount();
-	gpu::transformRays(stream(), n, input->rays(), out.ensureT<Mat3x4f>(std::max(n, 1)), transform);
+	it.op = kOpTransformRays;
+	it.capacity = n;
+	it.in0 = input->rays();
+	it.out0 = out.ensureT<Mat3x4f>(std::max(n, 1));
+	it.m0 = transform;
+}
+void TransformRaysNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList unused;
+	pointOp(it, unused);
+	gpu::pointOp(stream(), it);
 }
 
 // Seeds: a process-wide sequence (base from RGL_NOISE_SEED, default fixed) so runs are reproducible.
@@ -417,11 +473,28 @@ void GaussianNoiseAngularRaysNode::setParameters(float m, float s, int a)
 	stDev = s;
 	axis = a;
 }
-void GaussianNoiseAngularRaysNode::enqueueImpl()
+void GaussianNoiseAngularRaysNode::pointOp(PointOpItem& it, FieldCopyList&)
 {
 	const int n = rayCount();
-	gpu::noiseAngularRays(stream(), n, input->rays(), out.ensureT<Mat3x4f>(std::max(n, 1)), mean, stDev, axis,
-	                      input->cumulativeRayTransform().inverse(), rng.seed, rng.run++, rng.tag);
+	it.op = kOpNoiseRays;
+	it.capacity = n;
+	it.in0 = input->rays();
+	it.out0 = out.ensureT<Mat3x4f>(std::max(n, 1));
+	it.m0 = input->cumulativeRayTransform().inverse();
+	it.m1 = it.m0.inverse();
+	it.a = mean;
+	it.b = stDev;
+	it.axis = axis;
+	it.seed = rng.seed;
+	it.run = rng.run++;
+	it.tag = rng.tag;
+}
+void GaussianNoiseAngularRaysNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList unused;
+	pointOp(it, unused);
+	gpu::pointOp(stream(), it);
 }
 
 // ------------------------------------------------------------------------------------------------ raytrace
@@ -455,7 +528,7 @@ void RaytraceNode::validateImpl()
 	if (fields.count(RGL_FIELD_RING_ID_U16) && raysNode->ringIds() == nullptr) {
 		throw InvalidPipeline("requested for field RING_ID_U16, but RaytraceNode cannot get ring ids");
 	}
-	if (fields.count(RGL_FIELD_TIME_STAMP_F64) && !Scene::instance().getTime().has_value()) {
+	if (fields.count(RGL_FIELD_TIME_STAMP_F64) && !scene->getTime().has_value()) {
 		throw InvalidPipeline("requested for field TIME_STAMP_F64, but RaytraceNode cannot get time from scene");
 	}
 	if (distortion && raysNode->timeOffsets() == nullptr) {
@@ -500,7 +573,7 @@ RaytraceParams RaytraceNode::buildParams(const SceneView&) const
 	p.nearNonHitDistance = nearDist;
 	p.farNonHitDistance = farDist;
 	p.defaultIntensity = defaultIntensity;
-	p.sceneDeltaTime = Scene::instance().deltaTimeSeconds();
+	p.sceneDeltaTime = scene->deltaTimeSeconds();
 	p.hHalfDivergence = hHalf;
 	p.vHalfDivergence = vHalf;
 	p.returnMode = static_cast<int32_t>(mode);
@@ -512,7 +585,7 @@ RaytraceParams RaytraceNode::buildParams(const SceneView&) const
 	return p;
 }
 
-void RaytraceNode::enqueueImpl()
+RaytraceBatchItem RaytraceNode::batchItem(hipStream_t s)
 {
 	if (defaultRange.get() == nullptr) {
 		const Vec2f r{0.0f, FLT_MAX};
@@ -529,10 +602,11 @@ void RaytraceNode::enqueueImpl()
 		}
 		b->ensure(fieldSizeChecked(f) * std::max(cap, 1));
 	}
-	const SceneView view = Scene::instance().prepare(stream());
-	const RaytraceParams p = buildParams(view);
+	RaytraceBatchItem it{};
+	it.sc = scene->prepare(s);
+	it.p = buildParams(it.sc);
 	auto ptr = [&](rgl_field_t f) -> void* { return data.count(f) ? data[f]->get() : nullptr; };
-	RaytraceOutputs o{};
+	RaytraceOutputs& o = it.out;
 	o.xyz = static_cast<Vec3f*>(ptr(RGL_FIELD_XYZ_VEC3_F32));
 	o.isHit = static_cast<int32_t*>(ptr(RGL_FIELD_IS_HIT_I32));
 	o.rayIdx = static_cast<uint32_t*>(ptr(RGL_FIELD_RAY_IDX_U32));
@@ -552,6 +626,13 @@ void RaytraceNode::enqueueImpl()
 	o.elevation = static_cast<float*>(ptr(RGL_FIELD_ELEVATION_F32));
 	o.normal = static_cast<Vec3f*>(ptr(RGL_FIELD_NORMAL_VEC3_F32));
 	o.incidentAngle = static_cast<float*>(ptr(RGL_FIELD_INCIDENT_ANGLE_F32));
+	overflowSlots.reset();
+	return it;
+}
+
+void RaytraceNode::enqueueImpl()
+{
+	const RaytraceBatchItem it = batchItem(stream());
 	if (timing) {
 		if (timingUsed == timingEvents.size()) {
 			hipEvent_t a, b;
@@ -561,13 +642,14 @@ void RaytraceNode::enqueueImpl()
 		}
 		RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, stream()));
 	}
-	gpu::raytrace(stream(), view, p, o);
+	gpu::raytrace(stream(), it.sc, it.p, it.out);
 	RGL_HIP(hipGetLastError());
 	if (timing) {
 		RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, stream()));
 		++timingUsed;
 	}
-	RGL_HIP(hipMemcpyAsync(overflowHost.ensure(sizeof(int32_t)), view.overflow, sizeof(int32_t), hipMemcpyDeviceToHost,
+	scene->noteUse(stream());
+	RGL_HIP(hipMemcpyAsync(overflowHost.ensure(sizeof(int32_t)), it.sc.overflow, sizeof(int32_t), hipMemcpyDeviceToHost,
 	                       stream()));
 }
 
@@ -600,7 +682,9 @@ RaytraceNode::~RaytraceNode()
 
 bool RaytraceNode::overflowed()
 {
-	return overflowHost.get() != nullptr && *static_cast<int32_t*>(overflowHost.get()) != 0;
+	const auto* flag = overflowSlots ? static_cast<const int32_t*>(overflowSlots->get()) + overflowSlot
+	                                 : static_cast<const int32_t*>(overflowHost.get());
+	return flag != nullptr && *flag != 0;
 }
 
 const void* RaytraceNode::fieldData(rgl_field_t f)
@@ -617,15 +701,15 @@ const void* RaytraceNode::fieldData(rgl_field_t f)
 
 void RaytraceNode::traversalStats(int32_t* nodes, int32_t* tris, int32_t* insts)
 {
-	const SceneView view = Scene::instance().prepare(stream());
+	const SceneView view = scene->prepare(stream());
 	gpu::traversalStats(stream(), view, buildParams(view), nodes, tris, insts);
 	RGL_HIP(hipGetLastError());
 }
 
 void RaytraceNode::bruteForce(float* t, int32_t* inst, int32_t* prim)
 {
-	const SceneView view = Scene::instance().prepare(stream());
-	const auto counts = Scene::instance().instanceTriCounts();
+	const SceneView view = scene->prepare(stream());
+	const auto counts = scene->instanceTriCounts();
 	DeviceBuffer dc;
 	RGL_HIP(hipMemcpy(dc.ensure(sizeof(int32_t) * std::max<size_t>(counts.size(), 1)), counts.data(),
 	                  sizeof(int32_t) * counts.size(), hipMemcpyHostToDevice));
@@ -635,12 +719,22 @@ void RaytraceNode::bruteForce(float* t, int32_t* inst, int32_t* prim)
 }
 
 // ------------------------------------------------------------------------------------------------ point nodes
-void TransformPointsNode::enqueueImpl()
+void TransformPointsNode::pointOp(PointOpItem& it, FieldCopyList&)
 {
 	const int cap = capacity();
-	gpu::transformPoints(stream(), cap, countDevice(),
-	                     static_cast<const Vec3f*>(input->fieldData(RGL_FIELD_XYZ_VEC3_F32)),
-	                     out.ensureT<Vec3f>(std::max(cap, 1)), transform);
+	it.op = kOpTransformPoints;
+	it.capacity = cap;
+	it.count = countDevice();
+	it.in0 = input->fieldData(RGL_FIELD_XYZ_VEC3_F32);
+	it.out0 = out.ensureT<Vec3f>(std::max(cap, 1));
+	it.m0 = transform;
+}
+void TransformPointsNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList unused;
+	pointOp(it, unused);
+	gpu::pointOp(stream(), it);
 }
 const void* TransformPointsNode::fieldData(rgl_field_t f)
 {
@@ -658,14 +752,30 @@ void GaussianNoiseAngularHitpointNode::validateImpl()
 	IPointsNodeSingleInput::validateImpl();
 	withDistance = input->hasField(RGL_FIELD_DISTANCE_F32);
 }
-void GaussianNoiseAngularHitpointNode::enqueueImpl()
+void GaussianNoiseAngularHitpointNode::pointOp(PointOpItem& it, FieldCopyList&)
 {
 	const int cap = capacity();
-	float* d = withDistance ? outDistance.ensureT<float>(std::max(cap, 1)) : nullptr;
-	gpu::noiseAngularHitpoint(stream(), cap, countDevice(),
-	                          static_cast<const Vec3f*>(input->fieldData(RGL_FIELD_XYZ_VEC3_F32)),
-	                          outXyz.ensureT<Vec3f>(std::max(cap, 1)), d, mean, stDev, axis,
-	                          input->lookAtOriginTransform(), rng.seed, rng.run++, rng.tag);
+	it.op = kOpNoiseHitpoint;
+	it.capacity = cap;
+	it.count = countDevice();
+	it.in0 = input->fieldData(RGL_FIELD_XYZ_VEC3_F32);
+	it.out0 = outXyz.ensureT<Vec3f>(std::max(cap, 1));
+	it.out1 = withDistance ? outDistance.ensureT<float>(std::max(cap, 1)) : nullptr;
+	it.m0 = input->lookAtOriginTransform();
+	it.m1 = it.m0.inverse();
+	it.a = mean;
+	it.b = stDev;
+	it.axis = axis;
+	it.seed = rng.seed;
+	it.run = rng.run++;
+	it.tag = rng.tag;
+}
+void GaussianNoiseAngularHitpointNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList unused;
+	pointOp(it, unused);
+	gpu::pointOp(stream(), it);
 }
 const void* GaussianNoiseAngularHitpointNode::fieldData(rgl_field_t f)
 {
@@ -684,13 +794,31 @@ void GaussianNoiseDistanceNode::setParameters(float m, float b, float r)
 	base = b;
 	rise = r;
 }
-void GaussianNoiseDistanceNode::enqueueImpl()
+void GaussianNoiseDistanceNode::pointOp(PointOpItem& it, FieldCopyList&)
 {
 	const int cap = capacity();
-	gpu::noiseDistance(stream(), cap, countDevice(), static_cast<const Vec3f*>(input->fieldData(RGL_FIELD_XYZ_VEC3_F32)),
-	                   static_cast<const float*>(input->fieldData(RGL_FIELD_DISTANCE_F32)),
-	                   outXyz.ensureT<Vec3f>(std::max(cap, 1)), outDistance.ensureT<float>(std::max(cap, 1)), mean, base,
-	                   rise, input->lookAtOriginTransform(), rng.seed, rng.run++, rng.tag);
+	it.op = kOpNoiseDistance;
+	it.capacity = cap;
+	it.count = countDevice();
+	it.in0 = input->fieldData(RGL_FIELD_XYZ_VEC3_F32);
+	it.in1 = input->fieldData(RGL_FIELD_DISTANCE_F32);
+	it.out0 = outXyz.ensureT<Vec3f>(std::max(cap, 1));
+	it.out1 = outDistance.ensureT<float>(std::max(cap, 1));
+	it.m0 = input->lookAtOriginTransform();
+	it.m1 = it.m0.inverse();
+	it.a = mean;
+	it.b = base;
+	it.c = rise;
+	it.seed = rng.seed;
+	it.run = rng.run++;
+	it.tag = rng.tag;
+}
+void GaussianNoiseDistanceNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList unused;
+	pointOp(it, unused);
+	gpu::pointOp(stream(), it);
 }
 const void* GaussianNoiseDistanceNode::fieldData(rgl_field_t f)
 {
@@ -704,10 +832,10 @@ const void* GaussianNoiseDistanceNode::fieldData(rgl_field_t f)
 }
 
 // src/graph/CompactByFieldPointsNode.cpp:33-105; every field of the input is compacted in one pass.
-void CompactByFieldPointsNode::enqueueImpl()
+CompactItem CompactByFieldPointsNode::compactItem(FieldCopyList& list)
 {
 	const int cap = capacity();
-	FieldCopyList list{};
+	list = FieldCopyList{};
 	std::set<rgl_field_t> keep;
 	for (auto f : input->fieldList()) {
 		if (isPadding(f) || f == RGL_FIELD_DYNAMIC_FORMAT) {
@@ -729,17 +857,32 @@ void CompactByFieldPointsNode::enqueueImpl()
 	for (auto it = data.begin(); it != data.end();) {
 		it = keep.count(it->first) ? std::next(it) : data.erase(it);
 	}
-	count.ensure(sizeof(int32_t));
-	tileCounts.ensure(sizeof(int32_t) * gpu::compactTiles(cap));
-	const auto* flags = cap > 0 ? static_cast<const int32_t*>(input->fieldData(by)) : nullptr;
-	gpu::compact(stream(), cap, input->countDevice(), flags, list, tileCounts.as<int32_t>(), count.as<int32_t>());
-	RGL_HIP(hipMemcpyAsync(countHost.ensure(sizeof(int32_t)), count.get(), sizeof(int32_t), hipMemcpyDeviceToHost,
+	CompactItem it{};
+	it.capacity = cap;
+	it.countIn = input->countDevice();
+	it.flags = cap > 0 ? static_cast<const int32_t*>(input->fieldData(by)) : nullptr;
+	it.numTiles = gpu::compactTiles(cap);
+	it.tileCounts = tileCounts.ensureT<int32_t>(it.numTiles);
+	it.countOut = count.ensureT<int32_t>(1);
+	countSlots.reset();
+	return it;
+}
+
+void CompactByFieldPointsNode::enqueueImpl()
+{
+	FieldCopyList list;
+	const CompactItem it = compactItem(list);
+	gpu::compact(stream(), it.capacity, it.countIn, it.flags, list, it.tileCounts, it.countOut);
+	RGL_HIP(hipMemcpyAsync(countHost.ensure(sizeof(int32_t)), it.countOut, sizeof(int32_t), hipMemcpyDeviceToHost,
 	                       stream()));
 }
 
 int CompactByFieldPointsNode::hostCount()
 {
 	waitForResults();
+	if (countSlots) {
+		return static_cast<const int32_t*>(countSlots->get())[countSlot];
+	}
 	return *static_cast<int32_t*>(countHost.get());
 }
 
@@ -779,10 +922,8 @@ size_t FormatPointsNode::fieldPointSize(rgl_field_t f) const
 {
 	return f == RGL_FIELD_DYNAMIC_FORMAT ? pointSize(fields) : fieldSizeChecked(f);
 }
-void FormatPointsNode::enqueueImpl()
+FieldCopyList FormatPointsNode::makeLayout()
 {
-	const int cap = capacity();
-	const size_t rec = pointSize(fields);
 	FieldCopyList list{};
 	size_t off = 0;
 	for (auto f : fields) {
@@ -794,8 +935,25 @@ void FormatPointsNode::enqueueImpl()
 		list.f[list.count++] = FieldCopy{src, nullptr, static_cast<int32_t>(sz), static_cast<int32_t>(off)};
 		off += sz;
 	}
-	gpu::formatSoaToAos(stream(), cap, countDevice(), static_cast<int>(rec), list,
-	                    out.ensureT<char>(std::max<size_t>(rec * cap, 1)));
+	return list;
+}
+void FormatPointsNode::pointOp(PointOpItem& it, FieldCopyList& layout)
+{
+	const int cap = capacity();
+	const size_t rec = pointSize(fields);
+	layout = makeLayout();
+	it.op = kOpFormat;
+	it.capacity = cap;
+	it.count = countDevice();
+	it.out0 = out.ensureT<char>(std::max<size_t>(rec * cap, 1));
+	it.recordSize = static_cast<int32_t>(rec);
+}
+void FormatPointsNode::enqueueImpl()
+{
+	PointOpItem it{};
+	FieldCopyList layout;
+	pointOp(it, layout);
+	gpu::formatSoaToAos(stream(), it.capacity, it.count, it.recordSize, layout, static_cast<char*>(it.out0));
 }
 const void* FormatPointsNode::fieldData(rgl_field_t f)
 {

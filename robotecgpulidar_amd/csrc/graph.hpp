@@ -25,6 +25,16 @@ namespace rgl {
 
 struct GraphRunCtx;
 
+// How a node takes part in a batched run.
+enum class BatchStage
+{
+	Passive,  // nothing to launch (configuration or view nodes)
+	Raytrace, // one item of the batched raytrace launch
+	PointOp,  // one item of the batched elementwise launch
+	Compact,  // one item of the batched compaction launches
+	Other,    // enqueued on its own, on the batch stream
+};
+
 struct Node : std::enable_shared_from_this<Node>
 {
 	using Ptr = std::shared_ptr<Node>;
@@ -46,6 +56,8 @@ struct Node : std::enable_shared_from_this<Node>
 
 	void validate();
 	void enqueue();
+	// Marks the node's work as enqueued by a batch launch whose completion `ev` records.
+	void markEnqueued(const std::shared_ptr<Event>& ev);
 	// Waits until this node's device work of the last run is complete; throws if the node never ran.
 	void waitForResults();
 	void markDirty() { dirty = true; }
@@ -53,6 +65,11 @@ struct Node : std::enable_shared_from_this<Node>
 
 	virtual std::string getName() const = 0;
 	hipStream_t stream() const;
+
+	virtual BatchStage batchStage() const { return BatchStage::Other; }
+	// Elementwise stages: fills the launch description of this run (firstBlock unset; `layout` is the field layout
+	// of a format stage, which `it.format` must point to in device memory).
+	virtual void pointOp(PointOpItem& it, FieldCopyList& layout);
 
 	template<typename T>
 	std::shared_ptr<T> exactlyOneInputOfType() const;
@@ -67,8 +84,10 @@ protected:
 	bool dirty = true;
 	bool ranOnce = false;
 	std::shared_ptr<GraphRunCtx> ctx;
-	std::unique_ptr<Event> done;
+	std::shared_ptr<Event> done; // shared by every node of a batch
+	int32_t batchLevel = 0;      // longest path from an entry node (batched runs)
 	friend struct GraphRunCtx;
+	friend struct GraphBatch;
 };
 
 // ------------------------------------------------------------------------------------------------ rays
@@ -155,8 +174,15 @@ struct GraphRunCtx : std::enable_shared_from_this<GraphRunCtx>
 	void detachAndDestroy();
 	void markNodesDirty();
 	void clearExecutionOrder() { order.clear(); }
-	hipStream_t stream() const { return stream_->get(); }
+	hipStream_t stream() const { return streamOverride ? streamOverride : stream_->get(); }
+	hipStream_t ownStream() const { return stream_->get(); }
 	const std::set<Node::Ptr>& getNodes() const { return nodes; }
+	// Validated execution order (computed on first use).
+	const std::vector<Node::Ptr>& validatedOrder();
+	// Batched runs: orders `batchStream` after this graph's earlier work on its own stream and makes every node
+	// enqueue on `batchStream` until endBatch(), which hands the batch's completion event to later runs.
+	void beginBatch(hipStream_t batchStream, Event& scratch);
+	void endBatch(std::shared_ptr<Event> completion);
 
 	static std::vector<Node::Ptr> findExecutionOrder(std::set<Node::Ptr> nodes);
 
@@ -164,7 +190,40 @@ private:
 	std::set<Node::Ptr> nodes;
 	std::vector<Node::Ptr> order;
 	std::unique_ptr<Stream> stream_ = std::make_unique<Stream>();
+	hipStream_t streamOverride = nullptr;
 	bool running = false;
+	bool lastRunSolo = false;
+	std::shared_ptr<Event> batchDone; // completion of the last batch this graph ran in
+};
+
+// Runs several independent graphs as one batch: stages of the same kind at the same depth of every graph execute in
+// one launch (raytrace, elementwise point/ray stages, compaction), on one stream.
+struct GraphBatch
+{
+	static GraphBatch& instance();
+	// Runs the graphs that contain `entryNodes` (any node of each graph; duplicates are allowed).
+	void run(const std::vector<Node::Ptr>& entryNodes);
+	void synchronize();
+	// HIP-event timing of the batched raytrace launches while enabled.
+	void setTiming(bool enable);
+	void getTiming(double* totalMs, int32_t* launches);
+
+private:
+	struct Plan;
+	std::shared_ptr<PinnedBuffer> hostSlots(size_t bytes);
+	std::shared_ptr<Event> completionEvent();
+
+	std::unique_ptr<Plan> plan;
+	std::unique_ptr<Stream> stream_;
+	std::unique_ptr<Event> uploaded, scratchEvent;
+	PinnedBuffer staging;
+	DeviceBuffer items;
+	DeviceBuffer slotsDev;
+	std::vector<std::shared_ptr<PinnedBuffer>> slotPool; // host copies of kept counts and overflow flags
+	std::vector<std::shared_ptr<Event>> eventPool;
+	bool timing = false;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
+	size_t timingUsed = 0;
 };
 
 // ------------------------------------------------------------------------------------------------ node kinds
@@ -172,6 +231,7 @@ struct FromMat3x4fRaysNode : IRaysNode
 {
 	void setParameters(const Mat3x4f* r, int count);
 	std::string getName() const override { return "FromMat3x4fRaysNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	int rayCount() const override { return count; }
 	const Mat3x4f* rays() const override { return buf.as<Mat3x4f>(); }
 
@@ -188,6 +248,7 @@ struct SetRingIdsRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const int32_t* ids, int count);
 	std::string getName() const override { return "SetRingIdsRaysNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const int32_t* ringIds() const override { return buf.as<int32_t>(); }
 	int ringIdsCount() const override { return count; }
 
@@ -204,6 +265,7 @@ struct SetRangeRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const Vec2f* r, int count);
 	std::string getName() const override { return "SetRangeRaysNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const Vec2f* ranges() const override { return buf.as<Vec2f>(); }
 	int rangesCount() const override { return count; }
 
@@ -220,6 +282,7 @@ struct SetTimeOffsetsRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const float* offsets, int count);
 	std::string getName() const override { return "SetTimeOffsetsRaysNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const float* timeOffsets() const override { return buf.as<float>(); }
 	int timeOffsetsCount() const override { return count; }
 
@@ -238,6 +301,8 @@ struct TransformRaysNode : IRaysNodeSingleInput
 	std::string getName() const override { return "TransformRaysNode"; }
 	const Mat3x4f* rays() const override { return out.as<Mat3x4f>(); }
 	Mat3x4f cumulativeRayTransform() const override { return transform * input->cumulativeRayTransform(); }
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 
 protected:
 	void enqueueImpl() override;
@@ -262,6 +327,8 @@ struct GaussianNoiseAngularRaysNode : IRaysNodeSingleInput
 	void setParameters(float mean, float stDev, int axis);
 	std::string getName() const override { return "GaussianNoiseAngularRaysNode"; }
 	const Mat3x4f* rays() const override { return out.as<Mat3x4f>(); }
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 	NoiseStream rng;
 
 protected:
@@ -275,8 +342,10 @@ private:
 
 struct RaytraceNode : IPointsNode
 {
-	void setParameters() {}
+	void setParameters(std::shared_ptr<Scene> s) { scene = std::move(s); }
+	Scene& getScene() const { return *scene; }
 	std::string getName() const override { return "RaytraceNode"; }
+	BatchStage batchStage() const override { return BatchStage::Raytrace; }
 	bool isDense() const override { return false; }
 	bool hasField(rgl_field_t f) const override { return fields.count(f) != 0; }
 	std::vector<rgl_field_t> fieldList() const override { return {fields.begin(), fields.end()}; }
@@ -315,6 +384,14 @@ struct RaytraceNode : IPointsNode
 	void bruteForce(float* t, int32_t* inst, int32_t* prim);
 	RaytraceParams buildParams(const SceneView& view) const;
 	bool overflowed();
+	// Prepares buffers and the scene on `s` and returns this node's launch description (firstBlock unset).
+	RaytraceBatchItem batchItem(hipStream_t s);
+	// Host copy of the overflow flag of a batched run: slot `index` of `slots`.
+	void setOverflowHost(std::shared_ptr<PinnedBuffer> slots, int index)
+	{
+		overflowSlots = std::move(slots);
+		overflowSlot = index;
+	}
 	// HIP-event timing of the raytrace kernel launches (on the graph stream) while enabled.
 	void setTiming(bool enable);
 	void getTiming(double* totalMs, int32_t* launches);
@@ -328,6 +405,7 @@ private:
 	std::set<rgl_field_t> findFieldsToCompute();
 
 	std::shared_ptr<IRaysNode> raysNode;
+	std::shared_ptr<Scene> scene = Scene::defaultShared();
 	std::set<rgl_field_t> fields;
 	std::map<rgl_field_t, std::unique_ptr<DeviceBuffer>> data;
 	DeviceBuffer defaultRange;
@@ -341,6 +419,8 @@ private:
 	rgl_return_mode_t mode = RGL_RETURN_FIRST;
 	float hHalf = 0.0f, vHalf = 0.0f;
 	PinnedBuffer overflowHost;
+	std::shared_ptr<PinnedBuffer> overflowSlots;
+	int overflowSlot = 0;
 	bool timing = false;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
 	size_t timingUsed = 0;
@@ -353,6 +433,8 @@ struct TransformPointsNode : IPointsNodeSingleInput
 	std::vector<rgl_field_t> requiredFields() const override { return {RGL_FIELD_XYZ_VEC3_F32}; }
 	Mat3x4f lookAtOriginTransform() const override { return transform.inverse() * input->lookAtOriginTransform(); }
 	const void* fieldData(rgl_field_t f) override;
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 
 protected:
 	void enqueueImpl() override;
@@ -369,6 +451,8 @@ struct GaussianNoiseAngularHitpointNode : IPointsNodeSingleInput
 	std::string getName() const override { return "GaussianNoiseAngularHitpointNode"; }
 	std::vector<rgl_field_t> requiredFields() const override { return {RGL_FIELD_XYZ_VEC3_F32}; }
 	const void* fieldData(rgl_field_t f) override;
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 	NoiseStream rng;
 
 protected:
@@ -392,6 +476,8 @@ struct GaussianNoiseDistanceNode : IPointsNodeSingleInput
 		return {RGL_FIELD_XYZ_VEC3_F32, RGL_FIELD_DISTANCE_F32};
 	}
 	const void* fieldData(rgl_field_t f) override;
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 	NoiseStream rng;
 
 protected:
@@ -406,11 +492,20 @@ struct CompactByFieldPointsNode : IPointsNodeSingleInput
 {
 	void setParameters(rgl_field_t f) { by = f; }
 	std::string getName() const override { return "CompactByFieldPointsNode"; }
+	BatchStage batchStage() const override { return BatchStage::Compact; }
 	std::vector<rgl_field_t> requiredFields() const override { return {by}; }
 	bool isDense() const override { return true; }
 	const int32_t* countDevice() const override { return count.as<int32_t>(); }
 	int hostCount() override;
 	const void* fieldData(rgl_field_t f) override;
+	// Prepares output buffers and returns this node's compaction description (firstTile unset).
+	CompactItem compactItem(FieldCopyList& list);
+	// Host copy of the kept count of a batched run: slot `index` of `slots`.
+	void setCountHost(std::shared_ptr<PinnedBuffer> slots, int index)
+	{
+		countSlots = std::move(slots);
+		countSlot = index;
+	}
 
 protected:
 	void enqueueImpl() override;
@@ -420,12 +515,15 @@ private:
 	std::map<rgl_field_t, std::unique_ptr<DeviceBuffer>> data;
 	DeviceBuffer count, tileCounts;
 	PinnedBuffer countHost;
+	std::shared_ptr<PinnedBuffer> countSlots;
+	int countSlot = 0;
 };
 
 struct YieldPointsNode : IPointsNodeSingleInput
 {
 	void setParameters(const std::vector<rgl_field_t>& f);
 	std::string getName() const override { return "YieldPointsNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	std::vector<rgl_field_t> requiredFields() const override { return fields; }
 	bool hasField(rgl_field_t f) const override;
 
@@ -444,11 +542,14 @@ struct FormatPointsNode : IPointsNodeSingleInput
 	bool hasField(rgl_field_t f) const override;
 	const void* fieldData(rgl_field_t f) override;
 	size_t fieldPointSize(rgl_field_t f) const override;
+	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
 
 protected:
 	void enqueueImpl() override;
 
 private:
+	FieldCopyList makeLayout();
 	std::vector<rgl_field_t> fields;
 	DeviceBuffer out;
 };
@@ -457,6 +558,7 @@ struct FromArrayPointsNode : IPointsNode
 {
 	void setParameters(const void* points, int count, const std::vector<rgl_field_t>& f);
 	std::string getName() const override { return "FromArrayPointsNode"; }
+	BatchStage batchStage() const override { return BatchStage::Passive; }
 	bool isDense() const override { return data.count(RGL_FIELD_IS_HIT_I32) == 0; }
 	bool hasField(rgl_field_t f) const override { return data.count(f) != 0; }
 	std::vector<rgl_field_t> fieldList() const override;

@@ -43,6 +43,12 @@ void traversalStats(hipStream_t stream, const SceneView& scene, const RaytracePa
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
                        float* outT, int32_t* outInstance, int32_t* outPrim);
 void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
+int raytraceBlocks(int rayCount);
+// Batched launches: `items` live in device memory.
+void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks);
+int pointOpBlocks(int capacity);
+void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks);
+void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles);
 
 // ---------------------------------------------------------------- point-cloud kernels (points.hip)
 // Point clouds are processed over `capacity` elements; `count` (device, may be nullptr) holds the live count
@@ -50,16 +56,10 @@ void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, c
 int compactTiles(int capacity);
 void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
              int32_t* tileCounts, int32_t* countOut);
-void transformPoints(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* in, Vec3f* out,
-                     const Mat3x4f& tf);
-void noiseAngularRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, float mean, float stDev, int axis,
-                      const Mat3x4f& lookAt, uint64_t seed, uint32_t run, uint32_t tag);
-void noiseAngularHitpoint(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, Vec3f* outXyz,
-                          float* outDistance, float mean, float stDev, int axis, const Mat3x4f& lookAt, uint64_t seed,
-                          uint32_t run, uint32_t tag);
-void noiseDistance(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, const float* inDistance,
-                   Vec3f* outXyz, float* outDistance, float mean, float stDevBase, float stDevRise, const Mat3x4f& lookAt,
-                   uint64_t seed, uint32_t run, uint32_t tag);
+// One elementwise stage (any PointOpKind but kOpFormat) over it.capacity elements (it.count live on the device).
+void pointOp(hipStream_t stream, const PointOpItem& it);
+// dst[i] = *src[i] for i < n (`src` lives in device memory).
+void gatherInts(hipStream_t stream, int n, const int32_t* const* src, int32_t* dst);
 void noiseNormals(hipStream_t stream, int n, float* out, uint64_t seed, uint32_t run, uint32_t tag);
 // SoA -> AoS (format) and AoS -> SoA (from-array) over `capacity` records of `recordSize` bytes.
 void formatSoaToAos(hipStream_t stream, int capacity, const int32_t* count, int recordSize, const FieldCopyList& fields,

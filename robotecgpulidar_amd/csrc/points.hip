@@ -1,13 +1,18 @@
-// Point-cloud kernels: stream compaction, point transform, Gaussian noise, SoA<->AoS formatting.
+// Point-cloud kernels: stream compaction, point transform, Gaussian noise, SoA<->AoS formatting, and their batched
+// (many graphs per launch) forms.
 //
 // Compaction replaces thrust::inclusive_scan + one kApplyCompaction launch per field
 // (src/gpu/nodeKernels.cu:58-68,424-436; src/graph/CompactByFieldPointsNode.cpp:33-99) with two launches in
 // total: per-tile counts, then one pass that finds each tile's offset, ranks the kept elements inside the tile
-// and scatters every requested field. Elements are kept in input order (stable), and the kept count stays on
-// the device so nothing downstream waits for the host.
-// Noise kernels follow src/gpu/gaussianNoiseKernels.cu:20-66 with the seeded stream of noise_rng.hpp.
+// with wave ballots and scatters every requested field. Elements are kept in input order (stable), and the kept
+// count stays on the device so nothing downstream waits for the host.
+// Noise follows src/gpu/gaussianNoiseKernels.cu:20-66 with the seeded stream of noise_rng.hpp.
+// Single-graph and batched kernels call the same element functions, so a graph's results do not depend on
+// whether it ran alone or in a batch.
 
 #include <hip/hip_runtime.h>
+
+#include <stdexcept>
 
 #include "device_types.hpp"
 #include "kernels.hpp"
@@ -20,14 +25,16 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems; // 2048 points per compaction tile
+constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ int liveCount(int capacity, const int32_t* count) { return count ? *count : capacity; }
 
-__global__ void kTileCounts(int capacity, const int32_t* countIn, const int32_t* flags, int32_t* tileCounts)
+// ------------------------------------------------------------------------------------------------ compaction
+__device__ void tileCount(int capacity, const int32_t* countIn, const int32_t* flags, int tile, int32_t* tileCounts)
 {
-	__shared__ int32_t warpSums[kBlock / 64];
+	__shared__ int32_t warpSums[kWaves];
 	const int n = liveCount(capacity, countIn);
-	const int base = blockIdx.x * kTile;
+	const int base = tile * kTile;
 	int c = 0;
 	for (int k = 0; k < kItems; ++k) {
 		const int i = base + k * kBlock + threadIdx.x;
@@ -42,10 +49,10 @@ __global__ void kTileCounts(int capacity, const int32_t* countIn, const int32_t*
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		int s = 0;
-		for (int w = 0; w < kBlock / 64; ++w) {
+		for (int w = 0; w < kWaves; ++w) {
 			s += warpSums[w];
 		}
-		tileCounts[blockIdx.x] = s;
+		tileCounts[tile] = s;
 	}
 }
 
@@ -73,18 +80,17 @@ __device__ __forceinline__ void copyElem(const FieldCopy& f, int src, int dst)
 	}
 }
 
-// Offset of this tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so
-// each of the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave)
-// counts in LDS, which preserves input order.
-__global__ void kCompactScatter(int capacity, const int32_t* countIn, const int32_t* flags, FieldCopyList fields,
-                                const int32_t* tileCounts, int numTiles, int32_t* countOut)
+// Offset of tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so each of
+// the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave) counts in
+// LDS, which preserves input order.
+__device__ void compactScatter(int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
+                               const int32_t* tileCounts, int tile, int numTiles, int32_t* countOut)
 {
 	__shared__ int32_t red[kBlock];
-	__shared__ int32_t waveBase[kItems][kBlock / 64];
+	__shared__ int32_t waveBase[kItems][kWaves];
 	const int n = liveCount(capacity, countIn);
-	// 1. tile offset
 	int s = 0;
-	for (int t = threadIdx.x; t < static_cast<int>(blockIdx.x); t += kBlock) {
+	for (int t = threadIdx.x; t < tile; t += kBlock) {
 		s += tileCounts[t];
 	}
 	red[threadIdx.x] = s;
@@ -96,14 +102,13 @@ __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int3
 		__syncthreads();
 	}
 	const int tileOffset = red[0];
-	if (blockIdx.x == numTiles - 1 && threadIdx.x == 0) {
-		*countOut = tileOffset + tileCounts[blockIdx.x];
+	if (tile == numTiles - 1 && threadIdx.x == 0) {
+		*countOut = tileOffset + tileCounts[tile];
 	}
-	// 2. flags, ballots and per-(round, wave) counts
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	const int base = blockIdx.x * kTile + threadIdx.x;
+	const int base = tile * kTile + threadIdx.x;
 	uint64_t masks[kItems];
 	for (int k = 0; k < kItems; ++k) {
 		const int i = base + k * kBlock;
@@ -116,7 +121,7 @@ __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int3
 	if (threadIdx.x == 0) {
 		int run = tileOffset;
 		for (int k = 0; k < kItems; ++k) {
-			for (int w = 0; w < kBlock / 64; ++w) {
+			for (int w = 0; w < kWaves; ++w) {
 				const int c = waveBase[k][w];
 				waveBase[k][w] = run;
 				run += c;
@@ -124,46 +129,83 @@ __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int3
 		}
 	}
 	__syncthreads();
-	// 3. scatter every field
+	const int fieldCount = fields.count;
 	for (int k = 0; k < kItems; ++k) {
 		if ((masks[k] >> lane) & 1ull) {
 			const int pos = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
-			for (int f = 0; f < fields.count; ++f) {
+			for (int f = 0; f < fieldCount; ++f) {
 				copyElem(fields.f[f], base + k * kBlock, pos);
 			}
 		}
 	}
 }
 
-__global__ void kTransformPoints(int capacity, const int32_t* count, const Vec3f* in, Vec3f* out, Mat3x4f tf)
+__global__ void kTileCounts(int capacity, const int32_t* countIn, const int32_t* flags, int32_t* tileCounts)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i < liveCount(capacity, count)) {
-		out[i] = tf * in[i];
+	tileCount(capacity, countIn, flags, blockIdx.x, tileCounts);
+}
+
+__global__ void kCompactScatter(int capacity, const int32_t* countIn, const int32_t* flags, FieldCopyList fields,
+                                const int32_t* tileCounts, int numTiles, int32_t* countOut)
+{
+	compactScatter(capacity, countIn, flags, fields, tileCounts, blockIdx.x, numTiles, countOut);
+}
+
+template<typename Item, typename Key>
+__device__ __forceinline__ int findItem(const Item* items, int itemCount, Key key)
+{
+	int lo = 0, hi = itemCount - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (key(items[mid]) <= static_cast<int>(blockIdx.x)) {
+			lo = mid;
+		} else {
+			hi = mid - 1;
+		}
 	}
+	return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ void kTileCountsBatch(const CompactItem* items, int itemCount)
+{
+	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
+	const CompactItem& it = items[i];
+	tileCount(it.capacity, it.countIn, it.flags, static_cast<int>(blockIdx.x) - it.firstTile, it.tileCounts);
+}
+
+__global__ void kCompactScatterBatch(const CompactItem* items, int itemCount)
+{
+	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
+	const CompactItem& it = items[i];
+	compactScatter(it.capacity, it.countIn, it.flags, *it.fields, it.tileCounts,
+	               static_cast<int>(blockIdx.x) - it.firstTile, it.numTiles, it.countOut);
+}
+
+// ------------------------------------------------------------------------------------------------ element ops
+__device__ __forceinline__ void opTransformPoints(int i, const Vec3f* in, Vec3f* out, const Mat3x4f& tf)
+{
+	out[i] = tf * in[i];
+}
+
+__device__ __forceinline__ void opTransformRays(int i, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf)
+{
+	out[i] = tf * in[i];
 }
 
 // src/gpu/gaussianNoiseKernels.cu:20-29
-__global__ void kNoiseAngularRays(int n, const Mat3x4f* in, Mat3x4f* out, float mean, float stDev, int axis,
-                                  Mat3x4f lookAt, Mat3x4f lookAtInv, uint64_t seed, uint32_t run, uint32_t tag)
+__device__ __forceinline__ void opNoiseRays(int i, const Mat3x4f* in, Mat3x4f* out, float mean, float stDev, int axis,
+                                            const Mat3x4f& lookAt, const Mat3x4f& lookAtInv, uint64_t seed, uint32_t run,
+                                            uint32_t tag)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= n) {
-		return;
-	}
 	const float err = mean + rngNormal(seed, run, tag, static_cast<uint32_t>(i)) * stDev;
 	out[i] = lookAtInv * (Mat3x4f::rotationAxisRad(axis, err) * (lookAt * in[i]));
 }
 
 // src/gpu/gaussianNoiseKernels.cu:31-47
-__global__ void kNoiseAngularHitpoint(int capacity, const int32_t* count, const Vec3f* inXyz, Vec3f* outXyz,
-                                      float* outDistance, float mean, float stDev, int axis, Mat3x4f lookAt,
-                                      Mat3x4f lookAtInv, uint64_t seed, uint32_t run, uint32_t tag)
+__device__ __forceinline__ void opNoiseHitpoint(int i, const Vec3f* inXyz, Vec3f* outXyz, float* outDistance, float mean,
+                                                float stDev, int axis, const Mat3x4f& lookAt, const Mat3x4f& lookAtInv,
+                                                uint64_t seed, uint32_t run, uint32_t tag)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= liveCount(capacity, count)) {
-		return;
-	}
 	const float err = mean + rngNormal(seed, run, tag, static_cast<uint32_t>(i)) * stDev;
 	const Vec3f q = Mat3x4f::rotationAxisRad(axis, err) * (lookAt * inXyz[i]);
 	if (outDistance != nullptr) {
@@ -173,14 +215,11 @@ __global__ void kNoiseAngularHitpoint(int capacity, const int32_t* count, const 
 }
 
 // src/gpu/gaussianNoiseKernels.cu:49-66
-__global__ void kNoiseDistance(int capacity, const int32_t* count, const Vec3f* inXyz, const float* inDistance,
-                               Vec3f* outXyz, float* outDistance, float mean, float stDevBase, float stDevRise,
-                               Mat3x4f lookAt, Mat3x4f lookAtInv, uint64_t seed, uint32_t run, uint32_t tag)
+__device__ __forceinline__ void opNoiseDistance(int i, const Vec3f* inXyz, const float* inDistance, Vec3f* outXyz,
+                                                float* outDistance, float mean, float stDevBase, float stDevRise,
+                                                const Mat3x4f& lookAt, const Mat3x4f& lookAtInv, uint64_t seed,
+                                                uint32_t run, uint32_t tag)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= liveCount(capacity, count)) {
-		return;
-	}
 	const float sigma = inDistance[i] * stDevRise + stDevBase;
 	const float err = mean + rngNormal(seed, run, tag, static_cast<uint32_t>(i)) * sigma;
 	const Vec3f pr = lookAt * inXyz[i];
@@ -188,20 +227,8 @@ __global__ void kNoiseDistance(int capacity, const int32_t* count, const Vec3f* 
 	outDistance[i] = inDistance[i] + err;
 }
 
-__global__ void kNoiseNormals(int n, float* out, uint64_t seed, uint32_t run, uint32_t tag)
+__device__ __forceinline__ void opSoaToAos(int i, int recordSize, const FieldCopyList& fields, char* out)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i < n) {
-		out[i] = rngNormal(seed, run, tag, static_cast<uint32_t>(i));
-	}
-}
-
-__global__ void kSoaToAos(int capacity, const int32_t* count, int recordSize, FieldCopyList fields, char* out)
-{
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= liveCount(capacity, count)) {
-		return;
-	}
 	char* rec = out + static_cast<size_t>(i) * recordSize;
 	for (int f = 0; f < fields.count; ++f) {
 		const FieldCopy& fc = fields.f[f];
@@ -215,6 +242,61 @@ __global__ void kSoaToAos(int capacity, const int32_t* count, int recordSize, Fi
 		for (int b = 0; b < fc.size; ++b) {
 			rec[fc.offset + b] = s[b];
 		}
+	}
+}
+
+// One element of an elementwise stage; shared by the single-graph and batched launches.
+__device__ __forceinline__ void applyOp(const PointOpItem& it, int i)
+{
+	switch (it.op) {
+		case kOpTransformPoints:
+			opTransformPoints(i, static_cast<const Vec3f*>(it.in0), static_cast<Vec3f*>(it.out0), it.m0);
+			break;
+		case kOpNoiseHitpoint:
+			opNoiseHitpoint(i, static_cast<const Vec3f*>(it.in0), static_cast<Vec3f*>(it.out0),
+			                static_cast<float*>(it.out1), it.a, it.b, it.axis, it.m0, it.m1, it.seed, it.run, it.tag);
+			break;
+		case kOpNoiseDistance:
+			opNoiseDistance(i, static_cast<const Vec3f*>(it.in0), static_cast<const float*>(it.in1),
+			                static_cast<Vec3f*>(it.out0), static_cast<float*>(it.out1), it.a, it.b, it.c, it.m0, it.m1,
+			                it.seed, it.run, it.tag);
+			break;
+		case kOpTransformRays:
+			opTransformRays(i, static_cast<const Mat3x4f*>(it.in0), static_cast<Mat3x4f*>(it.out0), it.m0);
+			break;
+		case kOpNoiseRays:
+			opNoiseRays(i, static_cast<const Mat3x4f*>(it.in0), static_cast<Mat3x4f*>(it.out0), it.a, it.b, it.axis,
+			            it.m0, it.m1, it.seed, it.run, it.tag);
+			break;
+		case kOpFormat:
+			opSoaToAos(i, it.recordSize, *it.format, static_cast<char*>(it.out0));
+			break;
+		default:
+			break;
+	}
+}
+
+__global__ void kPointOp(PointOpItem it)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < liveCount(it.capacity, it.count)) {
+		applyOp(it, i);
+	}
+}
+
+__global__ void kNoiseNormals(int n, float* out, uint64_t seed, uint32_t run, uint32_t tag)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) {
+		out[i] = rngNormal(seed, run, tag, static_cast<uint32_t>(i));
+	}
+}
+
+__global__ void kSoaToAos(int capacity, const int32_t* count, int recordSize, FieldCopyList fields, char* out)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < liveCount(capacity, count)) {
+		opSoaToAos(i, recordSize, fields, out);
 	}
 }
 
@@ -237,6 +319,26 @@ __global__ void kAosToSoa(int n, int recordSize, const char* in, FieldCopyList f
 	}
 }
 
+// Elementwise stages of many graphs in one launch.
+__global__ void kPointOpsBatch(const PointOpItem* items, int itemCount)
+{
+	const int k = findItem(items, itemCount, [](const PointOpItem& p) { return p.firstBlock; });
+	const PointOpItem& it = items[k];
+	const int i = (static_cast<int>(blockIdx.x) - it.firstBlock) * kBlock + threadIdx.x;
+	if (i < liveCount(it.capacity, it.count)) {
+		applyOp(it, i);
+	}
+}
+
+// dst[i] = *src[i]: collects the device-resident counts of many graphs for one copy to the host.
+__global__ void kGatherInts(int n, const int32_t* const* src, int32_t* dst)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) {
+		dst[i] = *src[i];
+	}
+}
+
 __global__ void kFillInt(int n, int32_t* p, int32_t v)
 {
 	const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -250,6 +352,7 @@ inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 } // namespace
 
 int compactTiles(int capacity) { return capacity <= 0 ? 1 : (capacity + kTile - 1) / kTile; }
+int pointOpBlocks(int capacity) { return capacity <= 0 ? 1 : blocks(capacity); }
 
 void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
              int32_t* tileCounts, int32_t* countOut)
@@ -264,41 +367,35 @@ void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int
 	                   tiles, countOut);
 }
 
-void transformPoints(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* in, Vec3f* out,
-                     const Mat3x4f& tf)
+void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles)
 {
-	if (capacity > 0) {
-		hipLaunchKernelGGL(kTransformPoints, dim3(blocks(capacity)), dim3(kBlock), 0, stream, capacity, count, in, out, tf);
+	if (itemCount > 0 && totalTiles > 0) {
+		hipLaunchKernelGGL(kTileCountsBatch, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kCompactScatterBatch, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	}
 }
 
-void noiseAngularRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, float mean, float stDev, int axis,
-                      const Mat3x4f& lookAt, uint64_t seed, uint32_t run, uint32_t tag)
+void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks)
+{
+	if (itemCount > 0 && totalBlocks > 0) {
+		hipLaunchKernelGGL(kPointOpsBatch, dim3(totalBlocks), dim3(kBlock), 0, stream, items, itemCount);
+	}
+}
+
+void pointOp(hipStream_t stream, const PointOpItem& it)
+{
+	if (it.op == kOpFormat) {
+		throw std::invalid_argument("pointOp: format stages take their field layout by value (formatSoaToAos)");
+	}
+	if (it.capacity > 0) {
+		hipLaunchKernelGGL(kPointOp, dim3(blocks(it.capacity)), dim3(kBlock), 0, stream, it);
+	}
+}
+
+void gatherInts(hipStream_t stream, int n, const int32_t* const* src, int32_t* dst)
 {
 	if (n > 0) {
-		hipLaunchKernelGGL(kNoiseAngularRays, dim3(blocks(n)), dim3(kBlock), 0, stream, n, in, out, mean, stDev, axis, lookAt,
-		                   lookAt.inverse(), seed, run, tag);
-	}
-}
-
-void noiseAngularHitpoint(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, Vec3f* outXyz,
-                          float* outDistance, float mean, float stDev, int axis, const Mat3x4f& lookAt, uint64_t seed,
-                          uint32_t run, uint32_t tag)
-{
-	if (capacity > 0) {
-		hipLaunchKernelGGL(kNoiseAngularHitpoint, dim3(blocks(capacity)), dim3(kBlock), 0, stream, capacity, count, inXyz,
-		                   outXyz, outDistance, mean, stDev, axis, lookAt, lookAt.inverse(), seed, run, tag);
-	}
-}
-
-void noiseDistance(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* inXyz, const float* inDistance,
-                   Vec3f* outXyz, float* outDistance, float mean, float stDevBase, float stDevRise, const Mat3x4f& lookAt,
-                   uint64_t seed, uint32_t run, uint32_t tag)
-{
-	if (capacity > 0) {
-		hipLaunchKernelGGL(kNoiseDistance, dim3(blocks(capacity)), dim3(kBlock), 0, stream, capacity, count, inXyz,
-		                   inDistance, outXyz, outDistance, mean, stDevBase, stDevRise, lookAt, lookAt.inverse(), seed, run,
-		                   tag);
+		hipLaunchKernelGGL(kGatherInts, dim3(blocks(n)), dim3(kBlock), 0, stream, n, src, dst);
 	}
 }
 

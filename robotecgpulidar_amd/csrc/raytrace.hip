@@ -430,12 +430,10 @@ struct Tracker
 	HitRec hit;
 };
 
-__global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+// One beam of one sensor: raygen, traversal, closest hit, beam reduction and output writes.
+__device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
+                                             const int beam)
 {
-	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
-	if (beam >= p.rayCount) {
-		return;
-	}
 	const Mat3x4f rayWorld = p.rays[beam];
 	const int R = p.returnCount;
 
@@ -594,6 +592,35 @@ __global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytracePara
 	}
 }
 
+__global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+{
+	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
+	if (beam < p.rayCount) {
+		raytraceBeam(sc, p, out, beam);
+	}
+}
+
+// Many sensors (possibly on different scenes) in one launch: block b belongs to the last item whose firstBlock <= b.
+// The item index is wave-uniform, so its descriptor is read with scalar loads.
+__global__ void __launch_bounds__(kRtBlock) kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
+{
+	int lo = 0, hi = itemCount - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (items[mid].firstBlock <= static_cast<int>(blockIdx.x)) {
+			lo = mid;
+		} else {
+			hi = mid - 1;
+		}
+	}
+	const int item = __builtin_amdgcn_readfirstlane(lo);
+	const RaytraceBatchItem& it = items[item];
+	const int beam = (static_cast<int>(blockIdx.x) - it.firstBlock) * kRtBlock + threadIdx.x;
+	if (beam < it.p.rayCount) {
+		raytraceBeam(it.sc, it.p, it.out, beam);
+	}
+}
+
 __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, RaytraceParams p, int32_t* nodesVisited,
                                                              int32_t* trisTested, int32_t* instancesEntered)
 {
@@ -666,6 +693,15 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 	}
 	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
 	hipLaunchKernelGGL(kRaytrace, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, out);
+}
+
+int raytraceBlocks(int rayCount) { return (rayCount + kRtBlock - 1) / kRtBlock; }
+
+void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks)
+{
+	if (itemCount > 0 && totalBlocks > 0) {
+		hipLaunchKernelGGL(kRaytraceBatch, dim3(totalBlocks), dim3(kRtBlock), 0, stream, items, itemCount);
+	}
 }
 
 void traversalStats(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, int32_t* nodesVisited,

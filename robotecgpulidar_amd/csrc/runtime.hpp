@@ -107,6 +107,7 @@ public:
 		return ptr;
 	}
 	void* get() const { return ptr; }
+	size_t capacity() const { return cap; }
 	void release()
 	{
 		if (ptr != nullptr) {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <set>
 
 namespace rgl {
 
@@ -116,8 +117,8 @@ void Entity::setTransform(const Mat3x4f& tf)
 	formerTransformTime = transformTime;
 	formerTransformValid = true;
 	transform = tf;
-	transformTime = Scene::instance().getTime();
-	Scene::instance().markDirty(false);
+	transformTime = scene->getTime();
+	scene->markDirty(false);
 }
 
 // src/scene/Entity.cpp:66-86
@@ -128,7 +129,7 @@ std::optional<Mat3x4f> Entity::previousFrameTransform() const
 		return transform;
 	}
 	const bool setInPrecedingFrame = formerTransformTime.has_value() &&
-	                                 formerTransformTime == Scene::instance().getPrevTime();
+	                                 formerTransformTime == scene->getPrevTime();
 	if (!setInPrecedingFrame) {
 		return std::nullopt;
 	}
@@ -136,13 +137,36 @@ std::optional<Mat3x4f> Entity::previousFrameTransform() const
 }
 
 // ------------------------------------------------------------------------------------------------ Scene
+namespace {
+std::set<Scene*>& liveScenes()
+{
+	static std::set<Scene*> s;
+	return s;
+}
+} // namespace
+
 Scene& Scene::instance()
 {
 	static Scene scene;
 	return scene;
 }
 
-Scene::Scene() {}
+std::shared_ptr<Scene> Scene::defaultShared()
+{
+	static std::shared_ptr<Scene> p(&instance(), [](Scene*) {});
+	return p;
+}
+
+void Scene::markAllDirty(bool topologyChanged)
+{
+	for (Scene* s : liveScenes()) {
+		s->markDirty(topologyChanged);
+	}
+}
+
+Scene::Scene() { liveScenes().insert(this); }
+
+Scene::~Scene() { liveScenes().erase(this); }
 
 void Scene::addEntity(const std::shared_ptr<Entity>& e)
 {
@@ -180,6 +204,15 @@ float Scene::deltaTimeSeconds() const
 	return static_cast<float>(dt);
 }
 
+void Scene::noteUse(hipStream_t s)
+{
+	auto& ev = uses[s];
+	if (!ev) {
+		ev = std::make_unique<Event>();
+	}
+	ev->record(s);
+}
+
 int32_t* Scene::overflowFlag()
 {
 	if (dOverflow.get() == nullptr) {
@@ -202,6 +235,10 @@ void Scene::rebuild(hipStream_t s)
 {
 	// The previous rebuild's uploads read the pinned staging arrays reused below.
 	RGL_HIP(hipStreamSynchronize(s));
+	// Raytrace launches still in flight read the device arrays rewritten below.
+	for (auto& [user, ev] : uses) {
+		RGL_HIP(hipStreamWaitEvent(s, ev->get(), 0));
+	}
 	const int n = static_cast<int>(entities.size());
 	maxBlasDepth = 0;
 	for (auto& e : entities) {
@@ -272,11 +309,15 @@ SceneView Scene::prepare(hipStream_t graphStream)
 	if (dirty) {
 		rebuild(stream->get());
 		ready->record(stream->get());
+		++readyGeneration;
 		dirty = false;
 		topologyDirty = false;
 	}
-	if (ready->wasRecorded()) {
+	// A stream that already waited for this build needs no second wait.
+	if (ready->wasRecorded() && !(waitedStream == graphStream && waitedGeneration == readyGeneration)) {
 		RGL_HIP(hipStreamWaitEvent(graphStream, ready->get(), 0));
+		waitedStream = graphStream;
+		waitedGeneration = readyGeneration;
 	}
 	if (view.overflow == nullptr) {
 		view.overflow = overflowFlag();

@@ -17,6 +17,8 @@
 
 namespace rgl {
 
+class Scene;
+
 struct Mesh
 {
 	Mesh(const Vec3f* vertices, int vertexCount, const Vec3i* indices, int indexCount);
@@ -48,11 +50,12 @@ struct Time
 
 struct Entity
 {
-	explicit Entity(std::shared_ptr<Mesh> m) : mesh(std::move(m)) {}
+	Entity(std::shared_ptr<Mesh> m, Scene* owner) : mesh(std::move(m)), scene(owner) {}
 	void setTransform(const Mat3x4f& tf);
 	std::optional<Mat3x4f> previousFrameTransform() const;
 
 	std::shared_ptr<Mesh> mesh;
+	Scene* scene;
 	std::shared_ptr<Texture> texture;
 	Mat3x4f transform = Mat3x4f::identity();
 	std::optional<uint64_t> transformTime;       // scene time when the transform was set
@@ -74,7 +77,15 @@ enum class TlasUpdate
 class Scene
 {
 public:
+	Scene();
+	~Scene();
+	Scene(const Scene&) = delete;
+	Scene& operator=(const Scene&) = delete;
+	// The implicit default scene (rgl_scene_t == NULL).
 	static Scene& instance();
+	static std::shared_ptr<Scene> defaultShared();
+	// Marks every live scene stale (e.g. a mesh shared by several scenes changed).
+	static void markAllDirty(bool topologyChanged);
 
 	void addEntity(const std::shared_ptr<Entity>& e);
 	void removeEntity(const std::shared_ptr<Entity>& e);
@@ -93,6 +104,9 @@ public:
 
 	// Builds whatever is stale on the scene stream and makes `graphStream` wait for it.
 	SceneView prepare(hipStream_t graphStream);
+	// Records that work enqueued on `s` so far reads the scene's device data; the next rebuild overwrites that data
+	// only after this work.
+	void noteUse(hipStream_t s);
 	// Traversal-stack overflow flag written by the raytrace kernel (never expected to be set).
 	int32_t* overflowFlag();
 	size_t entityCount() const { return entities.size(); }
@@ -103,7 +117,6 @@ public:
 	int lastTlasDepth = 0;
 
 private:
-	Scene();
 	void rebuild(hipStream_t s);
 
 	std::vector<std::shared_ptr<Entity>> entities;
@@ -116,6 +129,10 @@ private:
 	PinnedBuffer hostInst, hostShade;
 	DeviceBuffer dInst, dShade, dBoxes, tlasNodes, tlasLeafPrim, tlasParent, tlasCounters, tlasHeight, tlasScratch;
 	DeviceBuffer dOverflow;
+	std::map<hipStream_t, std::unique_ptr<Event>> uses;
+	uint64_t readyGeneration = 0;
+	hipStream_t waitedStream = nullptr;
+	uint64_t waitedGeneration = 0;
 	SceneView view{};
 	uint64_t nextOrder = 0;
 	friend struct Entity;

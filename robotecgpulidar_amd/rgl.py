@@ -180,6 +180,11 @@ _PROTOTYPES = [
     ("rgl_noise_normals", [C.c_uint64, C.c_uint32, C.c_uint32, f32p, C.c_int32]),
     ("rgl_node_raytrace_set_kernel_timing", [c_node, C.c_bool]),
     ("rgl_node_raytrace_get_kernel_timing", [c_node, P(C.c_double), i32p]),
+    ("rgl_scene_create", [P(C.c_void_p)]),
+    ("rgl_scene_destroy", [C.c_void_p]),
+    ("rgl_graph_run_batch", [P(c_node), C.c_int32]),
+    ("rgl_graph_batch_set_kernel_timing", [C.c_bool]),
+    ("rgl_graph_batch_get_kernel_timing", [P(C.c_double), i32p]),
 ]
 VOID_FUNCS = {"rgl_get_last_error_string"}
 
@@ -254,10 +259,20 @@ def mesh_set_texture_coords(mesh, uvs) -> None:
     check("rgl_mesh_set_texture_coords", mesh, _ptr(u), len(u))
 
 
-def entity_create(mesh) -> C.c_void_p:
+def entity_create(mesh, scene=None) -> C.c_void_p:
     h = C.c_void_p()
-    check("rgl_entity_create", C.byref(h), None, mesh)
+    check("rgl_entity_create", C.byref(h), scene, mesh)
     return h
+
+
+def scene_create() -> C.c_void_p:
+    h = C.c_void_p()
+    check("rgl_scene_create", C.byref(h))
+    return h
+
+
+def scene_destroy(scene) -> None:
+    check("rgl_scene_destroy", scene)
 
 
 def entity_set_transform(entity, m) -> None:
@@ -323,8 +338,8 @@ def node_points_transform(node: Node, m) -> Node:
     return node
 
 
-def node_raytrace(node: Node) -> Node:
-    check("rgl_node_raytrace", node.ref, None)
+def node_raytrace(node: Node, scene=None) -> Node:
+    check("rgl_node_raytrace", node.ref, scene)
     return node
 
 
@@ -364,6 +379,23 @@ def chain(*nodes: Node) -> None:
 
 def graph_run(node: Node) -> None:
     check("rgl_graph_run", node.h)
+
+
+def graph_run_batch(nodes) -> None:
+    """Runs the graphs of `nodes` (one node per graph) as one batch."""
+    arr = (c_node * max(len(nodes), 1))(*[n.h.value for n in nodes])
+    check("rgl_graph_run_batch", arr, len(nodes))
+
+
+def batch_kernel_timing(enable: bool | None = None):
+    """Enables/disables batched raytrace timing, or (enable=None) returns (total_ms, launches)."""
+    if enable is not None:
+        check("rgl_graph_batch_set_kernel_timing", bool(enable))
+        return None
+    ms = C.c_double()
+    n = C.c_int32()
+    check("rgl_graph_batch_get_kernel_timing", C.byref(ms), C.byref(n))
+    return ms.value, n.value
 
 
 def result_size(node: Node, field: int) -> tuple[int, int]:

@@ -145,19 +145,29 @@ class Lidar:
     azimuths: int = 0
 
 
-def load_into_rgl(spec: SceneSpec):
-    """Creates meshes, textures and entities through the RGL API. Returns (meshes, entities, textures)."""
+def load_into_rgl(spec: SceneSpec, scene=None, meshes=None):
+    """Creates meshes, textures and entities through the RGL API (in `scene`, None = the default scene).
+
+    `meshes` reuses mesh handles created earlier for the same spec. Returns (meshes, entities, textures)."""
     from . import rgl
-    meshes = []
-    for m in spec.meshes:
-        h = rgl.mesh_create(m.vertices, m.indices)
-        if m.uvs is not None:
-            rgl.mesh_set_texture_coords(h, m.uvs)
-        meshes.append(h)
+    if meshes is not None:
+        meshes = list(meshes)
+    else:
+        meshes = []
+        for m in spec.meshes:
+            h = rgl.mesh_create(m.vertices, m.indices)
+            if m.uvs is not None:
+                rgl.mesh_set_texture_coords(h, m.uvs)
+            meshes.append(h)
+    return _load_entities(spec, scene, meshes)
+
+
+def _load_entities(spec: SceneSpec, scene, meshes):
+    from . import rgl
     textures = [rgl.texture_create(t) for t in spec.textures]
     entities = []
     for inst in spec.instances:
-        e = rgl.entity_create(meshes[inst.mesh])
+        e = rgl.entity_create(meshes[inst.mesh], scene)
         rgl.entity_set_transform(e, inst.transform)
         if inst.entity_id is not None:
             rgl.entity_set_id(e, inst.entity_id)
@@ -261,8 +271,8 @@ def config1_lidar() -> Lidar:
                  azimuths=1800)
 
 
-def config2_scene(seed: int = 0xC0F2, target_tris: int = 2_000_000) -> SceneSpec:
-    """Urban scene: 500 instanced icospheres (subdivision 3), boxes and 4-12 sided building prisms (heights
+def config2_scene(seed: int = 0xC0F2, target_tris: int = 2_000_000, instances: int = 500) -> SceneSpec:
+    """Urban scene: `instances` instanced icospheres (subdivision 3), boxes and 4-12 sided building prisms (heights
     U[3,60] m) under random yaw, on a ground grid sized so the total is about `target_tris` triangles."""
     rng = np.random.default_rng(seed)
     spec = SceneSpec()
@@ -270,10 +280,10 @@ def config2_scene(seed: int = 0xC0F2, target_tris: int = 2_000_000) -> SceneSpec
     sphere = spec.add_mesh(sv, si)
     cube = spec.add_mesh(CUBE_VERTICES, CUBE_INDICES)
     prisms = {n: spec.add_mesh(*prism(n)) for n in range(4, 13)}
-    kinds = rng.integers(0, 3, 500)
+    kinds = rng.integers(0, 3, instances)
     inst_tris = 0
     placed = []
-    for k in range(500):
+    for k in range(instances):
         yaw = rng.uniform(0.0, 360.0)
         px, py = rng.uniform(-190.0, 190.0, 2)
         if kinds[k] == 0:
@@ -305,3 +315,31 @@ def config2_lidar() -> Lidar:
     ch, az = 128, 2048
     offsets = np.tile(np.arange(az, dtype=np.float64) * (100.0 / az), ch).astype(np.float32)
     return Lidar(spinning_lidar(ch, az, -25.0, 15.0), f32(translation(0, 0, 1.8)), (0.0, 200.0), offsets, ch, az)
+
+
+@dataclass
+class BatchSensor:
+    scene: int     # index into the workload's scene list
+    lidar: Lidar
+
+
+def config4_workload(seed: int = 0xC0F4, sensors: int = 64, scenes: int = 8):
+    """Batched multi-sensor step: `sensors` lidars of uneven size (channels U{16..128}, azimuths U{900..4096}, so
+    14.4k-524k rays each; elevations [-25, +15] deg, range 200 m) placed at random positions and yaws over `scenes`
+    urban scenes of (stratified) log-uniform 50k-5M triangles. Returns (scene specs, sensors)."""
+    rng = np.random.default_rng(seed)
+    # Stratified log-uniform sizes: one draw per equal slice of [log 50k, log 5M], so the range is covered.
+    lo, hi = math.log(5e4), math.log(5e6)
+    targets = np.exp(lo + (np.arange(scenes) + rng.uniform(0.0, 1.0, scenes)) * (hi - lo) / scenes)
+    specs = [config2_scene(seed=seed + 1 + k, target_tris=int(t), instances=int(np.clip(t / 4000, 24, 500)))
+             for k, t in enumerate(targets)]
+    out = []
+    for _ in range(sensors):
+        ch = int(rng.integers(16, 129))
+        az = int(rng.integers(900, 4097))
+        sc = int(rng.integers(0, scenes))
+        px, py = rng.uniform(-60.0, 60.0, 2)
+        yaw = rng.uniform(0.0, 360.0)
+        pose = f32(trs((px, py, 1.8), (0, 0, yaw)))
+        out.append(BatchSensor(sc, Lidar(spinning_lidar(ch, az, -25.0, 15.0), pose, (0.0, 200.0), None, ch, az)))
+    return specs, out

@@ -1,0 +1,195 @@
+"""Batched execution (rgl_graph_run_batch): many sensor graphs, on two scenes, run as one batch must give every
+graph exactly the results it gives when run alone (bit for bit, noise included), and a noise-free sensor in the
+batch must match the FP64 oracle to the parity bar of the single-graph path.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from robotecgpulidar_amd import scenes as S
+from tests.helpers import Case, run_oracle
+from tests.test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+XYZ, IS_HIT, ENT, DIST, RING, NORMAL, PAD32 = 1, 4, 7, 8, 11, 22, 1026
+FORMAT = 13842
+FIELDS = [XYZ, DIST, ENT, IS_HIT, NORMAL]
+FORMAT_FIELDS = [XYZ, PAD32, DIST, ENT]
+
+# (channels, azimuth steps, scene index, noise, compact)
+SENSORS = [
+    (16, 360, 0, True, True),
+    (32, 512, 1, True, True),
+    (8, 100, 0, False, False),
+    (64, 900, 1, True, False),
+    (1, 1, 0, True, True),
+    (128, 64, 1, False, True),
+]
+
+
+def second_scene_spec(seed=5):
+    rng = np.random.default_rng(seed)
+    spec = S.SceneSpec()
+    sphere = spec.add_mesh(*S.icosphere(3))
+    cube = spec.add_mesh(S.CUBE_VERTICES, S.CUBE_INDICES)
+    ground = spec.add_mesh(*S.grid_mesh(200.0, 200.0, 20, 20))
+    spec.add_instance(ground, S.translation(0, 0, -1.5), entity_id=900)
+    for k in range(40):
+        pos = (rng.uniform(-40, 40), rng.uniform(-40, 40), rng.uniform(-1, 3))
+        spec.add_instance(sphere if k % 2 else cube, S.trs(pos, rng.uniform(0, 360, 3), rng.uniform(0.5, 3, 3)),
+                          entity_id=1000 + k)
+    return spec
+
+
+class SensorGraph:
+    """rays -> pose -> range -> [angular ray noise] -> raytrace(scene) -> [distance noise] -> [compact]
+    -> points transform -> yield + format."""
+
+    def __init__(self, rgl, ch, az, scene, noise, compact, seed):
+        self.rgl = rgl
+        lidar = S.spinning_lidar(ch, az, -20.0, 10.0) if ch > 1 else S.f32(S.identity()[None])
+        self.rays_np = lidar
+        self.pose = S.f32(S.trs((1.0 + 0.1 * seed, -2.0, 1.7), (0, 0, 7.0 * seed), (1, 1, 1)))
+        self.rays = rgl.node_rays_from_mat3x4f(rgl.Node(), lidar)
+        chain = [self.rays, rgl.node_rays_transform(rgl.Node(), self.pose),
+                 rgl.node_rays_set_range(rgl.Node(), np.array([[0.2, 150.0]], np.float32))]
+        self.noise = []
+        if noise:
+            n = rgl.Node()
+            rgl.check("rgl_node_gaussian_noise_angular_ray", n.ref, 0.0, 0.002, 3)
+            self.noise.append(n)
+            chain.append(n)
+        self.raytrace = rgl.node_raytrace(rgl.Node(), scene)
+        rgl.check("rgl_node_raytrace_configure_non_hits", self.raytrace.h, 0.0, 150.0)
+        chain.append(self.raytrace)
+        if noise:
+            n = rgl.Node()
+            rgl.check("rgl_node_gaussian_noise_distance", n.ref, 0.0, 0.01, 0.001)
+            self.noise.append(n)
+            chain.append(n)
+        if compact:
+            chain.append(rgl.node_points_compact_by_field(rgl.Node(), IS_HIT))
+        chain.append(rgl.node_points_transform(rgl.Node(), S.f32(S.translation(0.5, 0.0, -0.25))))
+        self.out = rgl.node_points_yield(rgl.Node(), FIELDS)
+        chain.append(self.out)
+        rgl.chain(*chain)
+        self.fmt = rgl.node_points_format(rgl.Node(), FORMAT_FIELDS)
+        rgl.add_child(chain[-2], self.fmt)
+        for k, n in enumerate(self.noise):
+            rgl.check("rgl_node_gaussian_noise_set_seed", n.h, C.c_uint64(0x5EED0000 + 97 * seed + k))
+
+    def results(self):
+        r = {f: self.rgl.result(self.out, f) for f in FIELDS}
+        r[FORMAT] = self.rgl.result(self.fmt, FORMAT)
+        return r
+
+    def noise_runs(self):
+        runs = []
+        for n in self.noise:
+            s, r, t = C.c_uint64(), C.c_uint32(), C.c_uint32()
+            self.rgl.check("rgl_node_gaussian_noise_get_stream", n.h, C.byref(s), C.byref(r), C.byref(t))
+            runs.append(r.value)
+        return runs
+
+
+def build_world(rgl):
+    spec0, spec1 = S.config1_scene(), second_scene_spec()
+    _, ents0, _ = S.load_into_rgl(spec0)
+    scene1 = rgl.scene_create()
+    _, ents1, _ = S.load_into_rgl(spec1, scene=scene1)
+    return (spec0, spec1), (None, scene1), (ents0, ents1)
+
+
+def make_set(rgl, scenes):
+    return [SensorGraph(rgl, ch, az, scenes[si], noise, compact, seed=k)
+            for k, (ch, az, si, noise, compact) in enumerate(SENSORS)]
+
+
+def assert_same(a, b, label):
+    for f in a:
+        assert a[f].shape == b[f].shape, (label, f, a[f].shape, b[f].shape)
+        assert a[f].tobytes() == b[f].tobytes(), (label, f)
+
+
+def test_batch_equals_solo_over_frames_and_scene_updates(rgl_lib):
+    rgl = rgl_lib
+    specs, scenes, ents = build_world(rgl)
+    solo, batched = make_set(rgl, scenes), make_set(rgl, scenes)
+    for frame in range(3):
+        if frame > 0:  # move entities of both scenes between frames (TLAS rebuild while graphs exist)
+            rgl.entity_set_transform(ents[0][3], S.f32(S.trs((2.0 * frame, 1.0, 0.0), (0, 0, 15 * frame), (1, 1, 1))))
+            rgl.entity_set_transform(ents[1][5], S.f32(S.trs((-3.0, 4.0 * frame, 0.5), (10 * frame, 0, 0), (2, 2, 2))))
+        if frame == 1:  # swap roles: the first set runs batched, the second alone
+            solo, batched = batched, solo
+        for g in solo:
+            rgl.graph_run(g.rays)
+        rgl.graph_run_batch([g.rays for g in batched])
+        for k, (a, b) in enumerate(zip(solo, batched)):
+            ra, rb = a.results(), b.results()
+            assert_same(ra, rb, f"frame {frame} sensor {k}")
+            assert a.noise_runs() == b.noise_runs() == [frame + 1] * len(a.noise)
+        hits = sum(int(g.results()[IS_HIT].sum()) for g in batched)
+        print(f"frame {frame}: {hits} hit points over {len(batched)} sensors")
+        assert hits > 1000
+
+
+def test_both_sets_in_one_batch_and_duplicates(rgl_lib):
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    a, b = make_set(rgl, scenes), make_set(rgl, scenes)
+    # Every graph named twice (entry node and output node): each graph runs exactly once.
+    rgl.graph_run_batch([g.rays for g in a + b] + [g.out for g in a + b])
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert_same(x.results(), y.results(), f"sensor {k}")
+        assert x.noise_runs() == [1] * len(x.noise)
+
+
+def test_batch_sensor_matches_oracle(rgl_lib, oracle):
+    rgl = rgl_lib
+    specs, scenes, _ = build_world(rgl)
+    graphs = make_set(rgl, scenes)
+    rgl.graph_run_batch([g.rays for g in graphs])
+    for k in (2,):  # noise-free, uncompacted sensor on the default scene
+        g = graphs[k]
+        ch, az, si, noise, compact = SENSORS[k]
+        assert not noise and not compact
+        case = Case(specs[si], g.rays_np, pose=g.pose, ranges=np.array([[0.2, 150.0]], np.float32), near=0.0, far=150.0)
+        orc = run_oracle(oracle, case, fields=FIELDS)
+        res = g.results()
+        gpu = {f: res[f] for f in FIELDS}
+        # The yield sits after a points transform by (0.5, 0, -0.25): undo it for XYZ before comparing.
+        gpu[XYZ] = (gpu[XYZ].astype(np.float64) - np.array([0.5, 0.0, -0.25])).astype(np.float32)
+        same = compare(gpu, orc, len(g.rays_np), f"batched sensor {k}")
+        assert np.abs(gpu[XYZ][same].astype(np.float64) - orc[XYZ][same]).max() <= 2e-3
+
+
+def test_invalid_graph_fails_the_whole_batch(rgl_lib):
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    good = make_set(rgl, scenes)[:2]
+    bad_rays = rgl.node_rays_from_mat3x4f(rgl.Node(), S.spinning_lidar(4, 16, -5.0, 5.0))
+    bad_rt = rgl.node_raytrace(rgl.Node())
+    bad_out = rgl.node_points_yield(rgl.Node(), [XYZ, RING])  # no ring ids on the rays: invalid pipeline
+    rgl.chain(bad_rays, bad_rt, bad_out)
+    status = rgl.call("rgl_graph_run_batch", (rgl.c_node * 3)(good[0].rays.h.value, bad_rays.h.value,
+                                                             good[1].rays.h.value), 3)
+    assert status == rgl.RGL_INVALID_PIPELINE, rgl.last_error()
+    assert "ring" in rgl.last_error().lower()
+    assert good[0].noise_runs() == [0, 0]  # nothing of the batch ran
+    rgl.graph_run_batch([g.rays for g in good])
+    assert good[0].noise_runs() == [1, 1]
+    assert len(good[0].results()[XYZ]) > 0
+
+
+def test_batch_kernel_timing(rgl_lib):
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    graphs = make_set(rgl, scenes)
+    rgl.batch_kernel_timing(True)
+    for _ in range(3):
+        rgl.graph_run_batch([g.rays for g in graphs])
+    ms, launches = rgl.batch_kernel_timing()
+    rgl.batch_kernel_timing(False)
+    assert launches == 3 and ms > 0.0
