@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["batch64", "urban"], default="batch64")
+    ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",
+                    help="node format: binary FP32, 4-wide FP32, or 4-wide 8-bit quantized")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for cpu_baseline (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copy kernels of MIB MiB (reference for rocprof byte counters)")
@@ -332,7 +334,11 @@ def main():
         sys.exit(2)
     rgl.load()
     rgl.check("rgl_configure_logging", 6, None, False)
+    fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8}[args.bvh_format]
+    rgl.set_bvh_format(fmt)
     result = run_batch64(args) if args.workload == "batch64" else run_urban(args)
+    result["config"]["bvh_format"] = {rgl.BVH2: "binary FP32 (64 B nodes)", rgl.BVH4_F32: "4-wide FP32 (128 B nodes)",
+                                      rgl.BVH4_Q8: "4-wide 8-bit quantized (64 B nodes)"}[fmt]
     rgl.cleanup()
     print(json.dumps(result))
 

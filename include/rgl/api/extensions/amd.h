@@ -19,6 +19,15 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_get_stream(rgl_node_t node, uint64_
 // refit = true keeps the topology and refits boxes; false rebuilds the TLAS (default).
 RGL_API rgl_status_t rgl_scene_set_tlas_refit(rgl_scene_t scene, bool refit);
 
+// BVH node format used by traversal in every scene: 0 = binary with FP32 child boxes (64 B nodes), 1 = 4-wide with
+// FP32 child boxes (128 B), 2 = 4-wide with child boxes quantized to 8 bits relative to the node box and rounded
+// outward (64 B). All formats index the same leaves, so results are identical; they differ in speed only. Scenes are
+// rebuilt on their next use.
+RGL_API rgl_status_t rgl_configure_bvh_format(int32_t format);
+
+// Reports the BVH node format in use.
+RGL_API rgl_status_t rgl_get_bvh_format(int32_t* out_format);
+
 // Per-ray traversal counters of a raytrace node's last configuration: internal BVH nodes visited, triangles tested
 // and instances entered. Arrays hold `ray_count` entries (the node's ray count).
 RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t* nodes_visited, int32_t* tris_tested,

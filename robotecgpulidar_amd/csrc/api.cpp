@@ -1066,6 +1066,22 @@ RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene)
 	});
 }
 
+RGL_API rgl_status_t rgl_configure_bvh_format(int32_t format)
+{
+	return safeCall([&]() {
+		CHECK_ARG(format >= 0 && format < rgl::kBvhFormats);
+		rgl::Scene::setBvhFormat(format);
+	});
+}
+
+RGL_API rgl_status_t rgl_get_bvh_format(int32_t* out_format)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_format != nullptr);
+		*out_format = rgl::Scene::bvhFormat();
+	});
+}
+
 RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t* nodes_visited, int32_t* tris_tested,
                                                        int32_t* instances_entered, int32_t ray_count)
 {

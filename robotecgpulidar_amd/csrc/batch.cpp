@@ -350,7 +350,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 					RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
 				}
 				gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
-				                   static_cast<int>(L.rt.size()), L.rtBlocks);
+				                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(),
+				                   std::any_of(L.rt.begin(), L.rt.end(),
+				                               [](const RaytraceBatchItem& it) { return gpu::beamIsMulti(it.p); }));
 				RGL_HIP(hipGetLastError());
 				if (timed) {
 					RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, s));

@@ -402,7 +402,7 @@ __global__ void kInstanceBoxes(int n, const InstanceShade* shades, const Instanc
 	if (i >= n) {
 		return;
 	}
-	const float* f = recs[i].blasNodes[0].f;
+	const float* f = shades[i].blas2[0].f;
 	float lo[3], hi[3];
 	for (int a = 0; a < 3; ++a) {
 		lo[a] = fminf(f[a], f[6 + a]);
@@ -428,6 +428,204 @@ __global__ void kInstanceBoxes(int n, const InstanceShade* shades, const Instanc
 		r.hi[a] += pad;
 	}
 	out[i] = r;
+}
+
+// ------------------------------------------------------------------------------------------------ 4-wide nodes
+// The 4-wide tree keeps every even-depth internal node of the binary LBVH; a kept node's children are the children
+// of its two binary children (or those children themselves when they are leaves). The binary tree's depth is
+// halved and the set of leaves under every node is unchanged.
+constexpr int kScanBlock = 1024;
+
+__global__ void kEvenDepthFlags(int inner, int n, const int32_t* parentSlot, int32_t* flags)
+{
+	const int k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= inner) {
+		return;
+	}
+	int depth = 0;
+	int code = n > 1 ? parentSlot[n + k] : -1;
+	while (code >= 0) {
+		++depth;
+		code = parentSlot[n + (code >> 1)];
+	}
+	flags[k] = (depth & 1) == 0 ? 1 : 0;
+}
+
+// Exclusive scan, three passes: per-block scan (1024 elements), scan of the block sums, add block offsets.
+__device__ int blockExclusiveScan(int v, int* total)
+{
+	__shared__ int waveSums[kScanBlock / 64];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int x = v;
+	for (int off = 1; off < 64; off <<= 1) {
+		const int y = __shfl_up(x, off, 64);
+		if (lane >= off) {
+			x += y;
+		}
+	}
+	if (lane == 63) {
+		waveSums[wave] = x;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int run = 0;
+		for (int w = 0; w < kScanBlock / 64; ++w) {
+			const int c = waveSums[w];
+			waveSums[w] = run;
+			run += c;
+		}
+		*total = run;
+	}
+	__syncthreads();
+	return waveSums[wave] + x - v;
+}
+
+__global__ void __launch_bounds__(kScanBlock) kScanLocal(int n, const int32_t* in, int32_t* out, int32_t* blockSums)
+{
+	__shared__ int total;
+	const int i = blockIdx.x * kScanBlock + threadIdx.x;
+	const int v = i < n ? in[i] : 0;
+	const int e = blockExclusiveScan(v, &total);
+	if (i < n) {
+		out[i] = e;
+	}
+	if (threadIdx.x == 0) {
+		blockSums[blockIdx.x] = total;
+	}
+}
+
+__global__ void __launch_bounds__(kScanBlock) kScanSums(int nb, int32_t* sums, int32_t* grandTotal)
+{
+	__shared__ int total;
+	int carry = 0;
+	for (int base = 0; base < nb; base += kScanBlock) {
+		const int i = base + threadIdx.x;
+		const int v = i < nb ? sums[i] : 0;
+		const int e = blockExclusiveScan(v, &total);
+		if (i < nb) {
+			sums[i] = carry + e;
+		}
+		carry += total;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		*grandTotal = carry;
+	}
+}
+
+__global__ void kScanAdd(int n, int32_t* out, const int32_t* blockSums)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) {
+		out[i] += blockSums[i / kScanBlock];
+	}
+}
+
+__device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
+
+// Smallest exponent e (scale 2^e) with fl(lo + 255 * 2^e) >= hi.
+__device__ int quantExponent(float lo, float hi)
+{
+	if (!(hi > lo)) {
+		return -126;
+	}
+	int e = max(-126, min(127, ilogbf((hi - lo) / 255.0f)));
+	while (e < 127 && __fadd_rn(lo, 255.0f * pow2f(e)) < hi) {
+		++e;
+	}
+	while (e > -126 && __fadd_rn(lo, 255.0f * pow2f(e - 1)) >= hi) {
+		--e;
+	}
+	return e;
+}
+
+template<bool kQuant>
+__global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags, const int32_t* rank, void* out)
+{
+	const int k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= inner || flags[k] == 0) {
+		return;
+	}
+	float clo[4][3], chi[4][3];
+	int ref[4] = {0, 0, 0, 0};
+	int cnt = 0;
+	auto add = [&](const float* f, int r) {
+		if (f[0] > f[3] || f[1] > f[4] || f[2] > f[5]) {
+			return; // empty slot (inverted box)
+		}
+		for (int a = 0; a < 3; ++a) {
+			clo[cnt][a] = f[a];
+			chi[cnt][a] = f[3 + a];
+		}
+		ref[cnt] = r;
+		++cnt;
+	};
+	const BvhNode b = nodes[k];
+	for (int s = 0; s < 2; ++s) {
+		const int c = b.child[s];
+		if (c < 0) {
+			add(b.f + 6 * s, c);
+		} else {
+			const BvhNode g = nodes[c];
+			for (int t = 0; t < 2; ++t) {
+				const int gc = g.child[t];
+				add(g.f + 6 * t, gc < 0 ? gc : rank[gc]);
+			}
+		}
+	}
+	const int slot = rank[k];
+	if (!kQuant) {
+		Bvh4FNode o;
+		for (int c = 0; c < 4; ++c) {
+			for (int a = 0; a < 3; ++a) {
+				o.lo[a][c] = c < cnt ? clo[c][a] : FLT_MAX;
+				o.hi[a][c] = c < cnt ? chi[c][a] : -FLT_MAX;
+			}
+			o.child[c] = c < cnt ? ref[c] : ref[0];
+		}
+		o.count = cnt;
+		o.pad[0] = o.pad[1] = o.pad[2] = 0;
+		static_cast<Bvh4FNode*>(out)[slot] = o;
+		return;
+	}
+	Bvh4QNode o;
+	for (int a = 0; a < 3; ++a) {
+		float lo = FLT_MAX, hi = -FLT_MAX;
+		for (int c = 0; c < cnt; ++c) {
+			lo = fminf(lo, clo[c][a]);
+			hi = fmaxf(hi, chi[c][a]);
+		}
+		const int e = quantExponent(lo, hi);
+		const float sc = pow2f(e);
+		o.origin[a] = lo;
+		o.exp[a] = static_cast<uint8_t>(e + 127);
+		for (int c = 0; c < 4; ++c) {
+			if (c >= cnt) {
+				o.qlo[a][c] = 255;
+				o.qhi[a][c] = 0;
+				continue;
+			}
+			// Lower bound rounded down, upper bound rounded up, both checked in the decoder's FP32 arithmetic.
+			int ql = static_cast<int>(floor((static_cast<double>(clo[c][a]) - lo) / sc));
+			ql = max(0, min(255, ql));
+			while (ql > 0 && __fadd_rn(lo, static_cast<float>(ql) * sc) > clo[c][a]) {
+				--ql;
+			}
+			int qh = static_cast<int>(ceil((static_cast<double>(chi[c][a]) - lo) / sc));
+			qh = max(0, min(255, qh));
+			while (qh < 255 && __fadd_rn(lo, static_cast<float>(qh) * sc) < chi[c][a]) {
+				++qh;
+			}
+			o.qlo[a][c] = static_cast<uint8_t>(ql);
+			o.qhi[a][c] = static_cast<uint8_t>(qh);
+		}
+	}
+	for (int c = 0; c < 4; ++c) {
+		o.child[c] = c < cnt ? ref[c] : ref[0];
+	}
+	o.count = static_cast<uint8_t>(cnt);
+	o.pad[0] = o.pad[1] = 0;
+	static_cast<Bvh4QNode*>(out)[slot] = o;
 }
 
 inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
@@ -526,6 +724,41 @@ void instanceBoxes(hipStream_t stream, int n, const InstanceShade* shades, const
 {
 	if (n > 0) {
 		hipLaunchKernelGGL(kInstanceBoxes, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, shades, recs, out);
+	}
+}
+
+size_t bvh4ScratchBytes(int n)
+{
+	const size_t inner = static_cast<size_t>(std::max(n - 1, 1));
+	const size_t blocks = (inner + kScanBlock - 1) / kScanBlock;
+	return ((2 * inner * sizeof(int32_t) + 255) & ~static_cast<size_t>(255)) + (blocks + 64) * sizeof(int32_t);
+}
+
+void bvh4Ranks(hipStream_t stream, int n, const int32_t* parentSlot, void* scratch, int32_t* countDev)
+{
+	const int inner = std::max(n - 1, 1);
+	int32_t* flags = static_cast<int32_t*>(scratch);
+	int32_t* rank = flags + inner;
+	int32_t* blockSums =
+	    reinterpret_cast<int32_t*>(static_cast<char*>(scratch) + ((2 * inner * sizeof(int32_t) + 255) & ~static_cast<size_t>(255)));
+	const int nb = (inner + kScanBlock - 1) / kScanBlock;
+	hipLaunchKernelGGL(kEvenDepthFlags, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, n, parentSlot, flags);
+	hipLaunchKernelGGL(kScanLocal, dim3(nb), dim3(kScanBlock), 0, stream, inner, flags, rank, blockSums);
+	hipLaunchKernelGGL(kScanSums, dim3(1), dim3(kScanBlock), 0, stream, nb, blockSums, countDev);
+	hipLaunchKernelGGL(kScanAdd, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, rank, blockSums);
+}
+
+void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* scratch, int format, void* out)
+{
+	const int inner = std::max(n - 1, 1);
+	const int32_t* flags = static_cast<const int32_t*>(scratch);
+	const int32_t* rank = flags + inner;
+	if (format == kBvh4Q) {
+		hipLaunchKernelGGL(kCollapse4<true>, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, nodes, flags, rank,
+		                   out);
+	} else {
+		hipLaunchKernelGGL(kCollapse4<false>, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, nodes, flags, rank,
+		                   out);
 	}
 }
 

@@ -24,6 +24,44 @@ struct alignas(16) BvhNode
 };
 static_assert(sizeof(BvhNode) == 64, "BVH node must be 64 B");
 
+// Node formats traversal can use. Every format is derived from the same binary LBVH, so the set of triangles a ray
+// can reach, and therefore the closest hit, does not depend on the format.
+enum BvhFormat : int32_t
+{
+	kBvh2 = 0,   // BvhNode: binary, FP32 child boxes, 64 B
+	kBvh4F = 1,  // Bvh4FNode: 4-wide, FP32 child boxes, 128 B
+	kBvh4Q = 2,  // Bvh4QNode: 4-wide, child boxes quantized to 8 bits relative to the node box, 64 B
+};
+constexpr int kBvhFormats = 3;
+
+// 4-wide node with FP32 child boxes stored per axis ([axis][child]) so the four slab tests load contiguous lanes.
+// Children occupy slots [0, count); child references follow the BvhNode convention.
+struct alignas(16) Bvh4FNode
+{
+	float lo[3][4];
+	float hi[3][4];
+	int32_t child[4];
+	int32_t count;
+	int32_t pad[3];
+};
+static_assert(sizeof(Bvh4FNode) == 128, "BVH4 FP32 node must be 128 B");
+
+// 4-wide node with quantized child boxes. A child bound on axis a decodes, in FP32, as
+// origin[a] + q * 2^(exp[a] - 127), one exact product and one rounded sum; the builder picks q so that the decoded
+// box contains the child's FP32 box (lower bounds rounded down, upper bounds rounded up, each checked in the same
+// FP32 arithmetic), so traversal with decoded boxes culls nothing the exact boxes would keep.
+struct alignas(16) Bvh4QNode
+{
+	float origin[3];
+	uint8_t exp[3];
+	uint8_t count;
+	uint8_t qlo[3][4]; // [axis][child]
+	uint8_t qhi[3][4];
+	int32_t child[4];
+	int32_t pad[2];
+};
+static_assert(sizeof(Bvh4QNode) == 64, "BVH4 quantized node must be 64 B");
+
 // Leaf triangle in BVH leaf order, 48 B: object-space vertices A, B, C. w of the first vec4 holds the
 // triangle's index in the mesh index buffer (bit pattern of an int32), which the shading stage uses for UVs
 // and which breaks distance ties deterministically.
@@ -40,7 +78,7 @@ static_assert(sizeof(LeafTri) == 48, "leaf triangle must be 48 B");
 struct alignas(16) InstanceRec
 {
 	Mat3x4f worldToObject;      // FP32, composed in FP64 on the host and rounded once
-	const BvhNode* blasNodes;   // BLAS of the instance's mesh
+	const void* blasNodes;      // BLAS of the instance's mesh, in the scene's node format
 	const LeafTri* blasTris;
 };
 static_assert(sizeof(InstanceRec) == 64, "instance record must be 64 B");
@@ -54,21 +92,25 @@ struct alignas(16) InstanceShade
 	const Vec3i* indices;       // mesh index buffer
 	const Vec2f* uvs;           // nullptr if the mesh has no texture coordinates
 	const uint8_t* texels;      // nullptr if the entity has no intensity texture
+	const BvhNode* blas2;       // binary BLAS of the mesh (its root holds the mesh's object-space box)
 	int32_t texWidth;
 	int32_t texHeight;
 	int32_t entityId;
 	float laserRetro;
 	int32_t hasPrev;
-	int32_t pad[3];
+	int32_t pad[1];
 };
+static_assert(sizeof(InstanceShade) == 160, "instance shading record must be 160 B");
 
 // Device view of a scene for one launch.
 struct SceneView
 {
-	const BvhNode* tlasNodes;       // nullptr if the scene is empty
+	const void* tlasNodes;          // nullptr if the scene is empty; node format `format`
+	const BvhNode* tlas2;           // the binary TLAS (canonical tree for traversal statistics)
 	const InstanceRec* instances;
 	const InstanceShade* shades;
 	int32_t instanceCount;
+	int32_t format;                 // BvhFormat of the TLAS and of every BLAS
 	int32_t* overflow;              // set to 1 if a traversal stack would overflow (the ray is then cut short)
 };
 

@@ -29,6 +29,12 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 // Refits the boxes of an existing LBVH topology to new primitive boxes.
 void refitLbvh(hipStream_t stream, int n, const Aabb* boxes, const int32_t* leafPrim, const int32_t* parentSlot,
                BvhNode* nodes, int32_t* counters, int32_t* height);
+// 4-wide nodes from a binary LBVH over n leaves (built by buildLbvh). bvh4Ranks marks the kept (even-depth) internal
+// nodes in `scratch` (bvh4ScratchBytes(n)) and writes their number to *countDev; bvh4Collapse then writes that many
+// nodes of `format` (kBvh4F or kBvh4Q) to `out`. Leaf references keep the binary tree's convention.
+size_t bvh4ScratchBytes(int n);
+void bvh4Ranks(hipStream_t stream, int n, const int32_t* parentSlot, void* scratch, int32_t* countDev);
+void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* scratch, int format, void* out);
 void triangleBoxes(hipStream_t stream, int n, const Vec3f* v, const Vec3i* idx, Aabb* out);
 void gatherTris(hipStream_t stream, int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out);
 void instanceBoxes(hipStream_t stream, int n, const InstanceShade* shades, const InstanceRec* recs, Aabb* out);
@@ -44,8 +50,12 @@ void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t
                        float* outT, int32_t* outInstance, int32_t* outPrim);
 void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
 int raytraceBlocks(int rayCount);
-// Batched launches: `items` live in device memory.
-void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks);
+// True if the beams of `p` are traced with the 58-sample divergence model.
+bool beamIsMulti(const RaytraceParams& p);
+// Batched launches: `items` live in device memory; every item's scene uses node format `format`; anyMulti must be
+// true if any item has divergent beams.
+void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
+                   bool anyMulti);
 int pointOpBlocks(int capacity);
 void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks);
 void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles);

@@ -27,6 +27,17 @@ namespace rgl::gpu {
 namespace {
 
 constexpr int kRtBlock = 128;
+// Occupancy target (waves per SIMD) of the single-sample raytrace kernels. The traversal is latency-bound, so more
+// waves in flight pay until register spills start: on gfx950, 5 waves (96 VGPRs) spill 16 B per lane and beat both
+// the compiler's own choice (4 waves, ~106 VGPRs) and 6 waves (80 VGPRs, 96 B of spills). 0 = compiler's choice.
+#ifndef RGL_RT_WAVES
+#define RGL_RT_WAVES 5
+#endif
+#if RGL_RT_WAVES > 0
+#define RGL_RT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RGL_RT_WAVES, 8)))
+#else
+#define RGL_RT_OCCUPANCY
+#endif
 constexpr int kLdsStack = 24;
 constexpr int kOvfStack = 104;
 constexpr int kSentinel = 0x7fffffff;
@@ -146,10 +157,23 @@ __device__ __forceinline__ void slab(const TravRay& r, const float* f, float tBe
 	tNear = tn;
 }
 
+// Slab test of a decoded child box.
+__device__ __forceinline__ void slab6(const TravRay& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                      float tBest, bool& hit, float& tNear)
+{
+	const float f[6] = {lx, ly, lz, hx, hy, hz};
+	slab(r, f, tBest, hit, tNear);
+}
+
+__device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<float>((w >> (8 * c)) & 0xffu); }
+
 // Closest hit of one ray against the two-level BVH; returns false if nothing is hit in (0, tmax].
-template<bool kStats>
+// kFmt selects the node format of the TLAS and BLASes (BvhFormat). With kStats the canonical binary trees are
+// traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted.
+template<bool kStats, int kFmt>
 __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
 {
+	static_assert(!kStats || kFmt == kBvh2, "traversal statistics are defined on the binary trees");
 	// Short stack in LDS, [depth][thread] so a wave's pushes hit distinct banks; deeper entries spill to scratch.
 	__shared__ int32_t ldsStack[kLdsStack * kRtBlock];
 	best.t = tmax;
@@ -179,10 +203,22 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		}                                                                                                              \
 	} while (0)
 #define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kRtBlock + tid] : __float_as_int(ovf[sp - kLdsStack]))
+#define RGL_CSWAP(a, b)                                                                                                \
+	do {                                                                                                               \
+		if (k##b < k##a) {                                                                                             \
+			const float tk = k##a;                                                                                     \
+			k##a = k##b;                                                                                               \
+			k##b = tk;                                                                                                 \
+			const int tr = r##a;                                                                                       \
+			r##a = r##b;                                                                                               \
+			r##b = tr;                                                                                                 \
+		}                                                                                                              \
+	} while (0)
 
 	const TravRay world = makeTravRay(o, d);
 	TravRay cur = world;
-	const BvhNode* nodes = sc.tlasNodes;
+	const void* tlas = kStats ? static_cast<const void*>(sc.tlas2) : sc.tlasNodes;
+	const void* nodes = tlas;
 	const LeafTri* tris = nullptr;
 	int inst = -1;
 	bool found = false;
@@ -191,26 +227,89 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		if (kStats) {
 			st->nodes++;
 		}
-		const float4* np = reinterpret_cast<const float4*>(nodes + node);
-		const float4 q0 = np[0], q1 = np[1], q2 = np[2];
-		const int4 q3 = reinterpret_cast<const int4*>(np)[3];
-		const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-		const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-		bool h0, h1;
-		float n0, n1;
-		slab(cur, b0, best.t, h0, n0);
-		slab(cur, b1, best.t, h1, n1);
 		int next;
-		if (h0 && h1) {
-			const bool firstNear = n0 <= n1;
-			RGL_PUSH(firstNear ? q3.y : q3.x);
-			next = firstNear ? q3.x : q3.y;
-		} else if (h0) {
-			next = q3.x;
-		} else if (h1) {
-			next = q3.y;
+		if constexpr (kFmt == kBvh2) {
+			const float4* np = reinterpret_cast<const float4*>(static_cast<const BvhNode*>(nodes) + node);
+			const float4 q0 = np[0], q1 = np[1], q2 = np[2];
+			const int4 q3 = reinterpret_cast<const int4*>(np)[3];
+			const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+			const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+			bool h0, h1;
+			float n0, n1;
+			slab(cur, b0, best.t, h0, n0);
+			slab(cur, b1, best.t, h1, n1);
+			if (h0 && h1) {
+				const bool firstNear = n0 <= n1;
+				RGL_PUSH(firstNear ? q3.y : q3.x);
+				next = firstNear ? q3.x : q3.y;
+			} else if (h0) {
+				next = q3.x;
+			} else if (h1) {
+				next = q3.y;
+			} else {
+				next = kPop;
+			}
 		} else {
-			next = kPop;
+			// Four children: slab-test each, order the hit ones near to far, descend into the nearest.
+			bool h0, h1, h2, h3;
+			float k0, k1, k2, k3;
+			int r0, r1, r2, r3, cnt;
+			if constexpr (kFmt == kBvh4F) {
+				const float4* np = reinterpret_cast<const float4*>(static_cast<const Bvh4FNode*>(nodes) + node);
+				const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+				const int4 ch = reinterpret_cast<const int4*>(np)[6];
+				cnt = reinterpret_cast<const int4*>(np)[7].x;
+				slab6(cur, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, best.t, h0, k0);
+				slab6(cur, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, best.t, h1, k1);
+				slab6(cur, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, best.t, h2, k2);
+				slab6(cur, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, best.t, h3, k3);
+				r0 = ch.x;
+				r1 = ch.y;
+				r2 = ch.z;
+				r3 = ch.w;
+			} else {
+				const uint4* up = reinterpret_cast<const uint4*>(static_cast<const Bvh4QNode*>(nodes) + node);
+				const uint4 u0 = up[0], u1 = up[1], u2 = up[2], u3 = up[3];
+				const float ox = __uint_as_float(u0.x), oy = __uint_as_float(u0.y), oz = __uint_as_float(u0.z);
+				// exp byte = e + 127, so (byte << 23) is the bit pattern of 2^e.
+				const float sx = __uint_as_float((u0.w & 0xffu) << 23);
+				const float sy = __uint_as_float(((u0.w >> 8) & 0xffu) << 23);
+				const float sz = __uint_as_float(((u0.w >> 16) & 0xffu) << 23);
+				cnt = static_cast<int>(u0.w >> 24);
+#define RGL_QBOX(c, h, k)                                                                                              \
+	slab6(cur, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                                \
+	      __fadd_rn(oz, ubyte(u1.z, c) * sz), __fadd_rn(ox, ubyte(u1.w, c) * sx), __fadd_rn(oy, ubyte(u2.x, c) * sy),  \
+	      __fadd_rn(oz, ubyte(u2.y, c) * sz), best.t, h, k)
+				RGL_QBOX(0, h0, k0);
+				RGL_QBOX(1, h1, k1);
+				RGL_QBOX(2, h2, k2);
+				RGL_QBOX(3, h3, k3);
+#undef RGL_QBOX
+				r0 = static_cast<int>(u2.z);
+				r1 = static_cast<int>(u2.w);
+				r2 = static_cast<int>(u3.x);
+				r3 = static_cast<int>(u3.y);
+			}
+			const float kMiss = INFINITY;
+			k0 = h0 ? k0 : kMiss;
+			k1 = (h1 && cnt > 1) ? k1 : kMiss;
+			k2 = (h2 && cnt > 2) ? k2 : kMiss;
+			k3 = (h3 && cnt > 3) ? k3 : kMiss;
+			RGL_CSWAP(0, 1);
+			RGL_CSWAP(2, 3);
+			RGL_CSWAP(0, 2);
+			RGL_CSWAP(1, 3);
+			RGL_CSWAP(1, 2);
+			if (k3 != kMiss) {
+				RGL_PUSH(r3);
+			}
+			if (k2 != kMiss) {
+				RGL_PUSH(r2);
+			}
+			if (k1 != kMiss) {
+				RGL_PUSH(r1);
+			}
+			next = k0 != kMiss ? r0 : kPop;
 		}
 		while (true) {
 			if (next == kPop) {
@@ -223,7 +322,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				next = RGL_POP();
 				if (next == kSentinel) {
 					cur = world;
-					nodes = sc.tlasNodes;
+					nodes = tlas;
 					inst = -1;
 					next = kPop;
 					continue;
@@ -237,7 +336,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				inst = leaf;
 				const InstanceRec& ir = sc.instances[inst];
 				cur = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
-				nodes = ir.blasNodes;
+				nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
 				tris = ir.blasTris;
 				if (kStats) {
 					st->insts++;
@@ -264,6 +363,9 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		}
 		node = next;
 	}
+#undef RGL_PUSH
+#undef RGL_POP
+#undef RGL_CSWAP
 }
 
 // Beam sample transform (src/gpu/optixPrograms.cu:230-248): rotation about X by the vertical angle times
@@ -431,6 +533,8 @@ struct Tracker
 };
 
 // One beam of one sensor: raygen, traversal, closest hit, beam reduction and output writes.
+// kMulti = false compiles the single-sample path only (beams without divergence); kMulti = true handles both.
+template<int kFmt, bool kMulti>
 __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                              const int beam)
 {
@@ -475,14 +579,14 @@ __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const Raytrace
 	}
 
 	const bool needVel = p.velocityRequested != 0;
-	if (!(p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f)) {
+	if (!kMulti || !(p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f)) {
 		// One sample per beam: the beam reduction degenerates to "every return is the primary sample"
 		// (src/gpu/nodeKernels.cu:345-354,399-421 with a single sample), so no trackers are kept.
 		Vec3f so, sd;
 		rayOriginDir(ray, so, sd);
 		HitRec h;
 		float d0 = p.farNonHitDistance;
-		if (traceClosest<false>(sc, so, sd, range.y, h, nullptr)) {
+		if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
 			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
 			if (sh.isHit) {
 				for (int r = 0; r < R; ++r) {
@@ -501,108 +605,112 @@ __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const Raytrace
 		return;
 	}
 
-	const int samples = kBeamSamples;
-	Tracker first, second, last, strongest, secondStrongest;
-	first.sample = -1;
-	float sample0Distance = p.farNonHitDistance;
-	for (int s = 0; s < samples; ++s) {
-		Mat3x4f sr = ray;
-		if (s > 0) {
-			const int layer = (s - 1) / kBeamVertices;
-			const int vertex = (s - 1) % kBeamVertices;
-			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
+	if constexpr (kMulti) {
+		const int samples = kBeamSamples;
+		Tracker first, second, last, strongest, secondStrongest;
+		first.sample = -1;
+		float sample0Distance = p.farNonHitDistance;
+		for (int s = 0; s < samples; ++s) {
+			Mat3x4f sr = ray;
+			if (s > 0) {
+				const int layer = (s - 1) / kBeamVertices;
+				const int vertex = (s - 1) % kBeamVertices;
+				sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
+			}
+			Vec3f so, sd;
+			rayOriginDir(sr, so, sd);
+			HitRec h;
+			const bool any = traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr);
+			Tracker cand;
+			cand.sample = s;
+			cand.hit = h;
+			int isHit = 0;
+			if (any) {
+				const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
+				isHit = sh.isHit;
+				cand.distance = sh.distance;
+				cand.intensity = sh.intensity;
+			} else {
+				cand.distance = p.farNonHitDistance;
+				cand.intensity = 0.0f;
+			}
+			if (s == 0) {
+				sample0Distance = cand.distance;
+			}
+			if (!isHit) {
+				continue;
+			}
+			if (first.sample < 0) {
+				first = second = last = strongest = secondStrongest = cand;
+				continue;
+			}
+			const float d = cand.distance;
+			if (d < first.distance) {
+				second = first;
+				first = cand;
+			} else if (first.distance < d && d < second.distance) {
+				second = cand;
+			}
+			if (d > last.distance) {
+				last = cand;
+			}
+			const float I = cand.intensity;
+			if (I > strongest.intensity) {
+				secondStrongest = strongest;
+				strongest = cand;
+			} else if (strongest.intensity > I && I > secondStrongest.intensity) {
+				secondStrongest = cand;
+			}
 		}
-		Vec3f so, sd;
-		rayOriginDir(sr, so, sd);
-		HitRec h;
-		const bool any = traceClosest<false>(sc, so, sd, range.y, h, nullptr);
-		Tracker cand;
-		cand.sample = s;
-		cand.hit = h;
-		int isHit = 0;
-		if (any) {
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
-			isHit = sh.isHit;
-			cand.distance = sh.distance;
-			cand.intensity = sh.intensity;
-		} else {
-			cand.distance = p.farNonHitDistance;
-			cand.intensity = 0.0f;
-		}
-		if (s == 0) {
-			sample0Distance = cand.distance;
-		}
-		if (!isHit) {
-			continue;
-		}
-		if (first.sample < 0) {
-			first = second = last = strongest = secondStrongest = cand;
-			continue;
-		}
-		const float d = cand.distance;
-		if (d < first.distance) {
-			second = first;
-			first = cand;
-		} else if (first.distance < d && d < second.distance) {
-			second = cand;
-		}
-		if (d > last.distance) {
-			last = cand;
-		}
-		const float I = cand.intensity;
-		if (I > strongest.intensity) {
-			secondStrongest = strongest;
-			strongest = cand;
-		} else if (strongest.intensity > I && I > secondStrongest.intensity) {
-			secondStrongest = cand;
-		}
-	}
 
-	for (int r = 0; r < R; ++r) {
-		const uint8_t type = returnTypeOf(p.returnMode, r);
-		const int rp = beam * R + r;
-		if (first.sample < 0) {
-			writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
-			continue;
+		for (int r = 0; r < R; ++r) {
+			const uint8_t type = returnTypeOf(p.returnMode, r);
+			const int rp = beam * R + r;
+			if (first.sample < 0) {
+				writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
+				continue;
+			}
+			if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
+				continue;
+			}
+			// Branch-wise copies (not an aggregate select of addresses) let the trackers stay in registers.
+			Tracker t = first; // RGL_RETURN_TYPE_FIRST
+			if (type == 3) {
+				t = second;
+			} else if (type == 2) {
+				t = last;
+			} else if (type == 1) {
+				t = strongest;
+			} else if (type == 5) {
+				t = secondStrongest;
+			}
+			Mat3x4f sr = ray;
+			if (t.sample > 0) {
+				const int layer = (t.sample - 1) / kBeamVertices;
+				const int vertex = (t.sample - 1) % kBeamVertices;
+				sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
+			}
+			Vec3f so, sd;
+			rayOriginDir(sr, so, sd);
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, t.hit, needVel);
+			writeReturnHit(out, p, beam, rp, type, sh);
 		}
-		if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
-			continue;
-		}
-		// Branch-wise copies (not an aggregate select of addresses) let the trackers stay in registers.
-		Tracker t = first; // RGL_RETURN_TYPE_FIRST
-		if (type == 3) {
-			t = second;
-		} else if (type == 2) {
-			t = last;
-		} else if (type == 1) {
-			t = strongest;
-		} else if (type == 5) {
-			t = secondStrongest;
-		}
-		Mat3x4f sr = ray;
-		if (t.sample > 0) {
-			const int layer = (t.sample - 1) / kBeamVertices;
-			const int vertex = (t.sample - 1) % kBeamVertices;
-			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
-		}
-		Vec3f so, sd;
-		rayOriginDir(sr, so, sd);
-		const Shade sh = shadeSample(sc, p, range.x, so, sd, t.hit, needVel);
-		writeReturnHit(out, p, beam, rp, type, sh);
 	}
 }
 
-__global__ void __launch_bounds__(kRtBlock) kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+template<int kFmt, bool kMulti>
+__device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out)
 {
 	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
 	if (beam < p.rayCount) {
-		raytraceBeam(sc, p, out, beam);
+		raytraceBeam<kFmt, kMulti>(sc, p, out, beam);
 	}
 }
 
 // Many sensors (possibly on different scenes) in one launch: block b belongs to the last item whose firstBlock <= b.
 // The item index is wave-uniform, so its descriptor is read with scalar loads.
-__global__ void __launch_bounds__(kRtBlock) kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
+template<int kFmt, bool kMulti>
+__device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items, int itemCount)
 {
 	int lo = 0, hi = itemCount - 1;
 	while (lo < hi) {
@@ -617,8 +725,34 @@ __global__ void __launch_bounds__(kRtBlock) kRaytraceBatch(const RaytraceBatchIt
 	const RaytraceBatchItem& it = items[item];
 	const int beam = (static_cast<int>(blockIdx.x) - it.firstBlock) * kRtBlock + threadIdx.x;
 	if (beam < it.p.rayCount) {
-		raytraceBeam(it.sc, it.p, it.out, beam);
+		raytraceBeam<kFmt, kMulti>(it.sc, it.p, it.out, beam);
 	}
+}
+
+// Single-sample beams (no divergence): occupancy-tuned. Divergent beams keep the compiler's register allocation,
+// since their five return trackers need more registers.
+template<int kFmt>
+__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+{
+	raytraceOne<kFmt, false>(sc, p, out);
+}
+
+template<int kFmt>
+__global__ void __launch_bounds__(kRtBlock) kRaytraceMulti(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+{
+	raytraceOne<kFmt, true>(sc, p, out);
+}
+
+template<int kFmt>
+__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
+{
+	raytraceBatchOne<kFmt, false>(items, itemCount);
+}
+
+template<int kFmt>
+__global__ void __launch_bounds__(kRtBlock) kRaytraceBatchMulti(const RaytraceBatchItem* items, int itemCount)
+{
+	raytraceBatchOne<kFmt, true>(items, itemCount);
 }
 
 __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, RaytraceParams p, int32_t* nodesVisited,
@@ -635,7 +769,7 @@ __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, Raytra
 	TravStats st{0, 0, 0};
 	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
 	if (!masked) {
-		traceClosest<true>(sc, so, sd, range.y, h, &st);
+		traceClosest<true, kBvh2>(sc, so, sd, range.y, h, &st);
 	}
 	nodesVisited[beam] = st.nodes;
 	trisTested[beam] = st.tris;
@@ -686,21 +820,57 @@ __global__ void kTransformRays(int n, const Mat3x4f* in, Mat3x4f* out, Mat3x4f t
 
 } // namespace
 
+namespace {
+template<int kFmt>
+void launchRaytrace(hipStream_t stream, int blocks, bool multi, const SceneView& scene, const RaytraceParams& params,
+                    const RaytraceOutputs& out)
+{
+	if (multi) {
+		hipLaunchKernelGGL(kRaytraceMulti<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, out);
+	} else {
+		hipLaunchKernelGGL(kRaytrace<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, out);
+	}
+}
+
+template<int kFmt>
+void launchRaytraceBatch(hipStream_t stream, int blocks, bool multi, const RaytraceBatchItem* items, int itemCount)
+{
+	if (multi) {
+		hipLaunchKernelGGL(kRaytraceBatchMulti<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, items, itemCount);
+	} else {
+		hipLaunchKernelGGL(kRaytraceBatch<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, items, itemCount);
+	}
+}
+} // namespace
+
+bool beamIsMulti(const RaytraceParams& p) { return p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f; }
+
 void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, const RaytraceOutputs& out)
 {
 	if (params.rayCount <= 0) {
 		return;
 	}
 	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
-	hipLaunchKernelGGL(kRaytrace, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, out);
+	const bool multi = beamIsMulti(params);
+	switch (scene.format) {
+		case kBvh4F: launchRaytrace<kBvh4F>(stream, blocks, multi, scene, params, out); break;
+		case kBvh4Q: launchRaytrace<kBvh4Q>(stream, blocks, multi, scene, params, out); break;
+		default: launchRaytrace<kBvh2>(stream, blocks, multi, scene, params, out); break;
+	}
 }
 
 int raytraceBlocks(int rayCount) { return (rayCount + kRtBlock - 1) / kRtBlock; }
 
-void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks)
+void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
+                   bool anyMulti)
 {
-	if (itemCount > 0 && totalBlocks > 0) {
-		hipLaunchKernelGGL(kRaytraceBatch, dim3(totalBlocks), dim3(kRtBlock), 0, stream, items, itemCount);
+	if (itemCount <= 0 || totalBlocks <= 0) {
+		return;
+	}
+	switch (format) {
+		case kBvh4F: launchRaytraceBatch<kBvh4F>(stream, totalBlocks, anyMulti, items, itemCount); break;
+		case kBvh4Q: launchRaytraceBatch<kBvh4Q>(stream, totalBlocks, anyMulti, items, itemCount); break;
+		default: launchRaytraceBatch<kBvh2>(stream, totalBlocks, anyMulti, items, itemCount); break;
 	}
 }
 

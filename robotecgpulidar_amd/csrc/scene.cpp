@@ -103,6 +103,39 @@ void Mesh::ensureBlas(hipStream_t s)
 	blasBuilt = true;
 }
 
+namespace {
+size_t nodeBytes(int format)
+{
+	return format == kBvh4F ? sizeof(Bvh4FNode) : (format == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+}
+} // namespace
+
+const void* Mesh::nodesFor(hipStream_t s, int format)
+{
+	ensureBlas(s);
+	if (format == kBvh2) {
+		return nodes.get();
+	}
+	if (format != kBvh4F && format != kBvh4Q) {
+		throw std::invalid_argument("unknown BVH node format " + std::to_string(format));
+	}
+	if (!wideBuilt[format]) {
+		const int n = triCount;
+		DeviceBuffer tmp, countDev;
+		tmp.ensure(gpu::bvh4ScratchBytes(n));
+		countDev.ensure(sizeof(int32_t));
+		gpu::bvh4Ranks(s, n, parentSlot.as<int32_t>(), tmp.get(), countDev.as<int32_t>());
+		int32_t m = 0;
+		RGL_HIP(hipMemcpyAsync(&m, countDev.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+		RGL_HIP(hipStreamSynchronize(s));
+		gpu::bvh4Collapse(s, n, nodes.as<BvhNode>(), tmp.get(), format, wide[format].ensure(nodeBytes(format) * m));
+		RGL_HIP(hipGetLastError());
+		RGL_HIP(hipStreamSynchronize(s));
+		wideBuilt[format] = true;
+	}
+	return wide[format].get();
+}
+
 // ------------------------------------------------------------------------------------------------ Texture
 Texture::Texture(const void* texels_, int w, int h) : width(w), height(h)
 {
@@ -155,6 +188,27 @@ std::shared_ptr<Scene> Scene::defaultShared()
 {
 	static std::shared_ptr<Scene> p(&instance(), [](Scene*) {});
 	return p;
+}
+
+namespace {
+int& formatSetting()
+{
+	static int f = kBvh4F; // fastest on gfx950 for the lidar workloads measured (latency-bound traversal)
+	return f;
+}
+} // namespace
+
+int Scene::bvhFormat() { return formatSetting(); }
+
+void Scene::setBvhFormat(int format)
+{
+	if (format != kBvh2 && format != kBvh4F && format != kBvh4Q) {
+		throw std::invalid_argument("unknown BVH node format " + std::to_string(format));
+	}
+	if (format != formatSetting()) {
+		formatSetting() = format;
+		markAllDirty(true);
+	}
 }
 
 void Scene::markAllDirty(bool topologyChanged)
@@ -240,6 +294,7 @@ void Scene::rebuild(hipStream_t s)
 		RGL_HIP(hipStreamWaitEvent(s, ev->get(), 0));
 	}
 	const int n = static_cast<int>(entities.size());
+	const int fmt = bvhFormat();
 	maxBlasDepth = 0;
 	for (auto& e : entities) {
 		e->mesh->ensureBlas(s);
@@ -247,6 +302,7 @@ void Scene::rebuild(hipStream_t s)
 	}
 	if (n == 0) {
 		view = SceneView{};
+		view.format = fmt;
 		view.overflow = overflowFlag();
 		return;
 	}
@@ -255,7 +311,7 @@ void Scene::rebuild(hipStream_t s)
 	for (int i = 0; i < n; ++i) {
 		const Entity& e = *entities[i];
 		inst[i].worldToObject = inverseFp64(e.transform);
-		inst[i].blasNodes = e.mesh->nodes.as<BvhNode>();
+		inst[i].blasNodes = e.mesh->nodesFor(s, fmt);
 		inst[i].blasTris = e.mesh->tris.as<LeafTri>();
 		InstanceShade& sh = shade[i];
 		std::memset(&sh, 0, sizeof(sh));
@@ -267,6 +323,7 @@ void Scene::rebuild(hipStream_t s)
 		sh.indices = e.mesh->indices.as<Vec3i>();
 		sh.uvs = e.mesh->hasUVs ? e.mesh->uvs.as<Vec2f>() : nullptr;
 		sh.texels = e.texture ? e.texture->texels.as<uint8_t>() : nullptr;
+		sh.blas2 = e.mesh->nodes.as<BvhNode>();
 		sh.texWidth = e.texture ? e.texture->width : 0;
 		sh.texHeight = e.texture ? e.texture->height : 0;
 		sh.entityId = e.id;
@@ -292,11 +349,21 @@ void Scene::rebuild(hipStream_t s)
 		gpu::buildLbvh(s, n, dBoxes.as<Aabb>(), true, tlasNodes.as<BvhNode>(), tlasLeafPrim.as<int32_t>(),
 		               tlasParent.as<int32_t>(), tlasCounters.as<int32_t>(), tlasHeight.as<int32_t>(), tlasScratch.get());
 	}
+	view.tlasNodes = tlasNodes.get();
+	view.tlas2 = tlasNodes.as<BvhNode>();
+	if (fmt != kBvh2) {
+		// Upper bound on the 4-wide node count: the binary tree's internal node count.
+		tlasWideScratch.ensure(gpu::bvh4ScratchBytes(n));
+		gpu::bvh4Ranks(s, n, tlasParent.as<int32_t>(), tlasWideScratch.get(), tlasWideCount.ensureT<int32_t>(1));
+		void* wideNodes = tlasWide.ensure(nodeBytes(fmt) * inner);
+		gpu::bvh4Collapse(s, n, tlasNodes.as<BvhNode>(), tlasWideScratch.get(), fmt, wideNodes);
+		view.tlasNodes = wideNodes;
+	}
 	RGL_HIP(hipGetLastError());
-	view.tlasNodes = tlasNodes.as<BvhNode>();
 	view.instances = dInst.as<InstanceRec>();
 	view.shades = dShade.as<InstanceShade>();
 	view.instanceCount = n;
+	view.format = fmt;
 	view.overflow = overflowFlag();
 }
 

@@ -25,6 +25,8 @@ struct Mesh
 	void setTexCoords(const Vec2f* uvs, int count);
 	// Builds the BLAS on `stream` if needed (first use); waits for it to finish.
 	void ensureBlas(hipStream_t stream);
+	// BLAS nodes in `format` (built on first use from the binary BLAS; waits for it to finish).
+	const void* nodesFor(hipStream_t stream, int format);
 
 	int vertexCount = 0;
 	int triCount = 0;
@@ -34,6 +36,8 @@ struct Mesh
 	bool blasBuilt = false;
 	int blasDepth = 0;
 	DeviceBuffer nodes, tris, leafPrim, parentSlot, counters, height, boxes, scratch;
+	DeviceBuffer wide[kBvhFormats]; // 4-wide BLAS per format (index kBvh4F, kBvh4Q)
+	bool wideBuilt[kBvhFormats] = {};
 };
 
 struct Texture
@@ -86,6 +90,9 @@ public:
 	static std::shared_ptr<Scene> defaultShared();
 	// Marks every live scene stale (e.g. a mesh shared by several scenes changed).
 	static void markAllDirty(bool topologyChanged);
+	// Node format used by traversal in every scene (BvhFormat); changing it marks every scene stale.
+	static int bvhFormat();
+	static void setBvhFormat(int format);
 
 	void addEntity(const std::shared_ptr<Entity>& e);
 	void removeEntity(const std::shared_ptr<Entity>& e);
@@ -129,6 +136,7 @@ private:
 	PinnedBuffer hostInst, hostShade;
 	DeviceBuffer dInst, dShade, dBoxes, tlasNodes, tlasLeafPrim, tlasParent, tlasCounters, tlasHeight, tlasScratch;
 	DeviceBuffer dOverflow;
+	DeviceBuffer tlasWide, tlasWideScratch, tlasWideCount; // 4-wide TLAS (when the format is 4-wide)
 	std::map<hipStream_t, std::unique_ptr<Event>> uses;
 	uint64_t readyGeneration = 0;
 	hipStream_t waitedStream = nullptr;

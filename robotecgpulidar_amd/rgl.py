@@ -185,7 +185,10 @@ _PROTOTYPES = [
     ("rgl_graph_run_batch", [P(c_node), C.c_int32]),
     ("rgl_graph_batch_set_kernel_timing", [C.c_bool]),
     ("rgl_graph_batch_get_kernel_timing", [P(C.c_double), i32p]),
+    ("rgl_configure_bvh_format", [C.c_int32]),
+    ("rgl_get_bvh_format", [i32p]),
 ]
+BVH2, BVH4_F32, BVH4_Q8 = 0, 1, 2
 VOID_FUNCS = {"rgl_get_last_error_string"}
 
 _lib = None
@@ -416,6 +419,16 @@ def result(node: Node, field: int) -> np.ndarray:
     if n > 0:
         check("rgl_graph_get_result_data", node.h, int(field), _ptr(out))
     return out
+
+
+def set_bvh_format(fmt: int) -> None:
+    check("rgl_configure_bvh_format", int(fmt))
+
+
+def get_bvh_format() -> int:
+    v = C.c_int32()
+    check("rgl_get_bvh_format", C.byref(v))
+    return v.value
 
 
 def cleanup() -> None:
