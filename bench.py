@@ -135,10 +135,16 @@ def latency_stats(lat_s):
 
 
 def roofline(b_alg: int, b_io: int, kernel_ms: float, traffic):
+    """`achieved` counts algorithmic bytes (canonical binary-BVH node, triangle and instance fetches per ray, each
+    once, plus ray and point I/O): an effective bandwidth that caches can push above the HBM peak. `traffic` is the
+    DRAM bytes rocprof measured for the same kernel, and `dram_frac` the fraction of peak they represent."""
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": b_alg,
-            "io_bytes_per_launch": b_io, "io_only_frac": round(b_io / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": b_alg,
+           "io_bytes_per_launch": b_io, "io_only_frac": round(b_io / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    if traffic:
+        out["dram_frac"] = round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
 
 
 def traffic_bytes(name: str):
