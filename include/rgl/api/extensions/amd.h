@@ -64,5 +64,56 @@ RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable);
 // Total device time (ms) and number of batched raytrace launches timed since timing was enabled; waits for them.
 RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int32_t* out_launches);
 
+// Radar range x azimuth binning (build-defined; the reference instead clusters radar hits by scopes with
+// physical-optics RCS, src/graph/RadarPostprocessPointsNode.cpp:88-189). Input: XYZ, DISTANCE, INCIDENT_ANGLE and
+// IS_HIT. A hit at distance d whose point has sensor-frame azimuth a = atan2(y, x) (sensor frame = the node's
+// look-at origin, i.e. before the rays' cumulative transform) falls in cell (floor(d / range_step),
+// floor((a - azimuth_min) / azimuth_step)); hits outside [0, range_max) x [azimuth_min, azimuth_max) are dropped.
+// Its RCS contribution is d^2 * ray_solid_angle * cos(incident angle) m^2 (the ray's footprint area projected
+// back along the ray), rounded to 2^-24 m^2 and summed exactly per cell. Output: one point per non-empty cell, in
+// cell order (range-major): XYZ (cell centre in the sensor frame, z = 0), DISTANCE and AZIMUTH (cell centre),
+// RCS_F32 (10 log10 of the summed contributions, dBsm), IS_HIT (1).
+RGL_API rgl_status_t rgl_node_points_radar_bin(rgl_node_t* node, float range_step, float range_max, float azimuth_step,
+                                               float azimuth_min, float azimuth_max, float ray_solid_angle);
+
+// ---------------------------------------------------------------------------------------------------- serving
+// A request server doing continuous batching of sensor frames (one graph per sensor) on the calling thread.
+// Requests queue in FIFO order; rgl_server_step retires finished batches and launches every admissible queued
+// request into each free slot as one batch (rgl_graph_run_batch semantics), so up to `slots` batches overlap on the
+// device. A request is admissible when its graph has no earlier request in flight and the acceleration structures
+// of the scenes it raytraces fit `hbm_budget_bytes` (<= 0: unlimited) after evicting least recently used idle
+// scenes (evicted scenes rebuild on their next use); a request whose scenes alone exceed the budget fails (state 3).
+// With `batching` false every request runs alone through
+// rgl_graph_run semantics (one per slot): the plain per-sensor path under the same queueing.
+// A request's results are read with rgl_graph_get_result_* once it is done, until its graph runs again.
+typedef struct RglServer* rgl_server_t;
+
+typedef struct
+{
+	int64_t submitted;
+	int64_t completed;
+	int64_t batches;
+	int64_t rays_completed;  // rays of completed requests (sum over their rays nodes)
+	int64_t evictions;       // scenes whose acceleration structures were evicted
+	int64_t deferred;        // times a queued request had to wait for its graph or for scene memory
+	int64_t failed;          // requests whose scenes alone exceed the HBM budget (state 3)
+	int64_t resident_bytes;  // acceleration-structure bytes of the admitted scenes
+	double p50_ms, p99_ms, mean_ms, max_ms; // request latency, submit to retirement
+} rgl_server_stats_t;
+
+RGL_API rgl_status_t rgl_server_create(rgl_server_t* out_server, int32_t slots, int32_t max_batch_graphs,
+                                       int64_t hbm_budget_bytes, bool batching);
+RGL_API rgl_status_t rgl_server_destroy(rgl_server_t server); // waits for queued and running requests
+// Queues one frame of the graph containing `node`; returns its ticket.
+RGL_API rgl_status_t rgl_server_submit(rgl_server_t server, rgl_node_t node, int64_t* out_ticket);
+// Retires finished batches and launches queued requests into free slots.
+RGL_API rgl_status_t rgl_server_step(rgl_server_t server, int32_t* out_retired);
+// State of a ticket: 0 queued, 1 running, 2 done, 3 failed; latency in ms once done.
+RGL_API rgl_status_t rgl_server_poll(rgl_server_t server, int64_t ticket, int32_t* out_state, double* out_latency_ms);
+// Steps until the ticket is done.
+RGL_API rgl_status_t rgl_server_wait(rgl_server_t server, int64_t ticket, double* out_latency_ms);
+RGL_API rgl_status_t rgl_server_get_stats(rgl_server_t server, rgl_server_stats_t* out_stats);
+RGL_API rgl_status_t rgl_server_reset_stats(rgl_server_t server);
+
 // Standard normal variates normal(seed, run, tag, i) for i in [0, count), computed on the GPU.
 RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);

@@ -327,3 +327,54 @@ def noise_angular_rays(rays, normals_, mean, st_dev, axis, look_at):
     lr = mm(np.broadcast_to(L, rays.shape), rays)
     rr = np.concatenate([R, np.zeros((len(R), 3, 1))], axis=2)
     return mm(np.broadcast_to(Li, rays.shape), mm(rr, lr))
+
+
+RCS_FIXED_SCALE = 2.0 ** 24
+
+
+def radar_bin(xyz_sensor, distance, incident, is_hit, range_step, range_max, azimuth_step, azimuth_min, azimuth_max,
+              ray_solid_angle, eps_range=1e-6, eps_azimuth=1e-4):
+    """Range x azimuth binning with RCS (the build-defined radar binning stage), in FP64.
+
+    A hit (is_hit != 0) at distance d with sensor-frame point p falls in cell (floor(d / range_step),
+    floor((atan2(p.y, p.x) - azimuth_min) / azimuth_step)) when inside [0, range_max) x [azimuth_min, azimuth_max);
+    its RCS contribution d^2 * ray_solid_angle * cos(incident) is rounded to 2^-24 m^2 and summed per cell.
+    Returns (cells, counts, rcs_db, alt_cells): sorted non-empty cell indices (range-major), hit counts, 10 log10 of
+    the summed contributions, and per input point the cells it could fall in under FP32 rounding other than its own
+    (a list of up to three cell indices; empty when its range position is farther than eps_range (relative) and its
+    azimuth position farther than eps_azimuth (in cells) from a cell edge)."""
+    d = np.asarray(distance, np.float64)
+    p = np.asarray(xyz_sensor, np.float64)
+    inc = np.asarray(incident, np.float64)
+    hit = np.asarray(is_hit) != 0
+    # The stage receives its parameters as FP32 (rgl_node_points_radar_bin takes floats).
+    range_step, range_max, azimuth_step, azimuth_min, azimuth_max, ray_solid_angle = (
+        float(np.float32(v)) for v in (range_step, range_max, azimuth_step, azimuth_min, azimuth_max, ray_solid_angle))
+    rc = int(np.ceil(range_max / range_step))
+    ac = int(np.ceil((azimuth_max - azimuth_min) / azimuth_step))
+    az = np.arctan2(p[:, 1], p[:, 0])
+    rpos = d / range_step
+    apos = (az - azimuth_min) / azimuth_step
+    rf, af = np.floor(rpos), np.floor(apos)
+    inside = hit & (rf >= 0) & (rf < rc) & (af >= 0) & (af < ac)
+    r_near = hit & (np.abs(rpos - np.round(rpos)) <= eps_range * np.maximum(np.abs(rpos), 1.0))
+    a_near = hit & (np.abs(apos - np.round(apos)) <= eps_azimuth)
+    r_alt = np.where(rpos - rf < 0.5, rf - 1, rf + 1)
+    a_alt = np.where(apos - af < 0.5, af - 1, af + 1)
+    alt_cells = []
+    for i in np.nonzero(r_near | a_near)[0]:
+        rs = [rf[i]] + ([r_alt[i]] if r_near[i] else [])
+        as_ = [af[i]] + ([a_alt[i]] if a_near[i] else [])
+        alt_cells.append((i, [int(r * ac + a) for r in rs for a in as_ if (r, a) != (rf[i], af[i])]))
+    cell = (rf * ac + af)[inside].astype(np.int64)
+    sigma = d[inside] ** 2 * ray_solid_angle * np.cos(inc[inside])
+    q = np.rint(sigma * RCS_FIXED_SCALE).astype(np.int64)
+    cells, inv = np.unique(cell, return_inverse=True)
+    counts = np.bincount(inv, minlength=len(cells)).astype(np.int64)
+    sums = np.bincount(inv, weights=q.astype(np.float64), minlength=len(cells))
+    with np.errstate(divide="ignore"):
+        rcs_db = 10.0 * np.log10(sums / RCS_FIXED_SCALE)
+    alt = [[] for _ in range(len(d))]
+    for i, cs in alt_cells:
+        alt[i] = cs
+    return cells, counts, rcs_db, alt

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "graph.hpp"
+#include "server.hpp"
 #include "scene.hpp"
 
 using namespace rgl;
@@ -313,6 +314,10 @@ RGL_API void rgl_get_last_error_string(const char** out_error_string)
 RGL_API rgl_status_t rgl_cleanup(void)
 {
 	return safeCall([&]() {
+		for (auto& [h, sv] : Registry<rgl::Server>::map()) {
+			sv->drain();
+		}
+		Registry<rgl::Server>::map().clear();
 		GraphBatch::instance().synchronize();
 		for (auto& n : allNodes()) {
 			if (n->hasCtx()) {
@@ -745,6 +750,17 @@ RGL_API rgl_status_t rgl_node_points_compact_by_field(rgl_node_t* node, rgl_fiel
 
 RGL_API rgl_status_t rgl_node_points_compact(rgl_node_t* node) { return rgl_node_points_compact_by_field(node, RGL_FIELD_IS_HIT_I32); }
 
+RGL_API rgl_status_t rgl_node_points_radar_bin(rgl_node_t* node, float range_step, float range_max, float azimuth_step,
+                                               float azimuth_min, float azimuth_max, float ray_solid_angle)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		createOrUpdateNode<RadarBinPointsNode>(node, "RadarBinPointsNode", false, [&](RadarBinPointsNode& n) {
+			n.setParameters(range_step, range_max, azimuth_step, azimuth_min, azimuth_max, ray_solid_angle);
+		});
+	});
+}
+
 RGL_API rgl_status_t rgl_node_points_spatial_merge(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
 {
 	return safeCall([&]() {
@@ -1063,6 +1079,91 @@ RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene)
 		}
 		sc->clear();
 		Registry<rgl::Scene>::release(scene, "Scene");
+	});
+}
+
+// ============================================================================================ serving
+RGL_API rgl_status_t rgl_server_create(rgl_server_t* out_server, int32_t slots, int32_t max_batch_graphs,
+                                       int64_t hbm_budget_bytes, bool batching)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_server != nullptr);
+		CHECK_ARG(slots > 0 && max_batch_graphs > 0);
+		auto sv = std::make_shared<rgl::Server>(slots, max_batch_graphs, hbm_budget_bytes, batching);
+		Registry<rgl::Server>::add(sv);
+		*out_server = reinterpret_cast<rgl_server_t>(sv.get());
+	});
+}
+
+RGL_API rgl_status_t rgl_server_destroy(rgl_server_t server)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr);
+		Registry<rgl::Server>::get(server, "Server")->drain();
+		Registry<rgl::Server>::release(server, "Server");
+	});
+}
+
+RGL_API rgl_status_t rgl_server_submit(rgl_server_t server, rgl_node_t node, int64_t* out_ticket)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr && node != nullptr && out_ticket != nullptr);
+		*out_ticket = Registry<rgl::Server>::get(server, "Server")->submit(lookupNode(node));
+	});
+}
+
+RGL_API rgl_status_t rgl_server_step(rgl_server_t server, int32_t* out_retired)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr);
+		const int n = Registry<rgl::Server>::get(server, "Server")->step();
+		if (out_retired != nullptr) {
+			*out_retired = n;
+		}
+	});
+}
+
+RGL_API rgl_status_t rgl_server_poll(rgl_server_t server, int64_t ticket, int32_t* out_state, double* out_latency_ms)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr && out_state != nullptr);
+		*out_state = Registry<rgl::Server>::get(server, "Server")->state(ticket, out_latency_ms);
+	});
+}
+
+RGL_API rgl_status_t rgl_server_wait(rgl_server_t server, int64_t ticket, double* out_latency_ms)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr);
+		Registry<rgl::Server>::get(server, "Server")->wait(ticket, out_latency_ms);
+	});
+}
+
+RGL_API rgl_status_t rgl_server_get_stats(rgl_server_t server, rgl_server_stats_t* out_stats)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr && out_stats != nullptr);
+		const rgl::ServerStats st = Registry<rgl::Server>::get(server, "Server")->stats();
+		out_stats->submitted = st.submitted;
+		out_stats->completed = st.completed;
+		out_stats->batches = st.batches;
+		out_stats->rays_completed = st.raysCompleted;
+		out_stats->evictions = st.evictions;
+		out_stats->deferred = st.deferred;
+		out_stats->failed = st.failed;
+		out_stats->resident_bytes = st.residentBytes;
+		out_stats->p50_ms = st.p50Ms;
+		out_stats->p99_ms = st.p99Ms;
+		out_stats->mean_ms = st.meanMs;
+		out_stats->max_ms = st.maxMs;
+	});
+}
+
+RGL_API rgl_status_t rgl_server_reset_stats(rgl_server_t server)
+{
+	return safeCall([&]() {
+		CHECK_ARG(server != nullptr);
+		Registry<rgl::Server>::get(server, "Server")->resetStats();
 	});
 }
 

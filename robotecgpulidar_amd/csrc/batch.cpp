@@ -91,6 +91,32 @@ std::shared_ptr<Event> GraphBatch::completionEvent()
 	return eventPool.back();
 }
 
+GraphBatch::GraphBatch() = default;
+
+GraphBatch::~GraphBatch()
+{
+	if (stream_) {
+		(void) hipStreamSynchronize(stream_->get());
+	}
+	for (auto& [a, b] : timingEvents) {
+		(void) hipEventDestroy(a);
+		(void) hipEventDestroy(b);
+	}
+}
+
+bool GraphBatch::busy() const
+{
+	if (!lastCompletion) {
+		return false;
+	}
+	const hipError_t e = hipEventQuery(lastCompletion->get());
+	if (e == hipErrorNotReady) {
+		return true;
+	}
+	RGL_HIP(e);
+	return false;
+}
+
 void GraphBatch::synchronize()
 {
 	if (stream_) {
@@ -405,8 +431,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		auto ev = completionEvent();
 		ev->record(s);
 		for (auto& c : pl.ctxs) {
-			c->endBatch(ev);
+			c->endBatch(s, ev);
 		}
+		lastCompletion = ev;
 		throw;
 	}
 	auto ev = completionEvent();
@@ -415,8 +442,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		for (auto& n : c->validatedOrder()) {
 			n->markEnqueued(ev);
 		}
-		c->endBatch(ev);
+		c->endBatch(s, ev);
 	}
+	lastCompletion = ev;
 }
 
 } // namespace rgl

@@ -184,6 +184,22 @@ struct FieldCopyList
 	int32_t count;
 };
 
+// ------------------------------------------------------------------------------------------------ radar cells
+// Range x azimuth binning of radar hits (build-defined). Cell (r, a) covers distances [r*rangeStep, (r+1)*rangeStep)
+// and sensor-frame azimuths [azMin + a*azStep, azMin + (a+1)*azStep).
+struct RadarBinParams
+{
+	float rangeStep;
+	float azStep;
+	float azMin;
+	float raySolidAngle; // steradians per ray
+	int32_t rangeCells;
+	int32_t azCells;
+};
+// Fixed-point scale of the RCS sums: contributions are rounded to multiples of 2^-24 m^2 and summed as 64-bit
+// integers, so a cell's sum does not depend on the order in which hits are added.
+constexpr double kRcsFixedScale = 16777216.0;
+
 // ------------------------------------------------------------------------------------------------ batches
 // One sensor of a batched raytrace launch; blocks [firstBlock, next item's firstBlock) trace its beams.
 struct RaytraceBatchItem

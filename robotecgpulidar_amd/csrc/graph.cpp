@@ -303,17 +303,35 @@ void GraphRunCtx::beginBatch(hipStream_t batchStream, Event& scratch)
 		scratch.record(stream_->get());
 		RGL_HIP(hipStreamWaitEvent(batchStream, scratch.get(), 0));
 	}
+	if (batchDone && batchDoneStream != batchStream) {
+		// The previous batch ran on another batch stream.
+		RGL_HIP(hipStreamWaitEvent(batchStream, batchDone->get(), 0));
+	}
 	streamOverride = batchStream;
 	running = true;
 	lastRunSolo = false;
 }
 
-void GraphRunCtx::endBatch(std::shared_ptr<Event> completion)
+void GraphRunCtx::endBatch(hipStream_t batchStream, std::shared_ptr<Event> completion)
 {
 	streamOverride = nullptr;
 	if (completion) {
 		batchDone = std::move(completion);
+		batchDoneStream = batchStream;
 	}
+}
+
+bool GraphRunCtx::inFlight() const
+{
+	if (!batchDone || !batchDone->wasRecorded()) {
+		return false;
+	}
+	const hipError_t e = hipEventQuery(batchDone->get());
+	if (e == hipErrorNotReady) {
+		return true;
+	}
+	RGL_HIP(e);
+	return false;
 }
 
 void GraphRunCtx::synchronize()
@@ -958,6 +976,78 @@ void FormatPointsNode::enqueueImpl()
 const void* FormatPointsNode::fieldData(rgl_field_t f)
 {
 	return f == RGL_FIELD_DYNAMIC_FORMAT ? out.get() : input->fieldData(f);
+}
+
+// ------------------------------------------------------------------------------------------------ radar cells
+namespace {
+const rgl_field_t kRadarFields[] = {RGL_FIELD_XYZ_VEC3_F32, RGL_FIELD_DISTANCE_F32, RGL_FIELD_AZIMUTH_F32,
+                                    RGL_FIELD_RCS_F32, RGL_FIELD_IS_HIT_I32};
+}
+
+void RadarBinPointsNode::setParameters(float rangeStep, float rangeMax, float azStep, float azMin, float azMax,
+                                       float raySolidAngle)
+{
+	if (!(rangeStep > 0.0f) || !(azStep > 0.0f) || !(rangeMax > 0.0f) || !(azMax > azMin) || !(raySolidAngle > 0.0f)) {
+		throw std::invalid_argument("radar binning needs positive steps, range and solid angle and azimuth_max > "
+		                            "azimuth_min");
+	}
+	const double rc = std::ceil(static_cast<double>(rangeMax) / rangeStep);
+	const double ac = std::ceil((static_cast<double>(azMax) - azMin) / azStep);
+	if (rc * ac > 64.0 * 1024 * 1024) {
+		throw std::invalid_argument("radar binning grid exceeds 64M cells");
+	}
+	bp = RadarBinParams{rangeStep, azStep, azMin, raySolidAngle, static_cast<int32_t>(rc), static_cast<int32_t>(ac)};
+}
+
+bool RadarBinPointsNode::hasField(rgl_field_t f) const
+{
+	return std::find(std::begin(kRadarFields), std::end(kRadarFields), f) != std::end(kRadarFields);
+}
+
+std::vector<rgl_field_t> RadarBinPointsNode::fieldList() const
+{
+	return {std::begin(kRadarFields), std::end(kRadarFields)};
+}
+
+void RadarBinPointsNode::enqueueImpl()
+{
+	const int cells = capacity();
+	FieldCopyList list{};
+	for (rgl_field_t f : kRadarFields) {
+		const size_t sz = fieldSizeChecked(f);
+		list.f[list.count++] = FieldCopy{static_cast<const char*>(dense[f].ensure(sz * cells)),
+		                                 static_cast<char*>(out[f].ensure(sz * cells)), static_cast<int32_t>(sz), 0};
+	}
+	const int cap = input->capacity();
+	gpu::radarBin(stream(), cap, input->countDevice(),
+	              static_cast<const Vec3f*>(input->fieldData(RGL_FIELD_XYZ_VEC3_F32)),
+	              static_cast<const float*>(input->fieldData(RGL_FIELD_DISTANCE_F32)),
+	              static_cast<const float*>(input->fieldData(RGL_FIELD_INCIDENT_ANGLE_F32)),
+	              static_cast<const int32_t*>(input->fieldData(RGL_FIELD_IS_HIT_I32)), input->lookAtOriginTransform(), bp,
+	              sums.ensureT<unsigned long long>(cells), counts.ensureT<uint32_t>(cells),
+	              dense[RGL_FIELD_XYZ_VEC3_F32].as<Vec3f>(), dense[RGL_FIELD_DISTANCE_F32].as<float>(),
+	              dense[RGL_FIELD_AZIMUTH_F32].as<float>(), dense[RGL_FIELD_RCS_F32].as<float>(),
+	              dense[RGL_FIELD_IS_HIT_I32].as<int32_t>(), keep.ensureT<int32_t>(cells));
+	gpu::compact(stream(), cells, nullptr, keep.as<int32_t>(), list, tileCounts.ensureT<int32_t>(gpu::compactTiles(cells)),
+	             count.ensureT<int32_t>(1));
+	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipMemcpyAsync(countHost.ensure(sizeof(int32_t)), count.get(), sizeof(int32_t), hipMemcpyDeviceToHost,
+	                       stream()));
+}
+
+int RadarBinPointsNode::hostCount()
+{
+	waitForResults();
+	return *static_cast<int32_t*>(countHost.get());
+}
+
+const void* RadarBinPointsNode::fieldData(rgl_field_t f)
+{
+	auto it = out.find(f);
+	if (it == out.end() || !hasField(f)) {
+		throw InvalidPipeline("RadarBinPointsNode does not provide field " + fieldName(f));
+	}
+	return it->second.get();
 }
 
 // src/graph/FromArrayPointsNode.cpp:18-46

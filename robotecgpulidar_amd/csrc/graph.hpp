@@ -182,7 +182,9 @@ struct GraphRunCtx : std::enable_shared_from_this<GraphRunCtx>
 	// Batched runs: orders `batchStream` after this graph's earlier work on its own stream and makes every node
 	// enqueue on `batchStream` until endBatch(), which hands the batch's completion event to later runs.
 	void beginBatch(hipStream_t batchStream, Event& scratch);
-	void endBatch(std::shared_ptr<Event> completion);
+	void endBatch(hipStream_t batchStream, std::shared_ptr<Event> completion);
+	// The graph's last batch is still executing.
+	bool inFlight() const;
 
 	static std::vector<Node::Ptr> findExecutionOrder(std::set<Node::Ptr> nodes);
 
@@ -194,16 +196,25 @@ private:
 	bool running = false;
 	bool lastRunSolo = false;
 	std::shared_ptr<Event> batchDone; // completion of the last batch this graph ran in
+	hipStream_t batchDoneStream = nullptr;
 };
 
 // Runs several independent graphs as one batch: stages of the same kind at the same depth of every graph execute in
 // one launch (raytrace, elementwise point/ray stages, compaction), on one stream.
 struct GraphBatch
 {
+	// The batch behind rgl_graph_run_batch. Servers own further instances (one stream each) to keep several
+	// batches in flight.
 	static GraphBatch& instance();
+	GraphBatch();
+	GraphBatch(const GraphBatch&) = delete;
+	GraphBatch& operator=(const GraphBatch&) = delete;
+	~GraphBatch();
 	// Runs the graphs that contain `entryNodes` (any node of each graph; duplicates are allowed).
 	void run(const std::vector<Node::Ptr>& entryNodes);
 	void synchronize();
+	// True while the last run's work is still executing on the device.
+	bool busy() const;
 	// HIP-event timing of the batched raytrace launches while enabled.
 	void setTiming(bool enable);
 	void getTiming(double* totalMs, int32_t* launches);
@@ -214,6 +225,7 @@ private:
 	std::shared_ptr<Event> completionEvent();
 
 	std::unique_ptr<Plan> plan;
+	std::shared_ptr<Event> lastCompletion;
 	std::unique_ptr<Stream> stream_;
 	std::unique_ptr<Event> uploaded, scratchEvent;
 	PinnedBuffer staging;
@@ -552,6 +564,37 @@ private:
 	FieldCopyList makeLayout();
 	std::vector<rgl_field_t> fields;
 	DeviceBuffer out;
+};
+
+// Radar range x azimuth cell binning with RCS (build-defined; the reference clusters radar hits by scopes instead,
+// src/graph/RadarPostprocessPointsNode.cpp:88-130). One output point per non-empty cell, in cell order.
+struct RadarBinPointsNode : IPointsNodeSingleInput
+{
+	void setParameters(float rangeStep, float rangeMax, float azStep, float azMin, float azMax, float raySolidAngle);
+	std::string getName() const override { return "RadarBinPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override
+	{
+		return {RGL_FIELD_XYZ_VEC3_F32, RGL_FIELD_DISTANCE_F32, RGL_FIELD_INCIDENT_ANGLE_F32, RGL_FIELD_IS_HIT_I32};
+	}
+	bool isDense() const override { return true; }
+	bool hasField(rgl_field_t f) const override;
+	std::vector<rgl_field_t> fieldList() const override;
+	int capacity() const override { return bp.rangeCells * bp.azCells; }
+	const int32_t* countDevice() const override { return count.as<int32_t>(); }
+	int hostCount() override;
+	rgl_return_mode_t returnMode() const override { return RGL_RETURN_UNKNOWN; }
+	int returnCount() const override { return 1; }
+	Mat3x4f lookAtOriginTransform() const override { return Mat3x4f::identity(); } // cells are in the sensor frame
+	const void* fieldData(rgl_field_t f) override;
+
+protected:
+	void enqueueImpl() override;
+
+private:
+	RadarBinParams bp{0.2f, 0.017453292f, -1.0471976f, 1e-4f, 1, 1};
+	DeviceBuffer sums, counts, keep, count, tileCounts;
+	std::map<rgl_field_t, DeviceBuffer> dense, out;
+	PinnedBuffer countHost;
 };
 
 struct FromArrayPointsNode : IPointsNode

@@ -76,6 +76,12 @@ void formatSoaToAos(hipStream_t stream, int capacity, const int32_t* count, int 
                     char* out);
 void formatAosToSoa(hipStream_t stream, int n, int recordSize, const char* in, const FieldCopyList& fields);
 void fillInt(hipStream_t stream, int n, int32_t* p, int32_t value);
+// Radar range x azimuth binning: dense per-cell outputs (rangeCells * azCells entries) and keep flags; the caller
+// compacts the non-empty cells. sums / counts are scratch of one entry per cell.
+void radarBin(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* xyz, const float* distance,
+              const float* incident, const int32_t* isHit, const Mat3x4f& toSensor, const RadarBinParams& bp,
+              unsigned long long* sums, uint32_t* counts, Vec3f* cellXyz, float* cellDistance, float* cellAzimuth,
+              float* cellRcs, int32_t* cellHit, int32_t* keep);
 
 } // namespace gpu
 } // namespace rgl

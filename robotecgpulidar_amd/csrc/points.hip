@@ -339,6 +339,62 @@ __global__ void kGatherInts(int n, const int32_t* const* src, int32_t* dst)
 	}
 }
 
+// ------------------------------------------------------------------------------------------------ radar cells
+__global__ void kRadarClear(int cells, unsigned long long* sums, uint32_t* counts)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < cells) {
+		sums[i] = 0ull;
+		counts[i] = 0u;
+	}
+}
+
+// One thread per point: cell from the FP32 distance and the sensor-frame azimuth atan2(y, x); contribution
+// d^2 * solid angle * cos(incidence), rounded to the fixed-point grid and added with a 64-bit integer atomic.
+__global__ void kRadarBin(int capacity, const int32_t* count, const Vec3f* xyz, const float* distance,
+                          const float* incident, const int32_t* isHit, Mat3x4f toSensor, RadarBinParams bp,
+                          unsigned long long* sums, uint32_t* counts)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= liveCount(capacity, count) || isHit[i] == 0) {
+		return;
+	}
+	const float d = distance[i];
+	const Vec3f p = toSensor * xyz[i];
+	const float az = atan2f(p.y, p.x);
+	const float rf = floorf(d / bp.rangeStep);
+	const float af = floorf((az - bp.azMin) / bp.azStep);
+	if (!(rf >= 0.0f && rf < static_cast<float>(bp.rangeCells) && af >= 0.0f && af < static_cast<float>(bp.azCells))) {
+		return;
+	}
+	const int cell = static_cast<int>(rf) * bp.azCells + static_cast<int>(af);
+	const float sigma = d * d * bp.raySolidAngle * cosf(incident[i]);
+	const unsigned long long q = static_cast<unsigned long long>(llrint(static_cast<double>(sigma) * kRcsFixedScale));
+	atomicAdd(&sums[cell], q);
+	atomicAdd(&counts[cell], 1u);
+}
+
+// Dense per-cell outputs and the keep flag of each cell (non-empty).
+__global__ void kRadarCells(RadarBinParams bp, const unsigned long long* sums, const uint32_t* counts, Vec3f* xyz,
+                            float* distance, float* azimuth, float* rcs, int32_t* isHit, int32_t* keep)
+{
+	const int c = blockIdx.x * kBlock + threadIdx.x;
+	const int cells = bp.rangeCells * bp.azCells;
+	if (c >= cells) {
+		return;
+	}
+	const int r = c / bp.azCells, a = c % bp.azCells;
+	const float dist = (static_cast<float>(r) + 0.5f) * bp.rangeStep;
+	const float az = bp.azMin + (static_cast<float>(a) + 0.5f) * bp.azStep;
+	xyz[c] = Vec3f{dist * cosf(az), dist * sinf(az), 0.0f};
+	distance[c] = dist;
+	azimuth[c] = az;
+	const double sum = static_cast<double>(sums[c]) / kRcsFixedScale;
+	rcs[c] = sum > 0.0 ? static_cast<float>(10.0 * log10(sum)) : -INFINITY;
+	isHit[c] = 1;
+	keep[c] = counts[c] > 0u ? 1 : 0;
+}
+
 __global__ void kFillInt(int n, int32_t* p, int32_t v)
 {
 	const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -427,6 +483,21 @@ void fillInt(hipStream_t stream, int n, int32_t* p, int32_t value)
 	if (n > 0) {
 		hipLaunchKernelGGL(kFillInt, dim3(blocks(n)), dim3(kBlock), 0, stream, n, p, value);
 	}
+}
+
+void radarBin(hipStream_t stream, int capacity, const int32_t* count, const Vec3f* xyz, const float* distance,
+              const float* incident, const int32_t* isHit, const Mat3x4f& toSensor, const RadarBinParams& bp,
+              unsigned long long* sums, uint32_t* counts, Vec3f* cellXyz, float* cellDistance, float* cellAzimuth,
+              float* cellRcs, int32_t* cellHit, int32_t* keep)
+{
+	const int cells = bp.rangeCells * bp.azCells;
+	hipLaunchKernelGGL(kRadarClear, dim3(blocks(cells)), dim3(kBlock), 0, stream, cells, sums, counts);
+	if (capacity > 0) {
+		hipLaunchKernelGGL(kRadarBin, dim3(blocks(capacity)), dim3(kBlock), 0, stream, capacity, count, xyz, distance,
+		                   incident, isHit, toSensor, bp, sums, counts);
+	}
+	hipLaunchKernelGGL(kRadarCells, dim3(blocks(cells)), dim3(kBlock), 0, stream, bp, sums, counts, cellXyz, cellDistance,
+	                   cellAzimuth, cellRcs, cellHit, keep);
 }
 
 } // namespace rgl::gpu

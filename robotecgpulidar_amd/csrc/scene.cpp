@@ -136,6 +136,39 @@ const void* Mesh::nodesFor(hipStream_t s, int format)
 	return wide[format].get();
 }
 
+int64_t Mesh::accelBytes(int format) const
+{
+	if (blasBuilt) {
+		int64_t b = 0;
+		for (const DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch}) {
+			b += static_cast<int64_t>(d->capacity());
+		}
+		for (const DeviceBuffer& w : wide) {
+			b += static_cast<int64_t>(w.capacity());
+		}
+		return b;
+	}
+	// Binary nodes 64 B, leaf triangles 48 B, leaf order 4 B, parent links 8 B, fit counters and heights 8 B,
+	// primitive boxes 24 B per triangle; at most one 4-wide node per binary internal node (an upper bound: about half
+	// of them are kept in practice).
+	const int64_t t = triCount;
+	const int64_t wideBytes = format == kBvh4F ? 128 * t : (format == kBvh4Q ? 64 * t : 0);
+	return t * (64 + 48 + 4 + 8 + 8 + 24) + wideBytes;
+}
+
+void Mesh::releaseAccel()
+{
+	for (DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch}) {
+		d->release();
+	}
+	for (int f = 0; f < kBvhFormats; ++f) {
+		wide[f].release();
+		wideBuilt[f] = false;
+	}
+	blasBuilt = false;
+	blasDepth = 0;
+}
+
 // ------------------------------------------------------------------------------------------------ Texture
 Texture::Texture(const void* texels_, int w, int h) : width(w), height(h)
 {
@@ -274,6 +307,27 @@ int32_t* Scene::overflowFlag()
 		RGL_HIP(hipMemset(dOverflow.get(), 0, sizeof(int32_t)));
 	}
 	return dOverflow.as<int32_t>();
+}
+
+std::vector<Mesh*> Scene::meshes() const
+{
+	std::vector<Mesh*> out;
+	for (auto& e : entities) {
+		if (std::find(out.begin(), out.end(), e->mesh.get()) == out.end()) {
+			out.push_back(e->mesh.get());
+		}
+	}
+	return out;
+}
+
+int64_t Scene::accelBytes(int format) const
+{
+	int64_t b = 0;
+	for (Mesh* m : meshes()) {
+		b += m->accelBytes(format);
+	}
+	const int64_t n = static_cast<int64_t>(entities.size());
+	return b + n * (64 + 160 + 24 + 64 + 4 + 8 + 8) + n * 128;
 }
 
 std::vector<int32_t> Scene::instanceTriCounts() const

@@ -27,6 +27,10 @@ struct Mesh
 	void ensureBlas(hipStream_t stream);
 	// BLAS nodes in `format` (built on first use from the binary BLAS; waits for it to finish).
 	const void* nodesFor(hipStream_t stream, int format);
+	// Device bytes of the acceleration structures (built, or estimated for `format` when not built yet).
+	int64_t accelBytes(int format) const;
+	// Frees the acceleration structures (rebuilt on next use). The caller makes every scene using the mesh stale.
+	void releaseAccel();
 
 	int vertexCount = 0;
 	int triCount = 0;
@@ -118,6 +122,10 @@ public:
 	int32_t* overflowFlag();
 	size_t entityCount() const { return entities.size(); }
 	const std::vector<std::shared_ptr<Entity>>& getEntities() const { return entities; }
+	// Distinct meshes of the scene's entities.
+	std::vector<Mesh*> meshes() const;
+	// Device bytes of the scene's acceleration structures (BLASes of its meshes plus TLAS), built or estimated.
+	int64_t accelBytes(int format) const;
 	std::vector<int32_t> instanceTriCounts() const;
 
 	TlasUpdate tlasUpdate = TlasUpdate::Rebuild;

@@ -187,6 +187,15 @@ _PROTOTYPES = [
     ("rgl_graph_batch_get_kernel_timing", [P(C.c_double), i32p]),
     ("rgl_configure_bvh_format", [C.c_int32]),
     ("rgl_get_bvh_format", [i32p]),
+    ("rgl_node_points_radar_bin", [P(c_node)] + [C.c_float] * 6),
+    ("rgl_server_create", [P(C.c_void_p), C.c_int32, C.c_int32, C.c_int64, C.c_bool]),
+    ("rgl_server_destroy", [C.c_void_p]),
+    ("rgl_server_submit", [C.c_void_p, c_node, P(C.c_int64)]),
+    ("rgl_server_step", [C.c_void_p, i32p]),
+    ("rgl_server_poll", [C.c_void_p, C.c_int64, i32p, P(C.c_double)]),
+    ("rgl_server_wait", [C.c_void_p, C.c_int64, P(C.c_double)]),
+    ("rgl_server_get_stats", [C.c_void_p, C.c_void_p]),
+    ("rgl_server_reset_stats", [C.c_void_p]),
 ]
 BVH2, BVH4_F32, BVH4_Q8 = 0, 1, 2
 VOID_FUNCS = {"rgl_get_last_error_string"}
@@ -363,6 +372,13 @@ def node_points_compact_by_field(node: Node, field: int = IS_HIT_I32) -> Node:
     return node
 
 
+def node_points_radar_bin(node: Node, range_step: float, range_max: float, azimuth_step: float, azimuth_min: float,
+                          azimuth_max: float, ray_solid_angle: float) -> Node:
+    check("rgl_node_points_radar_bin", node.ref, range_step, range_max, azimuth_step, azimuth_min, azimuth_max,
+          ray_solid_angle)
+    return node
+
+
 def node_points_from_array(node: Node, records: np.ndarray, fields) -> Node:
     f = np.ascontiguousarray(np.asarray(fields, dtype=np.int32))
     buf = np.ascontiguousarray(records)
@@ -419,6 +435,58 @@ def result(node: Node, field: int) -> np.ndarray:
     if n > 0:
         check("rgl_graph_get_result_data", node.h, int(field), _ptr(out))
     return out
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("batches", C.c_int64),
+                ("rays_completed", C.c_int64), ("evictions", C.c_int64), ("deferred", C.c_int64), ("failed", C.c_int64),
+                ("resident_bytes", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """rgl_server_t: continuous batching of sensor frames (see include/rgl/api/extensions/amd.h)."""
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, slots: int = 2, max_batch_graphs: int = 64, hbm_budget: int = 0, batching: bool = True):
+        self.h = C.c_void_p()
+        check("rgl_server_create", C.byref(self.h), slots, max_batch_graphs, C.c_int64(hbm_budget), bool(batching))
+
+    def submit(self, node: Node) -> int:
+        t = C.c_int64()
+        check("rgl_server_submit", self.h, node.h, C.byref(t))
+        return t.value
+
+    def step(self) -> int:
+        n = C.c_int32()
+        check("rgl_server_step", self.h, C.byref(n))
+        return n.value
+
+    def poll(self, ticket: int):
+        st, ms = C.c_int32(), C.c_double()
+        check("rgl_server_poll", self.h, C.c_int64(ticket), C.byref(st), C.byref(ms))
+        return st.value, ms.value
+
+    def wait(self, ticket: int) -> float:
+        ms = C.c_double()
+        check("rgl_server_wait", self.h, C.c_int64(ticket), C.byref(ms))
+        return ms.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check("rgl_server_get_stats", self.h, C.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        check("rgl_server_reset_stats", self.h)
+
+    def destroy(self) -> None:
+        if self.h:
+            check("rgl_server_destroy", self.h)
+            self.h = C.c_void_p()
 
 
 def set_bvh_format(fmt: int) -> None:
