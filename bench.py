@@ -258,6 +258,111 @@ def run_batch64(args):
     }
 
 
+RADAR_CELL = dict(range_step=0.2, range_max=200.0, azimuth_step=float(np.deg2rad(1.0)),
+                  azimuth_min=float(np.deg2rad(-60.0)), azimuth_max=float(np.deg2rad(60.0)))
+RADAR_SOLID_ANGLE = float(np.deg2rad(120.0 / 256) * np.deg2rad(20.0 / 63))  # azimuth step x elevation step
+
+
+def build_radar_graph(lidar: S.Lidar, scene):
+    """rays -> pose -> range -> raytrace(scene) -> radar range x azimuth binning (0.2 m x 1 deg cells, RCS)."""
+    n = rgl.Node
+    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
+    pose = rgl.node_rays_transform(n(), lidar.pose)
+    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
+    rt = rgl.node_raytrace(n(), scene)
+    out = rgl.node_points_radar_bin(n(), RADAR_CELL["range_step"], RADAR_CELL["range_max"], RADAR_CELL["azimuth_step"],
+                                    RADAR_CELL["azimuth_min"], RADAR_CELL["azimuth_max"], RADAR_SOLID_ANGLE)
+    rgl.chain(rays, pose, rng, rt, out)
+    return rays, rt, out
+
+
+def poisson_trace(rate_hz: float, duration_s: float, clients: int, seed: int):
+    rng = np.random.default_rng(seed)
+    n = int(rate_hz * duration_s * 1.5) + 16
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, n))
+    t = t[t < duration_s]
+    return t, rng.integers(0, clients, len(t))
+
+
+def serve_open_loop(entries, rays, times, who, batching: bool, slots: int, budget: int):
+    """Replays the arrival trace in real time through a server; returns its stats plus wall time and Mrays/s."""
+    server = rgl.Server(slots=slots, max_batch_graphs=len(entries), hbm_budget=budget, batching=batching)
+    k, n = 0, len(times)
+    t0 = time.perf_counter()
+    while k < n:
+        now = time.perf_counter() - t0
+        while k < n and times[k] <= now:
+            server.submit(entries[who[k]])
+            k += 1
+        server.step()
+    while server.stats()["completed"] < n:
+        server.step()
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    server.destroy()
+    st["wall_s"] = wall
+    st["mrays_per_s"] = st["rays_completed"] / wall / 1e6
+    return st
+
+
+def serve_closed_loop(entries, batching: bool, slots: int, budget: int, duration_s: float):
+    """Every client resubmits as soon as its previous frame is done: the server's saturated throughput."""
+    server = rgl.Server(slots=slots, max_batch_graphs=len(entries), hbm_budget=budget, batching=batching)
+    tickets = [server.submit(e) for e in entries]
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < duration_s:
+        server.step()
+        for i, t in enumerate(tickets):
+            if server.poll(t)[0] == rgl.Server.DONE:
+                tickets[i] = server.submit(entries[i])
+    while server.stats()["completed"] < server.stats()["submitted"]:
+        server.step()
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    server.destroy()
+    st["wall_s"] = wall
+    st["mrays_per_s"] = st["rays_completed"] / wall / 1e6
+    return st
+
+
+def run_serve(args, rate_hz: float = 2000.0, duration_s: float = 2.0, budget: int = 32 << 30):
+    specs, clients = S.config5_workload()
+    scenes = []
+    for spec in specs:
+        sc = rgl.scene_create()
+        S.load_into_rgl(spec, scene=sc)
+        scenes.append(sc)
+    graphs = []
+    for k, c in enumerate(clients):
+        if c.kind == "radar":
+            graphs.append(build_radar_graph(c.lidar, scenes[c.scene]))
+        else:
+            graphs.append(build_sensor_graph(c.lidar, scenes[c.scene], 0xC0F50000 + k))
+    entries = [g[0] for g in graphs]
+    rays = [len(c.lidar.rays) for c in clients]
+    for e in entries:  # builds every scene's structures and every graph's buffers
+        rgl.check("rgl_graph_run", e.h)
+    rgl.batch_kernel_timing(False)
+    times, who = poisson_trace(rate_hz, duration_s, len(clients), 0xC0F5)
+    keys = ("completed", "batches", "rays_completed", "evictions", "deferred", "resident_bytes")
+    out = {"arrival_rate_hz": rate_hz, "duration_s": duration_s, "requests": int(len(times)),
+           "clients": len(clients), "lidars": sum(c.kind == "lidar" for c in clients),
+           "radars": sum(c.kind == "radar" for c in clients), "scenes": len(specs),
+           "scene_triangles": [s.triangle_count for s in specs], "hbm_budget_bytes": budget,
+           "offered_mrays_per_s": round(sum(rays[w] for w in who) / duration_s / 1e6, 3)}
+    for name, batching, slots in (("batched", True, 3), ("plain", False, len(clients))):
+        st = serve_open_loop(entries, rays, times, who, batching, slots, budget)
+        sat = serve_closed_loop(entries, batching, slots, budget, 1.0)
+        out[name] = {"p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
+                     "max_ms": round(st["max_ms"], 4), "mrays_per_s": round(st["mrays_per_s"], 3),
+                     "saturated_mrays_per_s": round(sat["mrays_per_s"], 3),
+                     "saturated_p99_ms": round(sat["p99_ms"], 4), "slots": slots,
+                     **{k: int(st[k]) for k in keys}}
+    out["saturated_batched_vs_plain"] = round(out["batched"]["saturated_mrays_per_s"] /
+                                              out["plain"]["saturated_mrays_per_s"], 3)
+    return out
+
+
 def run_urban(args):
     spec, lidar = S.config2_scene(), S.config2_lidar()
     S.load_into_rgl(spec)
@@ -317,7 +422,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["batch64", "urban"], default="batch64")
+    ap.add_argument("--workload", choices=["batch64", "urban", "serve"], default="batch64")
+    ap.add_argument("--no-serve", action="store_true", help="batch64: skip the serving-loop measurement")
     ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",
                     help="node format: binary FP32, 4-wide FP32, or 4-wide 8-bit quantized")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for cpu_baseline (0 = skip)")
@@ -342,7 +448,14 @@ def main():
     rgl.check("rgl_configure_logging", 6, None, False)
     fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8}[args.bvh_format]
     rgl.set_bvh_format(fmt)
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args)))
+        rgl.cleanup()
+        return
     result = run_batch64(args) if args.workload == "batch64" else run_urban(args)
+    if args.workload == "batch64" and not args.no_serve:
+        rgl.cleanup()
+        result["serving"] = run_serve(args)
     result["config"]["bvh_format"] = {rgl.BVH2: "binary FP32 (64 B nodes)", rgl.BVH4_F32: "4-wide FP32 (128 B nodes)",
                                       rgl.BVH4_Q8: "4-wide 8-bit quantized (64 B nodes)"}[fmt]
     rgl.cleanup()

@@ -353,6 +353,7 @@ def radar_bin(xyz_sensor, distance, incident, is_hit, range_step, range_max, azi
     rc = int(np.ceil(range_max / range_step))
     ac = int(np.ceil((azimuth_max - azimuth_min) / azimuth_step))
     az = np.arctan2(p[:, 1], p[:, 0])
+    d = np.where(hit, d, -1.0)  # non-hits carry infinite distances; they are dropped below
     rpos = d / range_step
     apos = (az - azimuth_min) / azimuth_step
     rf, af = np.floor(rpos), np.floor(apos)

@@ -343,3 +343,49 @@ def config4_workload(seed: int = 0xC0F4, sensors: int = 64, scenes: int = 8):
         pose = f32(trs((px, py, 1.8), (0, 0, yaw)))
         out.append(BatchSensor(sc, Lidar(spinning_lidar(ch, az, -25.0, 15.0), pose, (0.0, 200.0), None, ch, az)))
     return specs, out
+
+
+def radar_rays(azimuths: int = 256, elevations: int = 64, az_half_deg: float = 60.0, el_half_deg: float = 10.0):
+    """Sensor-frame radar rays over [-az_half, +az_half] x [-el_half, +el_half] degrees (z up, azimuth from +X), with
+    azimuths offset by 0.37 of a step so that no beam lies exactly on a 1-degree cell edge."""
+    az = -az_half_deg + (np.arange(azimuths) + 0.37) * (2 * az_half_deg / azimuths)
+    el = np.linspace(-el_half_deg, el_half_deg, elevations)
+    A, E = np.meshgrid(np.deg2rad(az), np.deg2rad(el), indexing="xy")
+    A, E = A.ravel(), E.ravel()
+    th = np.pi / 2 - E
+    ca, sa, ct, st = np.cos(A), np.sin(A), np.cos(th), np.sin(th)
+    m = np.zeros((A.size, 3, 4))
+    m[:, 0, 0], m[:, 0, 1], m[:, 0, 2] = ca * ct, -sa, ca * st
+    m[:, 1, 0], m[:, 1, 1], m[:, 1, 2] = sa * ct, ca, sa * st
+    m[:, 2, 0], m[:, 2, 1], m[:, 2, 2] = -st, 0.0, ct
+    return f32(m)
+
+
+@dataclass
+class Client:
+    scene: int
+    kind: str      # "lidar" or "radar"
+    lidar: Lidar   # sensor rays and pose (for radars: 256 x 64 radar rays)
+
+
+def config5_workload(seed: int = 0xC0F5, clients: int = 32, radars: int = 8, scenes: int = 16):
+    """Serving loop: `clients` simulated clients (the last `radars` of them radars of 256 x 64 rays over +-60 deg
+    azimuth and +-10 deg elevation, range 200 m; the others lidars drawn as in config4_workload) over `scenes` urban
+    scenes of (stratified) log-uniform 50k-10M triangles. Returns (scene specs, clients)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(5e4), math.log(1e7)
+    targets = np.exp(lo + (np.arange(scenes) + rng.uniform(0.0, 1.0, scenes)) * (hi - lo) / scenes)
+    specs = [config2_scene(seed=seed + 1 + k, target_tris=int(t), instances=int(np.clip(t / 4000, 24, 500)))
+             for k, t in enumerate(targets)]
+    out = []
+    for c in range(clients):
+        sc = int(rng.integers(0, scenes))
+        px, py = rng.uniform(-60.0, 60.0, 2)
+        pose = f32(trs((px, py, 1.8), (0, 0, rng.uniform(0.0, 360.0))))
+        if c >= clients - radars:
+            out.append(Client(sc, "radar", Lidar(radar_rays(), pose, (0.0, 200.0), None, 64, 256)))
+        else:
+            ch = int(rng.integers(16, 129))
+            az = int(rng.integers(900, 4097))
+            out.append(Client(sc, "lidar", Lidar(spinning_lidar(ch, az, -25.0, 15.0), pose, (0.0, 200.0), None, ch, az)))
+    return specs, out

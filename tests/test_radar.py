@@ -136,7 +136,7 @@ def test_gpu_radar_bin_matches_oracle_on_c1(rgl_lib, oracle):
     assert hit.sum() > 5000
     p64 = np.vstack([pose.astype(np.float64), [0, 0, 0, 1]])
     inv = np.linalg.inv(p64)
-    xyz_s = xyz.astype(np.float64) @ inv[:3, :3].T + inv[:3, 3]
+    xyz_s = np.where(hit[:, None] == 1, xyz.astype(np.float64), 0.0) @ inv[:3, :3].T + inv[:3, 3]
     cells, counts, rcs_db, alt = oracle.radar_bin(xyz_s, d, inc, hit, RANGE_STEP, RANGE_MAX, AZ_STEP, AZ_MIN, AZ_MAX,
                                                   OMEGA)
     ac = AZ_CELLS
