@@ -363,6 +363,92 @@ def run_serve(args, rate_hz: float = 2000.0, duration_s: float = 2.0, budget: in
     return out
 
 
+MR_FIELDS = [rgl.XYZ_VEC3_F32, rgl.DISTANCE_F32, rgl.INTENSITY_F32, rgl.ENTITY_ID_I32, rgl.IS_HIT_I32,
+             rgl.RETURN_TYPE_U8]
+
+
+def build_multireturn_graph(lidar: S.Lidar, mode: int, divergence: float):
+    n = rgl.Node
+    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
+    pose = rgl.node_rays_transform(n(), lidar.pose)
+    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
+    rt = rgl.node_raytrace(n())
+    rgl.check("rgl_node_raytrace_configure_return_mode", rt.h, mode)
+    rgl.check("rgl_node_raytrace_configure_beam_divergence", rt.h, divergence, divergence)
+    rgl.check("rgl_node_raytrace_configure_default_intensity", rt.h, 30.0)
+    comp = rgl.node_points_compact_by_field(n(), rgl.IS_HIT_I32)
+    out = rgl.node_points_yield(n(), MR_FIELDS)
+    rgl.chain(rays, pose, rng, rt, comp, out)
+    return rays, rt, out
+
+
+def run_multireturn(args, moving_fraction: float = 0.2, divergence: float = 0.13e-3):
+    """C3: ~10M triangles in 2,001 instances with a 1024 x 1024 intensity texture; a 64 x 4096 lidar with 0.13 mrad
+    divergence (58 samples per beam, the reference's beam model) traced twice per frame: FIRST_LAST and STRONGEST
+    (the reference allows at most two returns per node), both graphs in one batch. Before every frame 20% of the
+    instances move by U[0, 0.5] m; the TLAS is refitted or rebuilt."""
+    spec, movable = S.config3_scene()
+    lidar = S.config3_lidar()
+    _, ents, _ = S.load_into_rgl(spec)
+    graphs = [build_multireturn_graph(lidar, rgl.RGL_RETURN_FIRST_LAST, divergence),
+              build_multireturn_graph(lidar, rgl.RGL_RETURN_STRONGEST, divergence)]
+    entries = [g[0] for g in graphs]
+    beams = 2 * len(lidar.rays)
+    base = [spec.instances[i].transform.astype(np.float64) for i in movable]
+    rng = np.random.default_rng(0xC0F3)
+    pos = None
+
+    def move():
+        nonlocal pos
+        pick = rng.choice(len(movable), int(moving_fraction * len(movable)), replace=False)
+        step = rng.uniform(0.0, 0.5, len(pick))
+        ang = rng.uniform(0.0, 2 * np.pi, len(pick))
+        pos[pick, 0] += step * np.cos(ang)
+        pos[pick, 1] += step * np.sin(ang)
+        for i in pick:
+            m = base[i].copy()
+            m[:2, 3] = pos[i]
+            rgl.entity_set_transform(ents[movable[i]], S.f32(m))
+
+    def frame():
+        rgl.graph_run_batch(entries)
+        return sum(kept_count(g[2]) for g in graphs)
+
+    out = {"triangles": spec.triangle_count, "instances": len(spec.instances), "beams_per_step": beams,
+           "samples_per_beam": 58, "divergence_rad": divergence, "moving_fraction": moving_fraction,
+           "returns": "FIRST_LAST + STRONGEST (two graphs, one batch)", "texture": "1024 x 1024 uint8"}
+    for mode in ("rebuild", "refit"):
+        rgl.check("rgl_scene_set_tlas_refit", None, mode == "refit")
+        pos = np.array([b[:2, 3] for b in base])
+        for i, b in enumerate(base):
+            rgl.entity_set_transform(ents[movable[i]], S.f32(b))
+        for _ in range(args.warmup):
+            move()
+            frame()
+        lat, upd = [], []
+        kept = 0
+        rgl.batch_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            u0 = time.perf_counter()
+            move()
+            f0 = time.perf_counter()
+            kept = frame()
+            lat.append(time.perf_counter() - f0)
+            upd.append(f0 - u0)
+        total = time.perf_counter() - t0
+        kms, launches = rgl.batch_kernel_timing()
+        rgl.batch_kernel_timing(False)
+        p50, p99 = latency_stats(lat)
+        frame_s = sum(lat) / len(lat)
+        out[mode] = {"mbeams_per_s": round(beams / frame_s / 1e6, 3),
+                     "traced_mrays_per_s": round(58 * beams / frame_s / 1e6, 3),
+                     "p50_ms": p50, "p99_ms": p99, "raytrace_kernel_ms": round(kms / args.steps, 4),
+                     "host_pose_update_ms": round(1e3 * sum(upd) / len(upd), 4),
+                     "wall_ms_per_step": round(total / args.steps * 1e3, 4), "kept_points_per_step": kept}
+    return out
+
+
 def run_urban(args):
     spec, lidar = S.config2_scene(), S.config2_lidar()
     S.load_into_rgl(spec)
@@ -422,7 +508,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["batch64", "urban", "serve"], default="batch64")
+    ap.add_argument("--workload", choices=["batch64", "urban", "serve", "multireturn"], default="batch64")
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the serving-loop measurement")
     ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",
                     help="node format: binary FP32, 4-wide FP32, or 4-wide 8-bit quantized")
@@ -448,8 +534,8 @@ def main():
     rgl.check("rgl_configure_logging", 6, None, False)
     fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8}[args.bvh_format]
     rgl.set_bvh_format(fmt)
-    if args.workload == "serve":
-        print(json.dumps(run_serve(args)))
+    if args.workload in ("serve", "multireturn"):
+        print(json.dumps(run_serve(args) if args.workload == "serve" else run_multireturn(args)))
         rgl.cleanup()
         return
     result = run_batch64(args) if args.workload == "batch64" else run_urban(args)

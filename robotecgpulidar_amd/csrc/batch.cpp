@@ -29,27 +29,29 @@ struct GraphBatch::Plan
 {
 	struct Level
 	{
-		std::vector<RaytraceBatchItem> rt;
-		std::vector<RaytraceNode*> rtNodes;
+		std::vector<RaytraceBatchItem> rt, rtm; // single-sample beams; divergent beams (one wave per beam)
+		std::vector<RaytraceNode*> rtNodes, rtmNodes;
 		std::vector<PointOpItem> op;
 		std::vector<int> opLayout; // index into lists, -1 = none
 		std::vector<CompactItem> cp;
 		std::vector<int> cpList;
 		std::vector<CompactByFieldPointsNode*> cpNodes;
 		std::vector<Node*> other;
-		int rtBlocks = 0, opBlocks = 0, cpTiles = 0;
-		size_t rtOff = 0, opOff = 0, cpOff = 0;
+		int rtBlocks = 0, rtmBlocks = 0, opBlocks = 0, cpTiles = 0;
+		size_t rtOff = 0, rtmOff = 0, opOff = 0, cpOff = 0;
 		void clear()
 		{
 			rt.clear();
 			rtNodes.clear();
+			rtm.clear();
+			rtmNodes.clear();
 			op.clear();
 			opLayout.clear();
 			cp.clear();
 			cpList.clear();
 			cpNodes.clear();
 			other.clear();
-			rtBlocks = opBlocks = cpTiles = 0;
+			rtBlocks = rtmBlocks = opBlocks = cpTiles = 0;
 		}
 	};
 	std::vector<Level> levels; // steps, in launch order
@@ -247,10 +249,17 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 					case BatchStage::Raytrace: {
 						auto* rt = dynamic_cast<RaytraceNode*>(n);
 						RaytraceBatchItem it = rt->batchItem(s);
-						it.firstBlock = L.rtBlocks;
-						L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount);
-						L.rt.push_back(it);
-						L.rtNodes.push_back(rt);
+						if (gpu::beamIsMulti(it.p)) {
+							it.firstBlock = L.rtmBlocks;
+							L.rtmBlocks += gpu::raytraceBlocks(it.p.rayCount, true);
+							L.rtm.push_back(it);
+							L.rtmNodes.push_back(rt);
+						} else {
+							it.firstBlock = L.rtBlocks;
+							L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount, false);
+							L.rt.push_back(it);
+							L.rtNodes.push_back(rt);
+						}
 						Scene* sc = &rt->getScene();
 						if (std::find(pl.scenes.begin(), pl.scenes.end(), sc) == pl.scenes.end()) {
 							pl.scenes.push_back(sc);
@@ -308,6 +317,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			for (auto& it : pl.levels[l].rt) {
 				pl.gather.push_back(it.sc.overflow);
 			}
+			for (auto& it : pl.levels[l].rtm) {
+				pl.gather.push_back(it.sc.overflow);
+			}
 		}
 
 		// One staging image: field lists, gather sources, then per-level item arrays.
@@ -322,6 +334,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		for (int l = 0; l <= maxLevel; ++l) {
 			Plan::Level& L = pl.levels[l];
 			L.rtOff = reserve(sizeof(RaytraceBatchItem) * L.rt.size());
+			L.rtmOff = reserve(sizeof(RaytraceBatchItem) * L.rtm.size());
 			L.opOff = reserve(sizeof(PointOpItem) * L.op.size());
 			L.cpOff = reserve(sizeof(CompactItem) * L.cp.size());
 		}
@@ -351,6 +364,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			if (!L.rt.empty()) {
 				std::memcpy(host + L.rtOff, L.rt.data(), sizeof(RaytraceBatchItem) * L.rt.size());
 			}
+			if (!L.rtm.empty()) {
+				std::memcpy(host + L.rtmOff, L.rtm.data(), sizeof(RaytraceBatchItem) * L.rtm.size());
+			}
 			if (!L.op.empty()) {
 				std::memcpy(host + L.opOff, L.op.data(), sizeof(PointOpItem) * L.op.size());
 			}
@@ -364,8 +380,8 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		// Launches, level by level; nodes of one level never depend on each other.
 		for (int l = 0; l <= maxLevel; ++l) {
 			Plan::Level& L = pl.levels[l];
-			if (!L.rt.empty()) {
-				const bool timed = timing && L.rtBlocks > 0;
+			if (!L.rt.empty() || !L.rtm.empty()) {
+				const bool timed = timing && (L.rtBlocks > 0 || L.rtmBlocks > 0);
 				if (timed) {
 					if (timingUsed == timingEvents.size()) {
 						hipEvent_t a, b;
@@ -375,11 +391,16 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 					}
 					RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
 				}
-				gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
-				                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(),
-				                   std::any_of(L.rt.begin(), L.rt.end(),
-				                               [](const RaytraceBatchItem& it) { return gpu::beamIsMulti(it.p); }));
-				RGL_HIP(hipGetLastError());
+				if (!L.rt.empty()) {
+					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
+					                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false);
+					RGL_HIP(hipGetLastError());
+				}
+				if (!L.rtm.empty()) {
+					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtmOff),
+					                   static_cast<int>(L.rtm.size()), L.rtmBlocks, Scene::bvhFormat(), true);
+					RGL_HIP(hipGetLastError());
+				}
 				if (timed) {
 					RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, s));
 					++timingUsed;
@@ -423,6 +444,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			}
 			for (int l = 0; l <= maxLevel; ++l) {
 				for (auto* rt : pl.levels[l].rtNodes) {
+					rt->setOverflowHost(hostBuf, k++);
+				}
+				for (auto* rt : pl.levels[l].rtmNodes) {
 					rt->setOverflowHost(hostBuf, k++);
 				}
 			}

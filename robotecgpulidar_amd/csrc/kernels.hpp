@@ -49,13 +49,14 @@ void traversalStats(hipStream_t stream, const SceneView& scene, const RaytracePa
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
                        float* outT, int32_t* outInstance, int32_t* outPrim);
 void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
-int raytraceBlocks(int rayCount);
+// Blocks of a raytrace launch over rayCount beams: one beam per thread, or one wave per divergent beam.
+int raytraceBlocks(int rayCount, bool multi);
 // True if the beams of `p` are traced with the 58-sample divergence model.
 bool beamIsMulti(const RaytraceParams& p);
-// Batched launches: `items` live in device memory; every item's scene uses node format `format`; anyMulti must be
-// true if any item has divergent beams.
+// Batched launches: `items` live in device memory; every item's scene uses node format `format`; with `multi` every
+// item has divergent beams (blocks from raytraceBlocks(n, true)), otherwise none has.
 void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
-                   bool anyMulti);
+                   bool multi);
 int pointOpBlocks(int capacity);
 void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks);
 void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles);

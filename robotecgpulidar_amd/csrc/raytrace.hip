@@ -524,186 +524,205 @@ __device__ void writeReturnNonHit(const RaytraceOutputs& out, const RaytracePara
 
 __device__ __forceinline__ uint8_t returnTypeOf(int mode, int r) { return static_cast<uint8_t>((mode >> (8 * r)) & 0xff); }
 
-struct Tracker
+// Beam-shared fields (src/gpu/optixPrograms.cu:295-326) of every return point of the beam.
+__device__ void writeBeamShared(const RaytraceOutputs& out, const RaytraceParams& p, int beam, const Mat3x4f& rayWorld)
 {
-	int sample;
-	float distance;
-	float intensity;
-	HitRec hit;
-};
+	if (!(out.rayIdx || out.azimuth || out.elevation || out.timeStampF64 || out.timeStampU32 || out.ringId)) {
+		return;
+	}
+	const int R = p.returnCount;
+	const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
+	for (int r = 0; r < R; ++r) {
+		const int rp = beam * R + r;
+		if (out.rayIdx) out.rayIdx[rp] = static_cast<uint32_t>(beam);
+		if (out.azimuth) out.azimuth[rp] = rayLocal.toRotationYOrderZXYLeftHandRad();
+		if (out.elevation) out.elevation[rp] = rayLocal.toRotationXOrderZXYLeftHandRad();
+		if (out.timeStampF64) {
+			out.timeStampF64[rp] = p.doDistortion ? static_cast<double>(p.timeOffsetsMs[beam] * 1e-3f) : 0.0;
+		}
+		if (out.timeStampU32) {
+			out.timeStampU32[rp] = p.doDistortion ? static_cast<uint32_t>(p.timeOffsetsMs[beam] * 1e6f) : 0u;
+		}
+		if (out.ringId && p.ringIds) out.ringId[rp] = static_cast<uint16_t>(p.ringIds[beam % p.ringIdsCount]);
+	}
+}
 
-// One beam of one sensor: raygen, traversal, closest hit, beam reduction and output writes.
-// kMulti = false compiles the single-sample path only (beams without divergence); kMulti = true handles both.
-template<int kFmt, bool kMulti>
+// The beam's ray after velocity distortion (src/gpu/optixPrograms.cu:61-70).
+__device__ __forceinline__ Mat3x4f distortedRay(const RaytraceParams& p, int beam, const Mat3x4f& rayWorld)
+{
+	if (!p.doDistortion) {
+		return rayWorld;
+	}
+	const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
+	const float toDeg = static_cast<float>(180.0 / M_PI);
+	const float off = p.timeOffsetsMs[beam];
+	const Vec3f t = (off * p.sensorLinearVelocity) * 0.001f;
+	const Vec3f rdeg = (off * (p.sensorAngularVelocity * toDeg)) * 0.001f;
+	return p.rayOriginToWorld * (TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f}) * rayLocal);
+}
+
+// One beam traced with a single sample (no divergence): the beam reduction degenerates to "every return is the
+// primary sample" (src/gpu/nodeKernels.cu:345-354,399-421 with one sample), so no trackers are kept.
+template<int kFmt>
 __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                              const int beam)
 {
 	const Mat3x4f rayWorld = p.rays[beam];
 	const int R = p.returnCount;
-
-	// Beam-shared fields (src/gpu/optixPrograms.cu:295-326).
-	if (out.rayIdx || out.azimuth || out.elevation || out.timeStampF64 || out.timeStampU32 || out.ringId) {
-		const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
-		for (int r = 0; r < R; ++r) {
-			const int rp = beam * R + r;
-			if (out.rayIdx) out.rayIdx[rp] = static_cast<uint32_t>(beam);
-			if (out.azimuth) out.azimuth[rp] = rayLocal.toRotationYOrderZXYLeftHandRad();
-			if (out.elevation) out.elevation[rp] = rayLocal.toRotationXOrderZXYLeftHandRad();
-			if (out.timeStampF64) {
-				out.timeStampF64[rp] = p.doDistortion ? static_cast<double>(p.timeOffsetsMs[beam] * 1e-3f) : 0.0;
-			}
-			if (out.timeStampU32) {
-				out.timeStampU32[rp] = p.doDistortion ? static_cast<uint32_t>(p.timeOffsetsMs[beam] * 1e6f) : 0u;
-			}
-			if (out.ringId && p.ringIds) out.ringId[rp] = static_cast<uint16_t>(p.ringIds[beam % p.ringIdsCount]);
-		}
-	}
-
+	writeBeamShared(out, p, beam, rayWorld);
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
-	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
-	if (masked) {
+	if (p.mask != nullptr && p.mask[beam] == 0) {
 		for (int r = 0; r < R; ++r) {
 			writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
 		}
 		return;
 	}
-
-	Mat3x4f ray = rayWorld;
-	if (p.doDistortion) {
-		const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
-		const float toDeg = static_cast<float>(180.0 / M_PI);
-		const float off = p.timeOffsetsMs[beam];
-		const Vec3f t = (off * p.sensorLinearVelocity) * 0.001f;
-		const Vec3f rdeg = (off * (p.sensorAngularVelocity * toDeg)) * 0.001f;
-		ray = p.rayOriginToWorld * (TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f}) * rayLocal);
-	}
-
-	const bool needVel = p.velocityRequested != 0;
-	if (!kMulti || !(p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f)) {
-		// One sample per beam: the beam reduction degenerates to "every return is the primary sample"
-		// (src/gpu/nodeKernels.cu:345-354,399-421 with a single sample), so no trackers are kept.
-		Vec3f so, sd;
-		rayOriginDir(ray, so, sd);
-		HitRec h;
-		float d0 = p.farNonHitDistance;
-		if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
-			if (sh.isHit) {
-				for (int r = 0; r < R; ++r) {
-					const uint8_t type = returnTypeOf(p.returnMode, r);
-					if (type >= 1 && type <= 5) {
-						writeReturnHit(out, p, beam, beam * R + r, type, sh);
-					}
+	const Mat3x4f ray = distortedRay(p, beam, rayWorld);
+	Vec3f so, sd;
+	rayOriginDir(ray, so, sd);
+	HitRec h;
+	float d0 = p.farNonHitDistance;
+	if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
+		const Shade sh = shadeSample(sc, p, range.x, so, sd, h, p.velocityRequested != 0);
+		if (sh.isHit) {
+			for (int r = 0; r < R; ++r) {
+				const uint8_t type = returnTypeOf(p.returnMode, r);
+				if (type >= 1 && type <= 5) {
+					writeReturnHit(out, p, beam, beam * R + r, type, sh);
 				}
-				return;
 			}
-			d0 = sh.distance;
+			return;
 		}
-		for (int r = 0; r < R; ++r) {
-			writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), d0);
+		d0 = sh.distance;
+	}
+	for (int r = 0; r < R; ++r) {
+		writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), d0);
+	}
+}
+
+// One divergent beam per wave: lane s < 58 traces sample s (the reference's 58-sample beam,
+// src/gpu/optixPrograms.cu:75-86,230-248), so the samples of a beam, which are nearly parallel, traverse together.
+// The beam reduction of src/gpu/nodeKernels.cu:334-422 (first hit seeds every tracker; then strict comparisons in
+// sample order) runs on every lane over the samples' (hit, distance, intensity) read across lanes, which keeps it
+// wave-uniform and exact; the lane holding each selected sample shades it in full and writes the return.
+template<int kFmt>
+__device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
+                                                 const int beam)
+{
+	const int lane = threadIdx.x & 63;
+	const Mat3x4f rayWorld = p.rays[beam];
+	const int R = p.returnCount;
+	if (lane == 0) {
+		writeBeamShared(out, p, beam, rayWorld);
+	}
+	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
+	if (p.mask != nullptr && p.mask[beam] == 0) {
+		if (lane == 0) {
+			for (int r = 0; r < R; ++r) {
+				writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
+			}
 		}
 		return;
 	}
-
-	if constexpr (kMulti) {
-		const int samples = kBeamSamples;
-		Tracker first, second, last, strongest, secondStrongest;
-		first.sample = -1;
-		float sample0Distance = p.farNonHitDistance;
-		for (int s = 0; s < samples; ++s) {
-			Mat3x4f sr = ray;
-			if (s > 0) {
-				const int layer = (s - 1) / kBeamVertices;
-				const int vertex = (s - 1) % kBeamVertices;
-				sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
-			}
-			Vec3f so, sd;
-			rayOriginDir(sr, so, sd);
-			HitRec h;
-			const bool any = traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr);
-			Tracker cand;
-			cand.sample = s;
-			cand.hit = h;
-			int isHit = 0;
-			if (any) {
-				const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
-				isHit = sh.isHit;
-				cand.distance = sh.distance;
-				cand.intensity = sh.intensity;
-			} else {
-				cand.distance = p.farNonHitDistance;
-				cand.intensity = 0.0f;
-			}
-			if (s == 0) {
-				sample0Distance = cand.distance;
-			}
-			if (!isHit) {
-				continue;
-			}
-			if (first.sample < 0) {
-				first = second = last = strongest = secondStrongest = cand;
-				continue;
-			}
-			const float d = cand.distance;
-			if (d < first.distance) {
-				second = first;
-				first = cand;
-			} else if (first.distance < d && d < second.distance) {
-				second = cand;
-			}
-			if (d > last.distance) {
-				last = cand;
-			}
-			const float I = cand.intensity;
-			if (I > strongest.intensity) {
-				secondStrongest = strongest;
-				strongest = cand;
-			} else if (strongest.intensity > I && I > secondStrongest.intensity) {
-				secondStrongest = cand;
-			}
+	const Mat3x4f ray = distortedRay(p, beam, rayWorld);
+	Vec3f so{0.0f, 0.0f, 0.0f}, sd{0.0f, 0.0f, 1.0f};
+	HitRec h;
+	h.t = 0.0f;
+	h.inst = h.prim = h.leaf = -1;
+	h.u = h.v = 0.0f;
+	int isHit = 0;
+	float dist = p.farNonHitDistance, inten = 0.0f;
+	if (lane < kBeamSamples) {
+		Mat3x4f sr = ray;
+		if (lane > 0) {
+			const int layer = (lane - 1) / kBeamVertices;
+			const int vertex = (lane - 1) % kBeamVertices;
+			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
 		}
-
-		for (int r = 0; r < R; ++r) {
-			const uint8_t type = returnTypeOf(p.returnMode, r);
-			const int rp = beam * R + r;
-			if (first.sample < 0) {
-				writeReturnNonHit(out, p, beam, rp, type, sample0Distance);
-				continue;
+		rayOriginDir(sr, so, sd);
+		if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
+			isHit = sh.isHit;
+			dist = sh.distance;
+			inten = sh.intensity;
+		}
+	}
+	// Trackers hold (sample, key): distance for FIRST / SECOND / LAST, intensity for STRONGEST / SECOND_STRONGEST.
+	int fS = -1, sS = -1, lS = -1, stS = -1, ssS = -1;
+	float fD = 0.0f, sD = 0.0f, lD = 0.0f, stI = 0.0f, ssI = 0.0f;
+	const float d0 = __shfl(dist, 0, 64);
+	for (int k = 0; k < kBeamSamples; ++k) {
+		if (__shfl(isHit, k, 64) == 0) {
+			continue;
+		}
+		const float d = __shfl(dist, k, 64);
+		const float I = __shfl(inten, k, 64);
+		if (fS < 0) {
+			fS = sS = lS = stS = ssS = k;
+			fD = sD = lD = d;
+			stI = ssI = I;
+			continue;
+		}
+		if (d < fD) {
+			sS = fS;
+			sD = fD;
+			fS = k;
+			fD = d;
+		} else if (fD < d && d < sD) {
+			sS = k;
+			sD = d;
+		}
+		if (d > lD) {
+			lS = k;
+			lD = d;
+		}
+		if (I > stI) {
+			ssS = stS;
+			ssI = stI;
+			stS = k;
+			stI = I;
+		} else if (stI > I && I > ssI) {
+			ssS = k;
+			ssI = I;
+		}
+	}
+	const bool needVel = p.velocityRequested != 0;
+	for (int r = 0; r < R; ++r) {
+		const uint8_t type = returnTypeOf(p.returnMode, r);
+		const int rp = beam * R + r;
+		if (fS < 0) {
+			if (lane == 0) {
+				writeReturnNonHit(out, p, beam, rp, type, d0);
 			}
-			if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
-				continue;
-			}
-			// Branch-wise copies (not an aggregate select of addresses) let the trackers stay in registers.
-			Tracker t = first; // RGL_RETURN_TYPE_FIRST
-			if (type == 3) {
-				t = second;
-			} else if (type == 2) {
-				t = last;
-			} else if (type == 1) {
-				t = strongest;
-			} else if (type == 5) {
-				t = secondStrongest;
-			}
-			Mat3x4f sr = ray;
-			if (t.sample > 0) {
-				const int layer = (t.sample - 1) / kBeamVertices;
-				const int vertex = (t.sample - 1) % kBeamVertices;
-				sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, layer, vertex);
-			}
-			Vec3f so, sd;
-			rayOriginDir(sr, so, sd);
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, t.hit, needVel);
+			continue;
+		}
+		if (type < 1 || type > 5) { // RGL_RETURN_TYPE_UNKNOWN or invalid: nothing is written, as in the reference
+			continue;
+		}
+		const int sel = type == 3 ? sS : (type == 2 ? lS : (type == 1 ? stS : (type == 5 ? ssS : fS)));
+		if (lane == sel) {
+			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
 			writeReturnHit(out, p, beam, rp, type, sh);
 		}
 	}
 }
 
+constexpr int kBeamsPerMultiBlock = kRtBlock / 64; // divergent beams: one wave each
+
 template<int kFmt, bool kMulti>
-__device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out)
+__device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
+                                            int localBlock)
 {
-	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
-	if (beam < p.rayCount) {
-		raytraceBeam<kFmt, kMulti>(sc, p, out, beam);
+	if constexpr (kMulti) {
+		const int beam = localBlock * kBeamsPerMultiBlock + static_cast<int>(threadIdx.x >> 6);
+		if (beam < p.rayCount) {
+			raytraceBeamWave<kFmt>(sc, p, out, beam);
+		}
+	} else {
+		const int beam = localBlock * kRtBlock + threadIdx.x;
+		if (beam < p.rayCount) {
+			raytraceBeam<kFmt>(sc, p, out, beam);
+		}
 	}
 }
 
@@ -723,24 +742,21 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 	}
 	const int item = __builtin_amdgcn_readfirstlane(lo);
 	const RaytraceBatchItem& it = items[item];
-	const int beam = (static_cast<int>(blockIdx.x) - it.firstBlock) * kRtBlock + threadIdx.x;
-	if (beam < it.p.rayCount) {
-		raytraceBeam<kFmt, kMulti>(it.sc, it.p, it.out, beam);
-	}
+	raytraceOne<kFmt, kMulti>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
 }
 
-// Single-sample beams (no divergence): occupancy-tuned. Divergent beams keep the compiler's register allocation,
-// since their five return trackers need more registers.
+// Single-sample beams (no divergence), one per thread: occupancy-tuned. Divergent beams, one per wave, keep the
+// compiler's register allocation.
 template<int kFmt>
 __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
-	raytraceOne<kFmt, false>(sc, p, out);
+	raytraceOne<kFmt, false>(sc, p, out, blockIdx.x);
 }
 
 template<int kFmt>
 __global__ void __launch_bounds__(kRtBlock) kRaytraceMulti(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
-	raytraceOne<kFmt, true>(sc, p, out);
+	raytraceOne<kFmt, true>(sc, p, out, blockIdx.x);
 }
 
 template<int kFmt>
@@ -850,8 +866,8 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 	if (params.rayCount <= 0) {
 		return;
 	}
-	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
 	const bool multi = beamIsMulti(params);
+	const int blocks = raytraceBlocks(params.rayCount, multi);
 	switch (scene.format) {
 		case kBvh4F: launchRaytrace<kBvh4F>(stream, blocks, multi, scene, params, out); break;
 		case kBvh4Q: launchRaytrace<kBvh4Q>(stream, blocks, multi, scene, params, out); break;
@@ -859,7 +875,11 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 	}
 }
 
-int raytraceBlocks(int rayCount) { return (rayCount + kRtBlock - 1) / kRtBlock; }
+int raytraceBlocks(int rayCount, bool multi)
+{
+	const int per = multi ? kBeamsPerMultiBlock : kRtBlock;
+	return (rayCount + per - 1) / per;
+}
 
 void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
                    bool anyMulti)

@@ -389,3 +389,50 @@ def config5_workload(seed: int = 0xC0F5, clients: int = 32, radars: int = 8, sce
             az = int(rng.integers(900, 4097))
             out.append(Client(sc, "lidar", Lidar(spinning_lidar(ch, az, -25.0, 15.0), pose, (0.0, 200.0), None, ch, az)))
     return specs, out
+
+
+def config3_scene(seed: int = 0xC0F3, target_tris: int = 10_000_000, instances: int = 2000):
+    """Multi-return dynamic scene: `instances` instanced icospheres, textured boxes and building prisms (as in
+    config2_scene) on a textured ground grid sized so the total is about `target_tris` triangles. The ground and
+    the boxes sample a seeded 1024 x 1024 uint8 intensity texture through their UVs (the ground tiles it 16 x 16).
+    Returns (spec, indices of the movable instances = every instance but the ground)."""
+    rng = np.random.default_rng(seed)
+    spec = SceneSpec()
+    spec.textures.append(rng.integers(0, 256, (1024, 1024)).astype(np.uint8))
+    sv, si = icosphere(3)
+    sphere = spec.add_mesh(sv, si)
+    cube = spec.add_mesh(CUBE_VERTICES, CUBE_INDICES, CUBE_UVS)
+    prisms = {n: spec.add_mesh(*prism(n)) for n in range(4, 13)}
+    kinds = rng.integers(0, 3, instances)
+    placed, inst_tris = [], 0
+    for k in range(instances):
+        yaw = rng.uniform(0.0, 360.0)
+        px, py = rng.uniform(-190.0, 190.0, 2)
+        if kinds[k] == 0:
+            r = rng.uniform(0.5, 3.0)
+            placed.append((sphere, trs((px, py, r), (0, 0, yaw), (r, r, r)), -1))
+            inst_tris += len(si)
+        elif kinds[k] == 1:
+            sx, sy, sz = rng.uniform(1.0, 5.0, 3)
+            placed.append((cube, trs((px, py, sz / 2), (0, 0, yaw), (sx / 2, sy / 2, sz / 2)), 0))
+            inst_tris += 12
+        else:
+            n = int(rng.integers(4, 13))
+            rad = rng.uniform(3.0, 15.0)
+            h = rng.uniform(3.0, 60.0)
+            placed.append((prisms[n], trs((px, py, 0.0), (0, 0, yaw), (rad, rad, h)), -1))
+            inst_tris += 4 * n - 4
+    cells = max(1, int(round(math.sqrt(max(target_tris - inst_tris, 2) / 2))))
+    gv, gi = grid_mesh(400.0, 400.0, cells, cells)
+    uv = (gv[:, :2] / 400.0 + 0.5) * 16.0
+    ground = spec.add_mesh(gv, gi, uv)
+    spec.add_instance(ground, identity(), entity_id=1, texture=0)
+    movable = []
+    for k, (mesh, tf, tex) in enumerate(placed):
+        movable.append(spec.add_instance(mesh, tf, entity_id=2 + k, texture=tex, laser_retro=float(k % 7)))
+    return spec, movable
+
+
+def config3_lidar() -> Lidar:
+    """64 channels over [-25, +15] deg x 4096 azimuths = 262,144 beams at (0, 0, 1.8), range 200 m."""
+    return Lidar(spinning_lidar(64, 4096, -25.0, 15.0), f32(translation(0, 0, 1.8)), (0.0, 200.0), None, 64, 4096)

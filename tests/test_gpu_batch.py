@@ -18,14 +18,15 @@ FORMAT = 13842
 FIELDS = [XYZ, DIST, ENT, IS_HIT, NORMAL]
 FORMAT_FIELDS = [XYZ, PAD32, DIST, ENT]
 
-# (channels, azimuth steps, scene index, noise, compact)
+# (channels, azimuth steps, scene index, noise, compact, divergent dual-return beams)
 SENSORS = [
-    (16, 360, 0, True, True),
-    (32, 512, 1, True, True),
-    (8, 100, 0, False, False),
-    (64, 900, 1, True, False),
-    (1, 1, 0, True, True),
-    (128, 64, 1, False, True),
+    (16, 360, 0, True, True, False),
+    (32, 512, 1, True, True, False),
+    (8, 100, 0, False, False, False),
+    (64, 900, 1, True, False, False),
+    (1, 1, 0, True, True, False),
+    (128, 64, 1, False, True, False),
+    (8, 200, 0, True, True, True),
 ]
 
 
@@ -47,7 +48,7 @@ class SensorGraph:
     """rays -> pose -> range -> [angular ray noise] -> raytrace(scene) -> [distance noise] -> [compact]
     -> points transform -> yield + format."""
 
-    def __init__(self, rgl, ch, az, scene, noise, compact, seed):
+    def __init__(self, rgl, ch, az, scene, noise, compact, seed, divergent=False):
         self.rgl = rgl
         lidar = S.spinning_lidar(ch, az, -20.0, 10.0) if ch > 1 else S.f32(S.identity()[None])
         self.rays_np = lidar
@@ -63,6 +64,9 @@ class SensorGraph:
             chain.append(n)
         self.raytrace = rgl.node_raytrace(rgl.Node(), scene)
         rgl.check("rgl_node_raytrace_configure_non_hits", self.raytrace.h, 0.0, 150.0)
+        if divergent:  # 58-sample beams, FIRST and LAST returns
+            rgl.check("rgl_node_raytrace_configure_beam_divergence", self.raytrace.h, 3e-3, 3e-3)
+            rgl.check("rgl_node_raytrace_configure_return_mode", self.raytrace.h, rgl.RGL_RETURN_FIRST_LAST)
         chain.append(self.raytrace)
         if noise:
             n = rgl.Node()
@@ -103,8 +107,8 @@ def build_world(rgl):
 
 
 def make_set(rgl, scenes):
-    return [SensorGraph(rgl, ch, az, scenes[si], noise, compact, seed=k)
-            for k, (ch, az, si, noise, compact) in enumerate(SENSORS)]
+    return [SensorGraph(rgl, ch, az, scenes[si], noise, compact, seed=k, divergent=div)
+            for k, (ch, az, si, noise, compact, div) in enumerate(SENSORS)]
 
 
 def assert_same(a, b, label):
@@ -153,7 +157,7 @@ def test_batch_sensor_matches_oracle(rgl_lib, oracle):
     rgl.graph_run_batch([g.rays for g in graphs])
     for k in (2,):  # noise-free, uncompacted sensor on the default scene
         g = graphs[k]
-        ch, az, si, noise, compact = SENSORS[k]
+        ch, az, si, noise, compact, _ = SENSORS[k]
         assert not noise and not compact
         case = Case(specs[si], g.rays_np, pose=g.pose, ranges=np.array([[0.2, 150.0]], np.float32), near=0.0, far=150.0)
         orc = run_oracle(oracle, case, fields=FIELDS)
