@@ -205,6 +205,43 @@ def run_batch64(args):
     kernel_ms = kms / kframes  # batched raytrace time per frame
     launches_per_frame = launches / kframes
 
+    # End to end with the points on the host: every frame also reads every yielded field of every sensor into host
+    # arrays, either by copying from device memory at query time or from pinned memory the run's own kernel wrote.
+    readback = {}
+    bufs = [{f: np.empty((len(x.lidar.rays),) + ((3,) if f in (rgl.XYZ_VEC3_F32, rgl.NORMAL_VEC3_F32) else ()),
+                         dtype=rgl.FIELD_DTYPE[f][0]) for f in FIELDS} for x in sensors]
+
+    def read_all():
+        total = 0
+        for o, b in zip(outs, bufs):
+            n = kept_count(o)
+            for f in FIELDS:
+                if n:
+                    rgl.check("rgl_graph_get_result_data", o.h, int(f), b[f].ctypes.data)
+                total += n * rgl.field_size(f)
+        return total
+
+    for name, prefetch in (("device_copy", False), ("prefetch", True)):
+        for o in outs:
+            rgl.check("rgl_node_points_yield_configure_prefetch", o.h, prefetch)
+        for _ in range(2):
+            rgl.graph_run_batch(entries)
+            read_all()
+        lat = []
+        t0 = time.perf_counter()
+        moved = 0
+        for _ in range(args.steps):
+            f0 = time.perf_counter()
+            rgl.graph_run_batch(entries)
+            moved = read_all()
+            lat.append(time.perf_counter() - f0)
+        tot = time.perf_counter() - t0
+        r50, r99 = latency_stats(lat)
+        readback[name] = {"mrays_per_s": round(args.steps * rays_per_frame / tot / 1e6, 3), "p50_ms": r50,
+                          "p99_ms": r99, "bytes_to_host_per_step": moved}
+    for o in outs:
+        rgl.check("rgl_node_points_yield_configure_prefetch", o.h, False)
+
     point_bytes = sum(rgl.field_size(f) for f in FIELDS)
     b_io = b_trav = 0
     sums = np.zeros(3, np.int64)
@@ -249,6 +286,7 @@ def run_batch64(args):
                   "p50_ms": pp50, "p99_ms": pp99, "raytrace_kernel_ms_per_step": round(p_kernel, 4),
                   "kept_points_per_step": p_kept, "mode": "one rgl_graph_run per sensor"},
         "batched_vs_plain": round(value / plain_value, 3),
+        "with_readback": readback,
         "raytrace_kernel_ms": round(kernel_ms, 4),
         "raytrace_launches_per_step": launches_per_frame,
         "traversal_per_ray": {"nodes": round(sums[0] / rays_per_frame, 2), "triangles": round(sums[1] / rays_per_frame, 2),

@@ -64,6 +64,12 @@ RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable);
 // Total device time (ms) and number of batched raytrace launches timed since timing was enabled; waits for them.
 RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int32_t* out_launches);
 
+// Prefetch of a yield node's results (the reference prefetches XYZ of every yield node to pinned host memory,
+// src/graph/YieldPointsNode.cpp:31-35): when enabled, every run of the graph also copies the live points of every
+// yielded field into pinned host memory with a kernel that writes over the bus, so the results are host-resident
+// when the run completes and rgl_graph_get_result_data is a host copy. Off by default.
+RGL_API rgl_status_t rgl_node_points_yield_configure_prefetch(rgl_node_t node, bool enable);
+
 // Radar range x azimuth binning (build-defined; the reference instead clusters radar hits by scopes with
 // physical-optics RCS, src/graph/RadarPostprocessPointsNode.cpp:88-189). Input: XYZ, DISTANCE, INCIDENT_ANGLE and
 // IS_HIT. A hit at distance d whose point has sensor-frame azimuth a = atan2(y, x) (sensor frame = the node's

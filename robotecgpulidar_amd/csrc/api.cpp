@@ -17,6 +17,8 @@
 #include <set>
 #include <string>
 
+#include <cstring>
+
 #include "graph.hpp"
 #include "server.hpp"
 #include "scene.hpp"
@@ -750,6 +752,16 @@ RGL_API rgl_status_t rgl_node_points_compact_by_field(rgl_node_t* node, rgl_fiel
 
 RGL_API rgl_status_t rgl_node_points_compact(rgl_node_t* node) { return rgl_node_points_compact_by_field(node, RGL_FIELD_IS_HIT_I32); }
 
+RGL_API rgl_status_t rgl_node_points_yield_configure_prefetch(rgl_node_t node, bool enable)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		auto n = nodeAs<YieldPointsNode>(node, "YieldPointsNode");
+		syncNodeGraph(n);
+		n->setPrefetch(enable);
+	});
+}
+
 RGL_API rgl_status_t rgl_node_points_radar_bin(rgl_node_t* node, float range_step, float range_max, float azimuth_step,
                                                float azimuth_min, float azimuth_max, float ray_solid_angle)
 {
@@ -960,7 +972,13 @@ RGL_API rgl_status_t rgl_graph_get_result_data(rgl_node_t node, rgl_field_t fiel
 		checkOverflow(pn);
 		const size_t bytes = static_cast<size_t>(pn->hostCount()) * pn->fieldPointSize(field);
 		if (bytes > 0) {
-			RGL_HIP(hipMemcpy(dst, pn->fieldData(field), bytes, hipMemcpyDeviceToHost));
+			auto yield = std::dynamic_pointer_cast<YieldPointsNode>(pn);
+			const void* prefetched = yield ? yield->hostData(field) : nullptr;
+			if (prefetched != nullptr) {
+				std::memcpy(dst, prefetched, bytes); // written by the run's own copy kernel
+			} else {
+				RGL_HIP(hipMemcpy(dst, pn->fieldData(field), bytes, hipMemcpyDeviceToHost));
+			}
 		}
 	});
 }

@@ -273,7 +273,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						it.firstBlock = L.opBlocks;
 						L.opBlocks += gpu::pointOpBlocks(it.capacity);
 						L.op.push_back(it);
-						if (it.op == kOpFormat) {
+						if (it.op == kOpFormat || it.op == kOpToHost) {
 							L.opLayout.push_back(static_cast<int>(pl.lists.size()));
 							pl.lists.push_back(layout);
 						} else {

@@ -219,6 +219,7 @@ enum PointOpKind : int32_t
 	kOpTransformRays = 3,   // out0 = m0 * in0 for rays                          (nodeKernels.cu:45-49)
 	kOpNoiseRays = 4,       // angular ray noise                                 (gaussianNoiseKernels.cu:20-29)
 	kOpFormat = 5,          // SoA -> AoS                                         (nodeKernels.cu:25-33)
+	kOpToHost = 6,          // copy fields (SoA) into host-mapped pinned memory  (YieldPointsNode.cpp:31-35)
 };
 
 struct PointOpItem
@@ -236,7 +237,7 @@ struct PointOpItem
 	int32_t axis;
 	uint64_t seed;
 	uint32_t run, tag;
-	const FieldCopyList* format; // kOpFormat field layout (device memory)
+	const FieldCopyList* format; // kOpFormat / kOpToHost field list (device memory)
 	int32_t recordSize;
 	int32_t firstBlock;
 };

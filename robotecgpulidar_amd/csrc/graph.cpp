@@ -207,6 +207,13 @@ void Node::markEnqueued(const std::shared_ptr<Event>& ev)
 	ranOnce = true;
 }
 
+void Node::waitForResultsIfRun()
+{
+	if (ranOnce && done) {
+		done->sync();
+	}
+}
+
 void Node::waitForResults()
 {
 	if (!isValid() || !ranOnce) {
@@ -921,6 +928,58 @@ void YieldPointsNode::setParameters(const std::vector<rgl_field_t>& f)
 	fields = f;
 }
 bool YieldPointsNode::hasField(rgl_field_t f) const { return std::find(fields.begin(), fields.end(), f) != fields.end(); }
+
+void YieldPointsNode::setPrefetch(bool on)
+{
+	prefetch = on;
+	prefetched.clear();
+}
+
+void YieldPointsNode::pointOp(PointOpItem& it, FieldCopyList& layout)
+{
+	const int cap = capacity();
+	layout = FieldCopyList{};
+	prefetched.clear();
+	for (auto f : fields) {
+		if (isPadding(f) || prefetched.count(f)) {
+			continue;
+		}
+		const size_t sz = fieldSizeChecked(f);
+		MappedHostBuffer& hb = host[f];
+		if (hb.capacity() < sz * std::max(cap, 1) && hb.capacity() > 0) {
+			waitForResultsIfRun(); // the previous run may still write into the buffer being replaced
+		}
+		hb.ensure(sz * std::max(cap, 1));
+		if (layout.count == kMaxCompactFields) {
+			throw std::logic_error("too many fields to prefetch");
+		}
+		layout.f[layout.count++] = FieldCopy{static_cast<const char*>(input->fieldData(f)), static_cast<char*>(hb.device()),
+		                                     static_cast<int32_t>(sz), 0};
+		prefetched.insert(f);
+	}
+	it.op = kOpToHost;
+	it.capacity = cap;
+	it.count = countDevice();
+}
+
+void YieldPointsNode::enqueueImpl()
+{
+	if (!prefetch) {
+		return;
+	}
+	PointOpItem it{};
+	FieldCopyList layout;
+	pointOp(it, layout);
+	gpu::copyFields(stream(), it.capacity, it.count, layout);
+}
+
+const void* YieldPointsNode::hostData(rgl_field_t f) const
+{
+	if (!prefetch || !prefetched.count(f)) {
+		return nullptr;
+	}
+	return host.at(f).host();
+}
 
 void FormatPointsNode::setParameters(const std::vector<rgl_field_t>& f)
 {

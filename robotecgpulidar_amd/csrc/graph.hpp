@@ -60,6 +60,8 @@ struct Node : std::enable_shared_from_this<Node>
 	void markEnqueued(const std::shared_ptr<Event>& ev);
 	// Waits until this node's device work of the last run is complete; throws if the node never ran.
 	void waitForResults();
+	// Waits for the last run's device work if there was one.
+	void waitForResultsIfRun();
 	void markDirty() { dirty = true; }
 	bool isValid() const { return !dirty; }
 
@@ -535,15 +537,24 @@ struct YieldPointsNode : IPointsNodeSingleInput
 {
 	void setParameters(const std::vector<rgl_field_t>& f);
 	std::string getName() const override { return "YieldPointsNode"; }
-	BatchStage batchStage() const override { return BatchStage::Passive; }
+	BatchStage batchStage() const override { return prefetch ? BatchStage::PointOp : BatchStage::Passive; }
 	std::vector<rgl_field_t> requiredFields() const override { return fields; }
 	bool hasField(rgl_field_t f) const override;
+	// Prefetch: every run also copies the yielded fields' live points into pinned host memory (one kernel writing
+	// over the bus), so results are host-resident when the run completes.
+	void setPrefetch(bool on);
+	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
+	// Host copy of a prefetched field of the last run, or nullptr.
+	const void* hostData(rgl_field_t f) const;
 
 protected:
-	void enqueueImpl() override {}
+	void enqueueImpl() override;
 
 private:
 	std::vector<rgl_field_t> fields;
+	bool prefetch = false;
+	std::map<rgl_field_t, MappedHostBuffer> host;
+	std::set<rgl_field_t> prefetched;
 };
 
 struct FormatPointsNode : IPointsNodeSingleInput

@@ -69,6 +69,8 @@ void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int
              int32_t* tileCounts, int32_t* countOut);
 // One elementwise stage (any PointOpKind but kOpFormat) over it.capacity elements (it.count live on the device).
 void pointOp(hipStream_t stream, const PointOpItem& it);
+// Copies element i of every field (f.src -> f.dst, f.size bytes) for i below the live count.
+void copyFields(hipStream_t stream, int capacity, const int32_t* count, const FieldCopyList& fields);
 // dst[i] = *src[i] for i < n (`src` lives in device memory).
 void gatherInts(hipStream_t stream, int n, const int32_t* const* src, int32_t* dst);
 void noiseNormals(hipStream_t stream, int n, float* out, uint64_t seed, uint32_t run, uint32_t tag);

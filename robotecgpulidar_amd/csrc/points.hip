@@ -271,8 +271,23 @@ __device__ __forceinline__ void applyOp(const PointOpItem& it, int i)
 		case kOpFormat:
 			opSoaToAos(i, it.recordSize, *it.format, static_cast<char*>(it.out0));
 			break;
+		case kOpToHost:
+			for (int f = 0; f < it.format->count; ++f) {
+				copyElem(it.format->f[f], i, i);
+			}
+			break;
 		default:
 			break;
+	}
+}
+
+__global__ void kCopyFields(int capacity, const int32_t* count, FieldCopyList fields)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < liveCount(capacity, count)) {
+		for (int f = 0; f < fields.count; ++f) {
+			copyElem(fields.f[f], i, i);
+		}
 	}
 }
 
@@ -440,11 +455,18 @@ void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, 
 
 void pointOp(hipStream_t stream, const PointOpItem& it)
 {
-	if (it.op == kOpFormat) {
-		throw std::invalid_argument("pointOp: format stages take their field layout by value (formatSoaToAos)");
+	if (it.op == kOpFormat || it.op == kOpToHost) {
+		throw std::invalid_argument("pointOp: field-list stages take their list by value (formatSoaToAos, copyFields)");
 	}
 	if (it.capacity > 0) {
 		hipLaunchKernelGGL(kPointOp, dim3(blocks(it.capacity)), dim3(kBlock), 0, stream, it);
+	}
+}
+
+void copyFields(hipStream_t stream, int capacity, const int32_t* count, const FieldCopyList& fields)
+{
+	if (capacity > 0 && fields.count > 0) {
+		hipLaunchKernelGGL(kCopyFields, dim3(blocks(capacity)), dim3(kBlock), 0, stream, capacity, count, fields);
 	}
 }
 

@@ -122,6 +122,44 @@ private:
 	size_t cap = 0;
 };
 
+// Pinned host memory mapped into the device's address space: kernels write results into it over the bus, and the
+// host reads them once the writing work has completed.
+class MappedHostBuffer
+{
+public:
+	MappedHostBuffer() = default;
+	MappedHostBuffer(const MappedHostBuffer&) = delete;
+	MappedHostBuffer& operator=(const MappedHostBuffer&) = delete;
+	~MappedHostBuffer() { release(); }
+	void* ensure(size_t bytes)
+	{
+		if (bytes > cap) {
+			release();
+			const size_t want = bytes < 256 ? 256 : bytes;
+			RGL_HIP(hipHostMalloc(&ptr, want, hipHostMallocMapped | hipHostMallocCoherent));
+			RGL_HIP(hipHostGetDevicePointer(&dev, ptr, 0));
+			cap = want;
+		}
+		return ptr;
+	}
+	void* host() const { return ptr; }
+	void* device() const { return dev; }
+	size_t capacity() const { return cap; }
+	void release()
+	{
+		if (ptr != nullptr) {
+			(void) hipHostFree(ptr);
+			ptr = dev = nullptr;
+			cap = 0;
+		}
+	}
+
+private:
+	void* ptr = nullptr;
+	void* dev = nullptr;
+	size_t cap = 0;
+};
+
 class Stream
 {
 public:

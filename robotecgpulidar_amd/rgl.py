@@ -188,6 +188,7 @@ _PROTOTYPES = [
     ("rgl_configure_bvh_format", [C.c_int32]),
     ("rgl_get_bvh_format", [i32p]),
     ("rgl_node_points_radar_bin", [P(c_node)] + [C.c_float] * 6),
+    ("rgl_node_points_yield_configure_prefetch", [c_node, C.c_bool]),
     ("rgl_server_create", [P(C.c_void_p), C.c_int32, C.c_int32, C.c_int64, C.c_bool]),
     ("rgl_server_destroy", [C.c_void_p]),
     ("rgl_server_submit", [C.c_void_p, c_node, P(C.c_int64)]),
