@@ -242,6 +242,11 @@ def run_batch64(args):
     for o in outs:
         rgl.check("rgl_node_points_yield_configure_prefetch", o.h, False)
 
+    rtv, cpv = C.c_int32(), C.c_int32()
+    rgl.check("rgl_graph_batch_get_variants", C.byref(rtv), C.byref(cpv))
+    tuned = {"raytrace_block_threads": {0: 128, 1: 64}.get(rtv.value, "tuning"),
+             "compaction_points_per_thread": {0: 8, 1: 4}.get(cpv.value, "tuning")}
+
     point_bytes = sum(rgl.field_size(f) for f in FIELDS)
     b_io = b_trav = 0
     sums = np.zeros(3, np.int64)
@@ -280,7 +285,7 @@ def run_batch64(args):
                         "(rays->pose->range->raytrace->points transform->distance noise->compact->yield)",
             "sensors": len(sensors), "scenes": len(specs), "triangles": sum(s.triangle_count for s in specs),
             "scene_triangles": [s.triangle_count for s in specs], "rays_per_step": rays_per_frame,
-            "kept_points_per_step": b_kept, "mode": "batched",
+            "kept_points_per_step": b_kept, "mode": "batched", "autotuned": tuned,
         },
         "plain": {"value": round(plain_value, 3), "ms_per_step": round(p_total / args.steps * 1e3, 4),
                   "p50_ms": pp50, "p99_ms": pp99, "raytrace_kernel_ms_per_step": round(p_kernel, 4),
@@ -545,7 +550,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=8, help="untimed frames (the first six also settle the autotuner)")
     ap.add_argument("--workload", choices=["batch64", "urban", "serve", "multireturn"], default="batch64")
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the serving-loop measurement")
     ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",

@@ -58,6 +58,15 @@ RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene);
 // rgl_graph_get_result_size / rgl_graph_get_result_data as usual. Graphs may raytrace different scenes.
 RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* nodes, int32_t node_count);
 
+// Autotuning of rgl_graph_run_batch's launch shapes (on by default): while untuned, successive batches alternate
+// between the raytrace block sizes (128 or 64 threads) and the compaction tiles (8 or 4 points per thread), each
+// launch of at least 100k rays / points timed with HIP events; after three timings per variant the fastest per
+// element is kept. Variants never change results. configure_autotune(true) restarts tuning; force_variants fixes
+// the variants (0 or 1 each) and disables tuning; get_variants reports the variants in use (-1 while tuning).
+RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable);
+RGL_API rgl_status_t rgl_graph_batch_force_variants(int32_t raytrace_variant, int32_t compact_variant);
+RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant, int32_t* out_compact_variant);
+
 // Enables (and resets) HIP-event timing of the batched raytrace launches.
 RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable);
 

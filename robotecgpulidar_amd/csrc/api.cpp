@@ -927,6 +927,24 @@ RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int
 	});
 }
 
+RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable)
+{
+	return safeCall([&]() { GraphBatch::instance().setAutotune(enable); });
+}
+
+RGL_API rgl_status_t rgl_graph_batch_force_variants(int32_t raytrace_variant, int32_t compact_variant)
+{
+	return safeCall([&]() { GraphBatch::instance().forceVariants(raytrace_variant, compact_variant); });
+}
+
+RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant, int32_t* out_compact_variant)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_raytrace_variant != nullptr && out_compact_variant != nullptr);
+		GraphBatch::instance().autotuneChoice(out_raytrace_variant, out_compact_variant);
+	});
+}
+
 RGL_API rgl_status_t rgl_graph_destroy(rgl_node_t raw_node)
 {
 	return safeCall([&]() {

@@ -95,12 +95,124 @@ std::shared_ptr<Event> GraphBatch::completionEvent()
 
 GraphBatch::GraphBatch() = default;
 
+namespace {
+constexpr double kTuneMinUnits = 100000.0;
+constexpr int kTuneSamples = 3;
+} // namespace
+
+int GraphBatch::pick(const Tuner& t) const
+{
+	if (!autotune) {
+		return &t == &rtTune ? forcedRt : forcedCp;
+	}
+	if (t.chosen >= 0) {
+		return t.chosen;
+	}
+	int best = 0;
+	for (int v = 1; v < t.variants; ++v) {
+		if (t.samples[v] < t.samples[best]) {
+			best = v;
+		}
+	}
+	return best;
+}
+
+// Folds completed tuning timings into their tuners and settles a tuner once every variant has enough samples.
+void GraphBatch::harvest(bool wait)
+{
+	for (size_t i = 0; i < pending.size();) {
+		PendingTiming& p = pending[i];
+		if (wait) {
+			RGL_HIP(hipEventSynchronize(p.b));
+		} else {
+			const hipError_t e = hipEventQuery(p.b);
+			if (e == hipErrorNotReady) {
+				++i;
+				continue;
+			}
+			RGL_HIP(e);
+		}
+		float ms = 0.0f;
+		RGL_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+		Tuner& t = *p.tuner;
+		if (t.chosen < 0) {
+			t.ns[p.variant] += 1e6 * ms;
+			t.units[p.variant] += p.units;
+			++t.samples[p.variant];
+			bool done = true;
+			for (int v = 0; v < t.variants; ++v) {
+				done = done && t.samples[v] >= kTuneSamples;
+			}
+			if (done) {
+				int best = 0;
+				for (int v = 1; v < t.variants; ++v) {
+					if (t.ns[v] / t.units[v] < t.ns[best] / t.units[best]) {
+						best = v;
+					}
+				}
+				t.chosen = best;
+			}
+		}
+		tuneEvents.emplace_back(p.a, p.b);
+		pending[i] = pending.back();
+		pending.pop_back();
+	}
+}
+
+void GraphBatch::timeStart(hipStream_t s, PendingTiming& p)
+{
+	if (tuneEvents.empty()) {
+		hipEvent_t a, b;
+		RGL_HIP(hipEventCreate(&a));
+		RGL_HIP(hipEventCreate(&b));
+		tuneEvents.emplace_back(a, b);
+	}
+	p.a = tuneEvents.back().first;
+	p.b = tuneEvents.back().second;
+	tuneEvents.pop_back();
+	RGL_HIP(hipEventRecord(p.a, s));
+}
+
+void GraphBatch::setAutotune(bool enable)
+{
+	synchronize();
+	harvest(true);
+	autotune = enable;
+	forcedRt = forcedCp = 0;
+	rtTune = Tuner{gpu::kRaytraceVariants};
+	cpTune = Tuner{gpu::kCompactVariants};
+}
+
+void GraphBatch::forceVariants(int raytraceVariant, int compactVariant)
+{
+	if (raytraceVariant < 0 || raytraceVariant >= gpu::kRaytraceVariants || compactVariant < 0 ||
+	    compactVariant >= gpu::kCompactVariants) {
+		throw std::invalid_argument("no such launch variant");
+	}
+	setAutotune(false);
+	forcedRt = raytraceVariant;
+	forcedCp = compactVariant;
+}
+
+void GraphBatch::autotuneChoice(int32_t* raytraceVariant, int32_t* compactVariant) const
+{
+	*raytraceVariant = autotune ? rtTune.chosen : forcedRt;
+	*compactVariant = autotune ? cpTune.chosen : forcedCp;
+}
+
 GraphBatch::~GraphBatch()
 {
 	if (stream_) {
 		(void) hipStreamSynchronize(stream_->get());
 	}
 	for (auto& [a, b] : timingEvents) {
+		(void) hipEventDestroy(a);
+		(void) hipEventDestroy(b);
+	}
+	for (auto& p : pending) {
+		tuneEvents.emplace_back(p.a, p.b);
+	}
+	for (auto& [a, b] : tuneEvents) {
 		(void) hipEventDestroy(a);
 		(void) hipEventDestroy(b);
 	}
@@ -178,6 +290,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	}
 	const hipStream_t s = stream_->get();
 	uploaded->sync(); // the previous batch's descriptor upload has read the staging buffer
+	harvest(false);
+	const int rtVariant = pick(rtTune);
+	const int cpVariant = pick(cpTune);
 	for (auto& c : pl.ctxs) {
 		c->beginBatch(s, *scratchEvent);
 	}
@@ -251,12 +366,12 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						RaytraceBatchItem it = rt->batchItem(s);
 						if (gpu::beamIsMulti(it.p)) {
 							it.firstBlock = L.rtmBlocks;
-							L.rtmBlocks += gpu::raytraceBlocks(it.p.rayCount, true);
+							L.rtmBlocks += gpu::raytraceBlocks(it.p.rayCount, true, 0);
 							L.rtm.push_back(it);
 							L.rtmNodes.push_back(rt);
 						} else {
 							it.firstBlock = L.rtBlocks;
-							L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount, false);
+							L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount, false, rtVariant);
 							L.rt.push_back(it);
 							L.rtNodes.push_back(rt);
 						}
@@ -284,7 +399,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 					case BatchStage::Compact: {
 						auto* cn = dynamic_cast<CompactByFieldPointsNode*>(n);
 						FieldCopyList list;
-						CompactItem it = cn->compactItem(list);
+						CompactItem it = cn->compactItem(list, cpVariant);
 						it.firstTile = L.cpTiles;
 						L.cpTiles += it.numTiles;
 						L.cp.push_back(it);
@@ -392,13 +507,26 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 					RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
 				}
 				if (!L.rt.empty()) {
+					double rays = 0.0;
+					for (const auto& it : L.rt) {
+						rays += it.p.rayCount;
+					}
+					const bool tune = autotune && rtTune.chosen < 0 && rays >= kTuneMinUnits;
+					PendingTiming pt{nullptr, nullptr, &rtTune, rtVariant, rays};
+					if (tune) {
+						timeStart(s, pt);
+					}
 					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
-					                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false);
+					                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false, rtVariant);
 					RGL_HIP(hipGetLastError());
+					if (tune) {
+						RGL_HIP(hipEventRecord(pt.b, s));
+						pending.push_back(pt);
+					}
 				}
 				if (!L.rtm.empty()) {
 					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtmOff),
-					                   static_cast<int>(L.rtm.size()), L.rtmBlocks, Scene::bvhFormat(), true);
+					                   static_cast<int>(L.rtm.size()), L.rtmBlocks, Scene::bvhFormat(), true, 0);
 					RGL_HIP(hipGetLastError());
 				}
 				if (timed) {
@@ -412,9 +540,22 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				RGL_HIP(hipGetLastError());
 			}
 			if (!L.cp.empty()) {
+				double points = 0.0;
+				for (const auto& it : L.cp) {
+					points += it.capacity;
+				}
+				const bool tune = autotune && cpTune.chosen < 0 && points >= kTuneMinUnits;
+				PendingTiming pt{nullptr, nullptr, &cpTune, cpVariant, points};
+				if (tune) {
+					timeStart(s, pt);
+				}
 				gpu::compactBatch(s, reinterpret_cast<const CompactItem*>(dev + L.cpOff), static_cast<int>(L.cp.size()),
-				                  L.cpTiles);
+				                  L.cpTiles, cpVariant);
 				RGL_HIP(hipGetLastError());
+				if (tune) {
+					RGL_HIP(hipEventRecord(pt.b, s));
+					pending.push_back(pt);
+				}
 			}
 			for (Node* n : L.other) {
 				n->enqueue();

@@ -857,7 +857,7 @@ const void* GaussianNoiseDistanceNode::fieldData(rgl_field_t f)
 }
 
 // src/graph/CompactByFieldPointsNode.cpp:33-105; every field of the input is compacted in one pass.
-CompactItem CompactByFieldPointsNode::compactItem(FieldCopyList& list)
+CompactItem CompactByFieldPointsNode::compactItem(FieldCopyList& list, int variant)
 {
 	const int cap = capacity();
 	list = FieldCopyList{};
@@ -886,7 +886,7 @@ CompactItem CompactByFieldPointsNode::compactItem(FieldCopyList& list)
 	it.capacity = cap;
 	it.countIn = input->countDevice();
 	it.flags = cap > 0 ? static_cast<const int32_t*>(input->fieldData(by)) : nullptr;
-	it.numTiles = gpu::compactTiles(cap);
+	it.numTiles = gpu::compactTiles(cap, variant);
 	it.tileCounts = tileCounts.ensureT<int32_t>(it.numTiles);
 	it.countOut = count.ensureT<int32_t>(1);
 	countSlots.reset();

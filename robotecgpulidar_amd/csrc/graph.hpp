@@ -220,9 +220,34 @@ struct GraphBatch
 	// HIP-event timing of the batched raytrace launches while enabled.
 	void setTiming(bool enable);
 	void getTiming(double* totalMs, int32_t* launches);
+	// Autotuning of the raytrace block size and the compaction tile: while a family is untuned, successive batches
+	// cycle through its variants, timed with HIP events (only launches of at least 100k elements count); once every
+	// variant has three timings the one with the lowest time per element is kept. -1 = still tuning.
+	void setAutotune(bool enable);
+	// Disables autotuning and uses the given variants.
+	void forceVariants(int raytraceVariant, int compactVariant);
+	void autotuneChoice(int32_t* raytraceVariant, int32_t* compactVariant) const;
 
 private:
 	struct Plan;
+	struct Tuner
+	{
+		int variants;
+		int chosen = -1;
+		double ns[4] = {0, 0, 0, 0};
+		double units[4] = {0, 0, 0, 0};
+		int samples[4] = {0, 0, 0, 0};
+	};
+	struct PendingTiming
+	{
+		hipEvent_t a, b;
+		Tuner* tuner;
+		int variant;
+		double units;
+	};
+	int pick(const Tuner& t) const;
+	void harvest(bool wait);
+	void timeStart(hipStream_t s, PendingTiming& p);
 	std::shared_ptr<PinnedBuffer> hostSlots(size_t bytes);
 	std::shared_ptr<Event> completionEvent();
 
@@ -238,6 +263,11 @@ private:
 	bool timing = false;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
 	size_t timingUsed = 0;
+	bool autotune = true;
+	int forcedRt = 0, forcedCp = 0;
+	Tuner rtTune{2}, cpTune{2};
+	std::vector<PendingTiming> pending;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> tuneEvents; // free pairs
 };
 
 // ------------------------------------------------------------------------------------------------ node kinds
@@ -512,8 +542,9 @@ struct CompactByFieldPointsNode : IPointsNodeSingleInput
 	const int32_t* countDevice() const override { return count.as<int32_t>(); }
 	int hostCount() override;
 	const void* fieldData(rgl_field_t f) override;
-	// Prepares output buffers and returns this node's compaction description (firstTile unset).
-	CompactItem compactItem(FieldCopyList& list);
+	// Prepares output buffers and returns this node's compaction description (firstTile unset) for tiles of
+	// compaction variant `variant`.
+	CompactItem compactItem(FieldCopyList& list, int variant = 0);
 	// Host copy of the kept count of a batched run: slot `index` of `slots`.
 	void setCountHost(std::shared_ptr<PinnedBuffer> slots, int index)
 	{

@@ -49,22 +49,26 @@ void traversalStats(hipStream_t stream, const SceneView& scene, const RaytracePa
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
                        float* outT, int32_t* outInstance, int32_t* outPrim);
 void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
+// Tunable variants of the batched launches (chosen by the batch autotuner): raytrace single-sample blocks of
+// 128 (0) or 64 (1) threads; compaction tiles of 8 (0) or 4 (1) points per thread.
+constexpr int kRaytraceVariants = 2;
+constexpr int kCompactVariants = 2;
 // Blocks of a raytrace launch over rayCount beams: one beam per thread, or one wave per divergent beam.
-int raytraceBlocks(int rayCount, bool multi);
+int raytraceBlocks(int rayCount, bool multi, int variant);
 // True if the beams of `p` are traced with the 58-sample divergence model.
 bool beamIsMulti(const RaytraceParams& p);
 // Batched launches: `items` live in device memory; every item's scene uses node format `format`; with `multi` every
 // item has divergent beams (blocks from raytraceBlocks(n, true)), otherwise none has.
 void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
-                   bool multi);
+                   bool multi, int variant);
 int pointOpBlocks(int capacity);
 void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks);
-void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles);
+void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles, int variant);
 
 // ---------------------------------------------------------------- point-cloud kernels (points.hip)
 // Point clouds are processed over `capacity` elements; `count` (device, may be nullptr) holds the live count
 // when it is only known on the device (after compaction).
-int compactTiles(int capacity);
+int compactTiles(int capacity, int variant = 0);
 void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
              int32_t* tileCounts, int32_t* countOut);
 // One elementwise stage (any PointOpKind but kOpFormat) over it.capacity elements (it.count live on the device).

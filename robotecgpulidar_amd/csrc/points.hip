@@ -23,20 +23,21 @@ namespace rgl::gpu {
 namespace {
 
 constexpr int kBlock = 256;
+// Points per thread of a compaction tile: 8 (tiles of 2048 points) by default; the batch autotuner also tries 4.
 constexpr int kItems = 8;
-constexpr int kTile = kBlock * kItems; // 2048 points per compaction tile
 constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ int liveCount(int capacity, const int32_t* count) { return count ? *count : capacity; }
 
 // ------------------------------------------------------------------------------------------------ compaction
+template<int kItems_ = kItems>
 __device__ void tileCount(int capacity, const int32_t* countIn, const int32_t* flags, int tile, int32_t* tileCounts)
 {
 	__shared__ int32_t warpSums[kWaves];
 	const int n = liveCount(capacity, countIn);
-	const int base = tile * kTile;
+	const int base = tile * kBlock * kItems_;
 	int c = 0;
-	for (int k = 0; k < kItems; ++k) {
+	for (int k = 0; k < kItems_; ++k) {
 		const int i = base + k * kBlock + threadIdx.x;
 		c += (i < n && flags[i] != 0) ? 1 : 0;
 	}
@@ -83,11 +84,12 @@ __device__ __forceinline__ void copyElem(const FieldCopy& f, int src, int dst)
 // Offset of tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so each of
 // the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave) counts in
 // LDS, which preserves input order.
+template<int kItems_ = kItems>
 __device__ void compactScatter(int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
                                const int32_t* tileCounts, int tile, int numTiles, int32_t* countOut)
 {
 	__shared__ int32_t red[kBlock];
-	__shared__ int32_t waveBase[kItems][kWaves];
+	__shared__ int32_t waveBase[kItems_][kWaves];
 	const int n = liveCount(capacity, countIn);
 	int s = 0;
 	for (int t = threadIdx.x; t < tile; t += kBlock) {
@@ -108,9 +110,9 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	const int base = tile * kTile + threadIdx.x;
-	uint64_t masks[kItems];
-	for (int k = 0; k < kItems; ++k) {
+	const int base = tile * kBlock * kItems_ + threadIdx.x;
+	uint64_t masks[kItems_];
+	for (int k = 0; k < kItems_; ++k) {
 		const int i = base + k * kBlock;
 		masks[k] = __ballot(i < n && flags[i] != 0);
 		if (lane == 0) {
@@ -120,7 +122,7 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		int run = tileOffset;
-		for (int k = 0; k < kItems; ++k) {
+		for (int k = 0; k < kItems_; ++k) {
 			for (int w = 0; w < kWaves; ++w) {
 				const int c = waveBase[k][w];
 				waveBase[k][w] = run;
@@ -130,7 +132,7 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 	}
 	__syncthreads();
 	const int fieldCount = fields.count;
-	for (int k = 0; k < kItems; ++k) {
+	for (int k = 0; k < kItems_; ++k) {
 		if ((masks[k] >> lane) & 1ull) {
 			const int pos = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
 			for (int f = 0; f < fieldCount; ++f) {
@@ -166,19 +168,21 @@ __device__ __forceinline__ int findItem(const Item* items, int itemCount, Key ke
 	return __builtin_amdgcn_readfirstlane(lo);
 }
 
+template<int kItems_>
 __global__ void kTileCountsBatch(const CompactItem* items, int itemCount)
 {
 	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
-	tileCount(it.capacity, it.countIn, it.flags, static_cast<int>(blockIdx.x) - it.firstTile, it.tileCounts);
+	tileCount<kItems_>(it.capacity, it.countIn, it.flags, static_cast<int>(blockIdx.x) - it.firstTile, it.tileCounts);
 }
 
+template<int kItems_>
 __global__ void kCompactScatterBatch(const CompactItem* items, int itemCount)
 {
 	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
-	compactScatter(it.capacity, it.countIn, it.flags, *it.fields, it.tileCounts,
-	               static_cast<int>(blockIdx.x) - it.firstTile, it.numTiles, it.countOut);
+	compactScatter<kItems_>(it.capacity, it.countIn, it.flags, *it.fields, it.tileCounts,
+	                        static_cast<int>(blockIdx.x) - it.firstTile, it.numTiles, it.countOut);
 }
 
 // ------------------------------------------------------------------------------------------------ element ops
@@ -422,7 +426,11 @@ inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
 } // namespace
 
-int compactTiles(int capacity) { return capacity <= 0 ? 1 : (capacity + kTile - 1) / kTile; }
+int compactTiles(int capacity, int variant)
+{
+	const int tile = kBlock * (variant == 1 ? 4 : kItems);
+	return capacity <= 0 ? 1 : (capacity + tile - 1) / tile;
+}
 int pointOpBlocks(int capacity) { return capacity <= 0 ? 1 : blocks(capacity); }
 
 void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
@@ -438,11 +446,17 @@ void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int
 	                   tiles, countOut);
 }
 
-void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles)
+void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles, int variant)
 {
-	if (itemCount > 0 && totalTiles > 0) {
-		hipLaunchKernelGGL(kTileCountsBatch, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
-		hipLaunchKernelGGL(kCompactScatterBatch, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+	if (itemCount <= 0 || totalTiles <= 0) {
+		return;
+	}
+	if (variant == 1) {
+		hipLaunchKernelGGL(kTileCountsBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kCompactScatterBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+	} else {
+		hipLaunchKernelGGL(kTileCountsBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kCompactScatterBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	}
 }
 

@@ -26,7 +26,10 @@ namespace rgl::gpu {
 
 namespace {
 
-constexpr int kRtBlock = 128;
+#ifndef RGL_RT_BLOCK
+#define RGL_RT_BLOCK 128
+#endif
+constexpr int kRtBlock = RGL_RT_BLOCK; // threads per raytrace block (the LDS short stack is laid out per block)
 // Occupancy target (waves per SIMD) of the single-sample raytrace kernels. The traversal is latency-bound, so more
 // waves in flight pay until register spills start: on gfx950, 5 waves (96 VGPRs) spill 16 B per lane and beat both
 // the compiler's own choice (4 waves, ~106 VGPRs) and 6 waves (80 VGPRs, 96 B of spills). 0 = compiler's choice.
@@ -170,12 +173,12 @@ __device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<f
 // Closest hit of one ray against the two-level BVH; returns false if nothing is hit in (0, tmax].
 // kFmt selects the node format of the TLAS and BLASes (BvhFormat). With kStats the canonical binary trees are
 // traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted.
-template<bool kStats, int kFmt>
+template<bool kStats, int kFmt, int kBlock = kRtBlock>
 __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
 {
 	static_assert(!kStats || kFmt == kBvh2, "traversal statistics are defined on the binary trees");
 	// Short stack in LDS, [depth][thread] so a wave's pushes hit distinct banks; deeper entries spill to scratch.
-	__shared__ int32_t ldsStack[kLdsStack * kRtBlock];
+	__shared__ int32_t ldsStack[kLdsStack * kBlock];
 	best.t = tmax;
 	best.inst = INT_MAX;
 	best.prim = INT_MAX;
@@ -193,7 +196,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 #define RGL_PUSH(v)                                                                                                    \
 	do {                                                                                                               \
 		if (sp < kLdsStack) {                                                                                          \
-			ldsStack[sp * kRtBlock + tid] = (v);                                                                       \
+			ldsStack[sp * kBlock + tid] = (v);                                                                         \
 			++sp;                                                                                                      \
 		} else if (sp < kLdsStack + kOvfStack) {                                                                       \
 			ovf[sp - kLdsStack] = __int_as_float(v);                                                                   \
@@ -202,7 +205,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			overflow = true;                                                                                           \
 		}                                                                                                              \
 	} while (0)
-#define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kRtBlock + tid] : __float_as_int(ovf[sp - kLdsStack]))
+#define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kBlock + tid] : __float_as_int(ovf[sp - kLdsStack]))
 #define RGL_CSWAP(a, b)                                                                                                \
 	do {                                                                                                               \
 		if (k##b < k##a) {                                                                                             \
@@ -563,7 +566,7 @@ __device__ __forceinline__ Mat3x4f distortedRay(const RaytraceParams& p, int bea
 
 // One beam traced with a single sample (no divergence): the beam reduction degenerates to "every return is the
 // primary sample" (src/gpu/nodeKernels.cu:345-354,399-421 with one sample), so no trackers are kept.
-template<int kFmt>
+template<int kFmt, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                              const int beam)
 {
@@ -582,7 +585,7 @@ __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const Raytrace
 	rayOriginDir(ray, so, sd);
 	HitRec h;
 	float d0 = p.farNonHitDistance;
-	if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
+	if (traceClosest<false, kFmt, kBlock>(sc, so, sd, range.y, h, nullptr)) {
 		const Shade sh = shadeSample(sc, p, range.x, so, sd, h, p.velocityRequested != 0);
 		if (sh.isHit) {
 			for (int r = 0; r < R; ++r) {
@@ -709,7 +712,7 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 
 constexpr int kBeamsPerMultiBlock = kRtBlock / 64; // divergent beams: one wave each
 
-template<int kFmt, bool kMulti>
+template<int kFmt, bool kMulti, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                             int localBlock)
 {
@@ -719,16 +722,16 @@ __device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceP
 			raytraceBeamWave<kFmt>(sc, p, out, beam);
 		}
 	} else {
-		const int beam = localBlock * kRtBlock + threadIdx.x;
+		const int beam = localBlock * kBlock + threadIdx.x;
 		if (beam < p.rayCount) {
-			raytraceBeam<kFmt>(sc, p, out, beam);
+			raytraceBeam<kFmt, kBlock>(sc, p, out, beam);
 		}
 	}
 }
 
 // Many sensors (possibly on different scenes) in one launch: block b belongs to the last item whose firstBlock <= b.
 // The item index is wave-uniform, so its descriptor is read with scalar loads.
-template<int kFmt, bool kMulti>
+template<int kFmt, bool kMulti, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items, int itemCount)
 {
 	int lo = 0, hi = itemCount - 1;
@@ -742,7 +745,7 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 	}
 	const int item = __builtin_amdgcn_readfirstlane(lo);
 	const RaytraceBatchItem& it = items[item];
-	raytraceOne<kFmt, kMulti>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
+	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
 }
 
 // Single-sample beams (no divergence), one per thread: occupancy-tuned. Divergent beams, one per wave, keep the
@@ -759,10 +762,10 @@ __global__ void __launch_bounds__(kRtBlock) kRaytraceMulti(SceneView sc, Raytrac
 	raytraceOne<kFmt, true>(sc, p, out, blockIdx.x);
 }
 
-template<int kFmt>
-__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
+template<int kFmt, int kBlock>
+__global__ void __launch_bounds__(kBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
 {
-	raytraceBatchOne<kFmt, false>(items, itemCount);
+	raytraceBatchOne<kFmt, false, kBlock>(items, itemCount);
 }
 
 template<int kFmt>
@@ -849,12 +852,15 @@ void launchRaytrace(hipStream_t stream, int blocks, bool multi, const SceneView&
 }
 
 template<int kFmt>
-void launchRaytraceBatch(hipStream_t stream, int blocks, bool multi, const RaytraceBatchItem* items, int itemCount)
+void launchRaytraceBatch(hipStream_t stream, int blocks, bool multi, int variant, const RaytraceBatchItem* items,
+                         int itemCount)
 {
 	if (multi) {
 		hipLaunchKernelGGL(kRaytraceBatchMulti<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, items, itemCount);
+	} else if (variant == 1) {
+		hipLaunchKernelGGL((kRaytraceBatch<kFmt, 64>), dim3(blocks), dim3(64), 0, stream, items, itemCount);
 	} else {
-		hipLaunchKernelGGL(kRaytraceBatch<kFmt>, dim3(blocks), dim3(kRtBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL((kRaytraceBatch<kFmt, kRtBlock>), dim3(blocks), dim3(kRtBlock), 0, stream, items, itemCount);
 	}
 }
 } // namespace
@@ -867,7 +873,7 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 		return;
 	}
 	const bool multi = beamIsMulti(params);
-	const int blocks = raytraceBlocks(params.rayCount, multi);
+	const int blocks = raytraceBlocks(params.rayCount, multi, 0);
 	switch (scene.format) {
 		case kBvh4F: launchRaytrace<kBvh4F>(stream, blocks, multi, scene, params, out); break;
 		case kBvh4Q: launchRaytrace<kBvh4Q>(stream, blocks, multi, scene, params, out); break;
@@ -875,22 +881,22 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 	}
 }
 
-int raytraceBlocks(int rayCount, bool multi)
+int raytraceBlocks(int rayCount, bool multi, int variant)
 {
-	const int per = multi ? kBeamsPerMultiBlock : kRtBlock;
+	const int per = multi ? kBeamsPerMultiBlock : (variant == 1 ? 64 : kRtBlock);
 	return (rayCount + per - 1) / per;
 }
 
 void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
-                   bool anyMulti)
+                   bool multi, int variant)
 {
 	if (itemCount <= 0 || totalBlocks <= 0) {
 		return;
 	}
 	switch (format) {
-		case kBvh4F: launchRaytraceBatch<kBvh4F>(stream, totalBlocks, anyMulti, items, itemCount); break;
-		case kBvh4Q: launchRaytraceBatch<kBvh4Q>(stream, totalBlocks, anyMulti, items, itemCount); break;
-		default: launchRaytraceBatch<kBvh2>(stream, totalBlocks, anyMulti, items, itemCount); break;
+		case kBvh4F: launchRaytraceBatch<kBvh4F>(stream, totalBlocks, multi, variant, items, itemCount); break;
+		case kBvh4Q: launchRaytraceBatch<kBvh4Q>(stream, totalBlocks, multi, variant, items, itemCount); break;
+		default: launchRaytraceBatch<kBvh2>(stream, totalBlocks, multi, variant, items, itemCount); break;
 	}
 }
 

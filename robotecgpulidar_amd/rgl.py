@@ -185,6 +185,9 @@ _PROTOTYPES = [
     ("rgl_graph_run_batch", [P(c_node), C.c_int32]),
     ("rgl_graph_batch_set_kernel_timing", [C.c_bool]),
     ("rgl_graph_batch_get_kernel_timing", [P(C.c_double), i32p]),
+    ("rgl_graph_batch_configure_autotune", [C.c_bool]),
+    ("rgl_graph_batch_force_variants", [C.c_int32, C.c_int32]),
+    ("rgl_graph_batch_get_variants", [i32p, i32p]),
     ("rgl_configure_bvh_format", [C.c_int32]),
     ("rgl_get_bvh_format", [i32p]),
     ("rgl_node_points_radar_bin", [P(c_node)] + [C.c_float] * 6),
