@@ -186,7 +186,14 @@ def run_batch64(args):
         plain_frame()
         batched_frame()
     p_total, p_lat, p_kept = timed(plain_frame, args.steps)
+    caps0 = rgl.batch_capture()
     b_total, b_lat, b_kept = timed(batched_frame, args.steps)
+    caps1 = rgl.batch_capture()
+    # The same batches launched kernel by kernel (HIP graph capture off), for comparison.
+    rgl.batch_capture(False)
+    batched_frame()
+    e_total, e_lat, _ = timed(batched_frame, args.steps)
+    rgl.batch_capture(True)
     # Kernel durations from HIP events, in separate frames so the timed frames carry no extra events.
     kframes = 3
     for g in graphs:
@@ -291,6 +298,11 @@ def run_batch64(args):
                   "p50_ms": pp50, "p99_ms": pp99, "raytrace_kernel_ms_per_step": round(p_kernel, 4),
                   "kept_points_per_step": p_kept, "mode": "one rgl_graph_run per sensor"},
         "batched_vs_plain": round(value / plain_value, 3),
+        "graph_replays_per_step": round((caps1[1] - caps0[1]) / args.steps, 3),
+        "batched_uncaptured": {"value": round(args.steps * rays_per_frame / e_total / 1e6, 3),
+                               "ms_per_step": round(e_total / args.steps * 1e3, 4),
+                               "p50_ms": latency_stats(e_lat)[0], "p99_ms": latency_stats(e_lat)[1],
+                               "mode": "same batches, kernels launched one by one (no HIP graph replay)"},
         "with_readback": readback,
         "raytrace_kernel_ms": round(kernel_ms, 4),
         "raytrace_launches_per_step": launches_per_frame,

@@ -927,6 +927,20 @@ RGL_API rgl_status_t rgl_graph_batch_get_kernel_timing(double* out_total_ms, int
 	});
 }
 
+RGL_API rgl_status_t rgl_graph_batch_configure_capture(bool enable)
+{
+	return safeCall([&]() { GraphBatch::instance().setGraphs(enable); });
+}
+
+RGL_API rgl_status_t rgl_graph_batch_get_capture_stats(int64_t* out_captures, int64_t* out_replays)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_captures != nullptr && out_replays != nullptr);
+		*out_captures = GraphBatch::instance().graphCaptures;
+		*out_replays = GraphBatch::instance().graphReplays;
+	});
+}
+
 RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable)
 {
 	return safeCall([&]() { GraphBatch::instance().setAutotune(enable); });

@@ -200,10 +200,36 @@ void GraphBatch::autotuneChoice(int32_t* raytraceVariant, int32_t* compactVarian
 	*compactVariant = autotune ? cpTune.chosen : forcedCp;
 }
 
+void GraphBatch::CapturedGraph::release()
+{
+	if (exec != nullptr) {
+		(void) hipGraphExecDestroy(exec);
+		exec = nullptr;
+	}
+	if (graph != nullptr) {
+		(void) hipGraphDestroy(graph);
+		graph = nullptr;
+	}
+}
+
+void GraphBatch::setGraphs(bool enable)
+{
+	synchronize();
+	for (auto& cg : captured) {
+		cg.release();
+	}
+	captured.clear();
+	recentSigs.clear();
+	graphs = enable;
+}
+
 GraphBatch::~GraphBatch()
 {
 	if (stream_) {
 		(void) hipStreamSynchronize(stream_->get());
+	}
+	for (auto& cg : captured) {
+		cg.release();
 	}
 	for (auto& [a, b] : timingEvents) {
 		(void) hipEventDestroy(a);
@@ -489,94 +515,193 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				std::memcpy(host + L.cpOff, L.cp.data(), sizeof(CompactItem) * L.cp.size());
 			}
 		}
-		RGL_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
-		uploaded->record(s);
+		// Kept counts and overflow flags of every graph come back through one gather and one copy.
+		const int g = static_cast<int>(pl.gather.size());
+		const size_t slotBytes = sizeof(int32_t) * static_cast<size_t>(g);
+		int32_t* slots = nullptr;
+		std::shared_ptr<PinnedBuffer> hostBuf;
+		if (g > 0) {
+			if (slotBytes > slotsDev.capacity()) {
+				synchronize();
+			}
+			slots = slotsDev.ensureT<int32_t>(g);
+			hostBuf = hostSlots(slotBytes);
+		}
 
-		// Launches, level by level; nodes of one level never depend on each other.
+		// Which launches are timed this run (user timing, or an untuned family with a large enough launch).
+		bool tuneRt = false, tuneCp = false, anyOther = false;
 		for (int l = 0; l <= maxLevel; ++l) {
-			Plan::Level& L = pl.levels[l];
-			if (!L.rt.empty() || !L.rtm.empty()) {
-				const bool timed = timing && (L.rtBlocks > 0 || L.rtmBlocks > 0);
-				if (timed) {
-					if (timingUsed == timingEvents.size()) {
-						hipEvent_t a, b;
-						RGL_HIP(hipEventCreate(&a));
-						RGL_HIP(hipEventCreate(&b));
-						timingEvents.emplace_back(a, b);
+			const Plan::Level& L = pl.levels[l];
+			double rays = 0.0, points = 0.0;
+			for (const auto& it : L.rt) {
+				rays += it.p.rayCount;
+			}
+			for (const auto& it : L.cp) {
+				points += it.capacity;
+			}
+			tuneRt = tuneRt || (autotune && rtTune.chosen < 0 && rays >= kTuneMinUnits);
+			tuneCp = tuneCp || (autotune && cpTune.chosen < 0 && points >= kTuneMinUnits);
+			anyOther = anyOther || !L.other.empty();
+		}
+		const bool timed = timing || tuneRt || tuneCp;
+
+		// The batch's kernels: stage launches level by level (nodes of one level never depend on each other), then the
+		// count gather.
+		auto enqueueLaunches = [&](bool eager) {
+			for (int l = 0; l <= maxLevel; ++l) {
+				Plan::Level& L = pl.levels[l];
+				if (!L.rt.empty() || !L.rtm.empty()) {
+					const bool userTimed = eager && timing && (L.rtBlocks > 0 || L.rtmBlocks > 0);
+					if (userTimed) {
+						if (timingUsed == timingEvents.size()) {
+							hipEvent_t ea, eb;
+							RGL_HIP(hipEventCreate(&ea));
+							RGL_HIP(hipEventCreate(&eb));
+							timingEvents.emplace_back(ea, eb);
+						}
+						RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
 					}
-					RGL_HIP(hipEventRecord(timingEvents[timingUsed].first, s));
+					if (!L.rt.empty()) {
+						double rays = 0.0;
+						for (const auto& it : L.rt) {
+							rays += it.p.rayCount;
+						}
+						const bool tune = eager && autotune && rtTune.chosen < 0 && rays >= kTuneMinUnits;
+						PendingTiming pt{nullptr, nullptr, &rtTune, rtVariant, rays};
+						if (tune) {
+							timeStart(s, pt);
+						}
+						gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
+						                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false, rtVariant);
+						RGL_HIP(hipGetLastError());
+						if (tune) {
+							RGL_HIP(hipEventRecord(pt.b, s));
+							pending.push_back(pt);
+						}
+					}
+					if (!L.rtm.empty()) {
+						gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtmOff),
+						                   static_cast<int>(L.rtm.size()), L.rtmBlocks, Scene::bvhFormat(), true, 0);
+						RGL_HIP(hipGetLastError());
+					}
+					if (userTimed) {
+						RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, s));
+						++timingUsed;
+					}
 				}
-				if (!L.rt.empty()) {
-					double rays = 0.0;
-					for (const auto& it : L.rt) {
-						rays += it.p.rayCount;
+				if (!L.op.empty()) {
+					gpu::pointOpsBatch(s, reinterpret_cast<const PointOpItem*>(dev + L.opOff),
+					                   static_cast<int>(L.op.size()), L.opBlocks);
+					RGL_HIP(hipGetLastError());
+				}
+				if (!L.cp.empty()) {
+					double points = 0.0;
+					for (const auto& it : L.cp) {
+						points += it.capacity;
 					}
-					const bool tune = autotune && rtTune.chosen < 0 && rays >= kTuneMinUnits;
-					PendingTiming pt{nullptr, nullptr, &rtTune, rtVariant, rays};
+					const bool tune = eager && autotune && cpTune.chosen < 0 && points >= kTuneMinUnits;
+					PendingTiming pt{nullptr, nullptr, &cpTune, cpVariant, points};
 					if (tune) {
 						timeStart(s, pt);
 					}
-					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
-					                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false, rtVariant);
+					gpu::compactBatch(s, reinterpret_cast<const CompactItem*>(dev + L.cpOff),
+					                  static_cast<int>(L.cp.size()), L.cpTiles, cpVariant);
 					RGL_HIP(hipGetLastError());
 					if (tune) {
 						RGL_HIP(hipEventRecord(pt.b, s));
 						pending.push_back(pt);
 					}
 				}
-				if (!L.rtm.empty()) {
-					gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtmOff),
-					                   static_cast<int>(L.rtm.size()), L.rtmBlocks, Scene::bvhFormat(), true, 0);
-					RGL_HIP(hipGetLastError());
-				}
-				if (timed) {
-					RGL_HIP(hipEventRecord(timingEvents[timingUsed].second, s));
-					++timingUsed;
+				for (Node* n : L.other) {
+					n->enqueue();
 				}
 			}
-			if (!L.op.empty()) {
-				gpu::pointOpsBatch(s, reinterpret_cast<const PointOpItem*>(dev + L.opOff), static_cast<int>(L.op.size()),
-				                   L.opBlocks);
+			if (g > 0) {
+				gpu::gatherInts(s, g, reinterpret_cast<const int32_t* const*>(dev + gatherOff), slots);
 				RGL_HIP(hipGetLastError());
 			}
-			if (!L.cp.empty()) {
-				double points = 0.0;
-				for (const auto& it : L.cp) {
-					points += it.capacity;
-				}
-				const bool tune = autotune && cpTune.chosen < 0 && points >= kTuneMinUnits;
-				PendingTiming pt{nullptr, nullptr, &cpTune, cpVariant, points};
-				if (tune) {
-					timeStart(s, pt);
-				}
-				gpu::compactBatch(s, reinterpret_cast<const CompactItem*>(dev + L.cpOff), static_cast<int>(L.cp.size()),
-				                  L.cpTiles, cpVariant);
-				RGL_HIP(hipGetLastError());
-				if (tune) {
-					RGL_HIP(hipEventRecord(pt.b, s));
-					pending.push_back(pt);
+		};
+
+		RGL_HIP(hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s));
+		uploaded->record(s);
+		// A batch whose kernel sequence (descriptor buffer, item counts, grid shapes, variants) repeats the previous
+		// batch's is captured once into a HIP graph and replayed: one launch instead of one per stage. Only the
+		// descriptor contents change between replays, and the kernels read them from the buffer uploaded above.
+		std::vector<int64_t> sig;
+		if (graphs && !timed && !anyOther) {
+			sig = {reinterpret_cast<int64_t>(dev), static_cast<int64_t>(gatherOff), g, reinterpret_cast<int64_t>(slots),
+			       rtVariant, cpVariant, Scene::bvhFormat(), maxLevel};
+			for (int l = 0; l <= maxLevel; ++l) {
+				const Plan::Level& L = pl.levels[l];
+				for (int64_t v : {static_cast<int64_t>(L.rt.size()), static_cast<int64_t>(L.rtBlocks),
+				                  static_cast<int64_t>(L.rtOff), static_cast<int64_t>(L.rtm.size()),
+				                  static_cast<int64_t>(L.rtmBlocks), static_cast<int64_t>(L.rtmOff),
+				                  static_cast<int64_t>(L.op.size()), static_cast<int64_t>(L.opBlocks),
+				                  static_cast<int64_t>(L.opOff), static_cast<int64_t>(L.cp.size()),
+				                  static_cast<int64_t>(L.cpTiles), static_cast<int64_t>(L.cpOff)}) {
+					sig.push_back(v);
 				}
 			}
-			for (Node* n : L.other) {
-				n->enqueue();
+		}
+		CapturedGraph* hit = nullptr;
+		if (!sig.empty()) {
+			for (auto& cg : captured) {
+				if (cg.sig == sig) {
+					hit = &cg;
+				}
 			}
+		}
+		if (hit != nullptr) {
+			RGL_HIP(hipGraphLaunch(hit->exec, s));
+			hit->lastUse = ++graphClock;
+			++graphReplays;
+		} else if (!sig.empty() && std::find(recentSigs.begin(), recentSigs.end(), sig) != recentSigs.end()) {
+			if (captured.size() >= kCapturedGraphs) {
+				auto lru = std::min_element(captured.begin(), captured.end(),
+				                            [](const CapturedGraph& a, const CapturedGraph& b) { return a.lastUse < b.lastUse; });
+				synchronize(); // the graph being replaced may still run
+				lru->release();
+				captured.erase(lru);
+			}
+			CapturedGraph cg;
+			cg.sig = sig;
+			RGL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			try {
+				enqueueLaunches(false);
+			} catch (...) {
+				hipGraph_t broken = nullptr;
+				(void) hipStreamEndCapture(s, &broken);
+				if (broken != nullptr) {
+					(void) hipGraphDestroy(broken);
+				}
+				throw;
+			}
+			RGL_HIP(hipStreamEndCapture(s, &cg.graph));
+			const hipError_t e = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0);
+			if (e != hipSuccess) {
+				cg.release();
+				RGL_HIP(e);
+			}
+			RGL_HIP(hipGraphLaunch(cg.exec, s));
+			cg.lastUse = ++graphClock;
+			captured.push_back(std::move(cg));
+			++graphCaptures;
+		} else {
+			enqueueLaunches(true);
+			if (!sig.empty()) {
+				recentSigs.push_back(std::move(sig));
+				if (recentSigs.size() > kRecentSigs) {
+					recentSigs.erase(recentSigs.begin());
+				}
+			}
+		}
+		if (g > 0) {
+			RGL_HIP(hipMemcpyAsync(hostBuf->get(), slots, slotBytes, hipMemcpyDeviceToHost, s));
 		}
 		for (Scene* sc : pl.scenes) {
 			sc->noteUse(s);
 		}
-
-		// Kept counts and overflow flags of every graph: one gather, one copy.
-		const int g = static_cast<int>(pl.gather.size());
 		if (g > 0) {
-			const size_t bytes = sizeof(int32_t) * g;
-			if (bytes > slotsDev.capacity()) {
-				synchronize();
-			}
-			auto* slots = slotsDev.ensureT<int32_t>(g);
-			gpu::gatherInts(s, g, reinterpret_cast<const int32_t* const*>(dev + gatherOff), slots);
-			RGL_HIP(hipGetLastError());
-			auto hostBuf = hostSlots(bytes);
-			RGL_HIP(hipMemcpyAsync(hostBuf->get(), slots, bytes, hipMemcpyDeviceToHost, s));
 			int k = 0;
 			for (int l = 0; l <= maxLevel; ++l) {
 				for (auto* cn : pl.levels[l].cpNodes) {

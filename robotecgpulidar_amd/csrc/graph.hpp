@@ -226,6 +226,9 @@ struct GraphBatch
 	void setAutotune(bool enable);
 	// Disables autotuning and uses the given variants.
 	void forceVariants(int raytraceVariant, int compactVariant);
+	// Capture of repeated launch sequences into HIP graphs (on by default); counters of captures and replays.
+	void setGraphs(bool enable);
+	int64_t graphCaptures = 0, graphReplays = 0;
 	void autotuneChoice(int32_t* raytraceVariant, int32_t* compactVariant) const;
 
 private:
@@ -268,6 +271,21 @@ private:
 	Tuner rtTune{2}, cpTune{2};
 	std::vector<PendingTiming> pending;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> tuneEvents; // free pairs
+	// Captured kernel sequences, keyed by their launch signature; a signature is captured the second time it is
+	// seen among the last kRecentSigs batches, and at most kCapturedGraphs graphs are kept (least recently used go).
+	struct CapturedGraph
+	{
+		std::vector<int64_t> sig;
+		hipGraph_t graph = nullptr;
+		hipGraphExec_t exec = nullptr;
+		uint64_t lastUse = 0;
+		void release();
+	};
+	static constexpr size_t kCapturedGraphs = 4, kRecentSigs = 8;
+	bool graphs = true;
+	std::vector<CapturedGraph> captured;
+	std::vector<std::vector<int64_t>> recentSigs;
+	uint64_t graphClock = 0;
 };
 
 // ------------------------------------------------------------------------------------------------ node kinds

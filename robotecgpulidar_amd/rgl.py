@@ -188,6 +188,8 @@ _PROTOTYPES = [
     ("rgl_graph_batch_configure_autotune", [C.c_bool]),
     ("rgl_graph_batch_force_variants", [C.c_int32, C.c_int32]),
     ("rgl_graph_batch_get_variants", [i32p, i32p]),
+    ("rgl_graph_batch_configure_capture", [C.c_bool]),
+    ("rgl_graph_batch_get_capture_stats", [P(C.c_int64), P(C.c_int64)]),
     ("rgl_configure_bvh_format", [C.c_int32]),
     ("rgl_get_bvh_format", [i32p]),
     ("rgl_node_points_radar_bin", [P(c_node)] + [C.c_float] * 6),
@@ -419,6 +421,17 @@ def batch_kernel_timing(enable: bool | None = None):
     n = C.c_int32()
     check("rgl_graph_batch_get_kernel_timing", C.byref(ms), C.byref(n))
     return ms.value, n.value
+
+
+def batch_capture(enable: bool | None = None):
+    """Enables/disables HIP graph capture of batches, or (enable=None) returns (captures, replays)."""
+    if enable is not None:
+        check("rgl_graph_batch_configure_capture", bool(enable))
+        return None
+    cap = C.c_int64()
+    rep = C.c_int64()
+    check("rgl_graph_batch_get_capture_stats", C.byref(cap), C.byref(rep))
+    return cap.value, rep.value
 
 
 def result_size(node: Node, field: int) -> tuple[int, int]:
