@@ -284,6 +284,7 @@ struct Inst
 	int mesh;
 	M34 toWorld, toObject, prev;
 	bool hasPrev;
+	const float* disp; // object-space vertex displacement since the previous frame, or nullptr
 	int id;
 	float retro;
 	int texture;
@@ -473,6 +474,16 @@ Shade shade(const World& w, const OrcRequest& q, const M34& rayToWorld, double m
 		if (in.hasPrev) {
 			const V3 hitObject = in.toObject.point(hitWorld);
 			disp = hitWorld - in.prev.point(hitObject);
+		}
+		if (in.disp != nullptr) {
+			// src/gpu/optixPrograms.cu:186-196: vertex displacements in world orientation, interpolated at the hit.
+			const int* ix = &m.idx[3 * h.c.prim];
+			V3 dv[3];
+			for (int k = 0; k < 3; ++k) {
+				const float* p = in.disp + 3 * ix[k];
+				dv[k] = in.toWorld.rotationOnly().dir(V3{p[0], p[1], p[2]});
+			}
+			disp = disp + dv[0] * (1.0 - h.c.u - h.c.v) + dv[1] * h.c.u + dv[2] * h.c.v;
 		}
 		const V3 absVel = disp * (1.0 / q.sceneDeltaTime);
 		const V3 absInSensor = rayToWorld.rotationOnly().inverse().dir(absVel);
@@ -723,6 +734,8 @@ World makeWorld(const OrcScene* sc, bool brute)
 		in.toObject = in.toWorld.inverse();
 		in.hasPrev = sc->instHasPrev != nullptr && sc->instHasPrev[i] != 0;
 		in.prev = in.hasPrev ? M34::fromFloat(sc->instPrevTransform + 12 * i) : M34::identity();
+		in.disp = (sc->instDispOffset != nullptr && sc->instDispOffset[i] >= 0) ? sc->displacements + sc->instDispOffset[i]
+		                                                                          : nullptr;
 		in.id = sc->instId[i];
 		in.retro = sc->instRetro ? sc->instRetro[i] : 0.0f;
 		in.texture = sc->instTexture ? sc->instTexture[i] : -1;

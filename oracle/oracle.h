@@ -33,6 +33,10 @@ typedef struct
 	const int32_t* texHeight;
 	const int64_t* texOffset; /* byte offset of each texture in texels */
 	const uint8_t* texels;
+	/* Vertex animation (src/scene/Entity.cpp:103-117): per instance, the float offset into `displacements` of its
+	   object-space displacement since the previous frame (xyz per mesh vertex), or -1. NULL = none anywhere. */
+	const int64_t* instDispOffset;
+	const float* displacements;
 } OrcScene;
 
 typedef struct

@@ -30,6 +30,7 @@ class OrcScene(C.Structure):
         ("instTransform", f32p), ("instPrevTransform", f32p), ("instHasPrev", i32p), ("instId", i32p),
         ("instRetro", f32p), ("instTexture", i32p), ("nTextures", C.c_int), ("texWidth", i32p), ("texHeight", i32p),
         ("texOffset", C.POINTER(C.c_int64)), ("texels", C.POINTER(C.c_uint8)),
+        ("instDispOffset", C.POINTER(C.c_int64)), ("displacements", f32p),
     ]
 
 
@@ -108,7 +109,8 @@ class _Keep:
         return a
 
 
-def make_scene(spec, keep: _Keep, prev_transforms=None) -> OrcScene:
+def make_scene(spec, keep: _Keep, prev_transforms=None, displacements=None) -> OrcScene:
+    """`displacements`: {instance index: (vertices, 3) object-space vertex displacement since the previous frame}."""
     vo, to = [0], [0]
     verts, idx, uvs, has_uv = [], [], [], []
     for m in spec.meshes:
@@ -142,6 +144,16 @@ def make_scene(spec, keep: _Keep, prev_transforms=None) -> OrcScene:
         pv = keep(np.ascontiguousarray(np.asarray(prev_transforms, np.float32).reshape(n, 3, 4)))
         s.instPrevTransform = _p(pv, C.c_float)
         s.instHasPrev = _p(keep(np.ones(n, np.int32)), C.c_int32)
+    if displacements:
+        offs = np.full(n, -1, np.int64)
+        chunks, at = [], 0
+        for i, dv in sorted(displacements.items()):
+            dv = np.ascontiguousarray(dv, np.float32).reshape(-1, 3)
+            offs[i] = at
+            chunks.append(dv.ravel())
+            at += dv.size
+        s.instDispOffset = _p(keep(offs), C.c_int64)
+        s.displacements = _p(keep(np.concatenate(chunks)), C.c_float)
     s.instId = _p(keep(np.array([268435455 if i.entity_id is None else i.entity_id for i in spec.instances],
                                 np.int32)), C.c_int32)
     s.instRetro = _p(keep(np.array([i.laser_retro for i in spec.instances], np.float32)), C.c_float)
@@ -161,12 +173,12 @@ def raytrace(spec, rays_world, ray_origin_to_world=None, ranges=((0.0, 3.4028234
              time_offsets=None, mask=None, distortion=False, lin_vel=(0, 0, 0), ang_vel=(0, 0, 0),
              near=float("inf"), far=float("inf"), default_intensity=0.0, scene_dt=0.0, divergence=(0.0, 0.0),
              return_mode=(1 << 24) | 4, brute_force=False, threads=None, eps=1e-5, prev_transforms=None,
-             fields=None, world: "World | None" = None) -> dict:
+             fields=None, world: "World | None" = None, displacements=None) -> dict:
     """Runs the raytrace node semantics on the CPU. rays_world: (N, 3, 4) rays after the rays chain. Returns a dict
     of per-return-point arrays keyed by rgl field id, plus 'ambiguous', 'primaryT', 'primaryInst', 'primaryPrim'."""
     lib = load()
     keep = _Keep()
-    sc = make_scene(spec, keep, prev_transforms) if world is None else None
+    sc = make_scene(spec, keep, prev_transforms, displacements) if world is None else None
     rays = keep(np.ascontiguousarray(np.asarray(rays_world, np.float32).reshape(-1, 3, 4)))
     n = len(rays)
     q = OrcRequest()

@@ -378,7 +378,9 @@ RGL_API rgl_status_t rgl_mesh_set_bone_weights(rgl_mesh_t mesh, const rgl_bone_w
 		CHECK_ARG(mesh != nullptr);
 		CHECK_ARG(bone_weights != nullptr);
 		CHECK_ARG(bone_weights_count > 0);
-		throw NotImplementedCall("skeletal animation is not part of this library");
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Mesh>::get(mesh, "Mesh")->setBoneWeights(reinterpret_cast<const BoneWeights*>(bone_weights),
+		                                                      bone_weights_count);
 	});
 }
 
@@ -388,7 +390,8 @@ RGL_API rgl_status_t rgl_mesh_set_restposes(rgl_mesh_t mesh, const rgl_mat3x4f* 
 		CHECK_ARG(mesh != nullptr);
 		CHECK_ARG(restposes != nullptr);
 		CHECK_ARG(bones_count > 0);
-		throw NotImplementedCall("skeletal animation is not part of this library");
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Mesh>::get(mesh, "Mesh")->setRestposes(reinterpret_cast<const Mat3x4f*>(restposes), bones_count);
 	});
 }
 
@@ -451,7 +454,8 @@ RGL_API rgl_status_t rgl_entity_set_pose_world(rgl_entity_t entity, const rgl_ma
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(pose != nullptr);
 		CHECK_ARG(bones_count > 0);
-		throw NotImplementedCall("skeletal animation is not part of this library");
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Entity>::get(entity, "Entity")->setPoseAndAnimate(reinterpret_cast<const Mat3x4f*>(pose), bones_count);
 	});
 }
 
@@ -501,7 +505,9 @@ RGL_API rgl_status_t rgl_entity_apply_external_animation(rgl_entity_t entity, co
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(vertices != nullptr);
 		CHECK_ARG(vertex_count > 0);
-		throw NotImplementedCall("external vertex animation is not part of this library yet");
+		GraphRunCtx::synchronizeAll();
+		Registry<rgl::Entity>::get(entity, "Entity")->applyExternalAnimation(reinterpret_cast<const Vec3f*>(vertices),
+		                                                                    vertex_count);
 	});
 }
 

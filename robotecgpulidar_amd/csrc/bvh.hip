@@ -369,6 +369,36 @@ __global__ void kTriangleBoxes(int n, const Vec3f* v, const Vec3i* idx, Aabb* ou
 	out[i] = r;
 }
 
+// Animated vertices: the displacement since the previous step, then the new positions.
+__global__ void kUpdateVertices(int n, const Vec3f* fresh, Vec3f* vertices, Vec3f* displacement)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const Vec3f f = fresh[i];
+	displacement[i] = f - vertices[i];
+	vertices[i] = f;
+}
+
+// Linear blend skinning with up to four bones per vertex, summed in bone-slot order.
+__global__ void kSkinVertices(int n, const Vec3f* rest, const BoneWeights* weights, const Mat3x4f* matrices,
+                              Vec3f* vertices, Vec3f* displacement)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const Vec3f r = rest[i];
+	const BoneWeights w = weights[i];
+	Vec3f f = (matrices[w.bones[0]] * r) * w.weights[0];
+	f = f + (matrices[w.bones[1]] * r) * w.weights[1];
+	f = f + (matrices[w.bones[2]] * r) * w.weights[2];
+	f = f + (matrices[w.bones[3]] * r) * w.weights[3];
+	displacement[i] = f - vertices[i];
+	vertices[i] = f;
+}
+
 __global__ void kGatherTris(int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out)
 {
 	const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -717,6 +747,22 @@ void gatherTris(hipStream_t stream, int n, const int32_t* leafPrim, const Vec3f*
 {
 	if (n > 0) {
 		hipLaunchKernelGGL(kGatherTris, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, leafPrim, v, idx, out);
+	}
+}
+
+void updateVertices(hipStream_t stream, int n, const Vec3f* fresh, Vec3f* vertices, Vec3f* displacement)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kUpdateVertices, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, fresh, vertices, displacement);
+	}
+}
+
+void skinVertices(hipStream_t stream, int n, const Vec3f* rest, const BoneWeights* weights, const Mat3x4f* matrices,
+                  Vec3f* vertices, Vec3f* displacement)
+{
+	if (n > 0) {
+		hipLaunchKernelGGL(kSkinVertices, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, rest, weights, matrices,
+		                   vertices, displacement);
 	}
 }
 

@@ -93,14 +93,23 @@ struct alignas(16) InstanceShade
 	const Vec2f* uvs;           // nullptr if the mesh has no texture coordinates
 	const uint8_t* texels;      // nullptr if the entity has no intensity texture
 	const BvhNode* blas2;       // binary BLAS of the mesh (its root holds the mesh's object-space box)
+	const Vec3f* displacement;  // animated entity: object-space vertex displacement since the previous frame, or nullptr
 	int32_t texWidth;
 	int32_t texHeight;
 	int32_t entityId;
 	float laserRetro;
 	int32_t hasPrev;
-	int32_t pad[1];
+	int32_t pad[3];
 };
-static_assert(sizeof(InstanceShade) == 160, "instance shading record must be 160 B");
+static_assert(sizeof(InstanceShade) == 176, "instance shading record must be 176 B");
+
+// Skinning weights of one vertex (the layout of rgl_bone_weights_t, include/rgl/api/core.h:160-171).
+struct BoneWeights
+{
+	float weights[4];
+	int32_t bones[4];
+};
+static_assert(sizeof(BoneWeights) == 32, "bone weights record must be 32 B");
 
 // Device view of a scene for one launch.
 struct SceneView

@@ -37,6 +37,11 @@ void bvh4Ranks(hipStream_t stream, int n, const int32_t* parentSlot, void* scrat
 void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* scratch, int format, void* out);
 void triangleBoxes(hipStream_t stream, int n, const Vec3f* v, const Vec3i* idx, Aabb* out);
 void gatherTris(hipStream_t stream, int n, const int32_t* leafPrim, const Vec3f* v, const Vec3i* idx, LeafTri* out);
+// Animation vertex updates (src/gpu/sceneKernels.cu:18-49). External: displacement = fresh - vertices, vertices =
+// fresh. Skinning: fresh = sum_k w_k (matrices[bone_k] * rest), then the same update.
+void updateVertices(hipStream_t stream, int n, const Vec3f* fresh, Vec3f* vertices, Vec3f* displacement);
+void skinVertices(hipStream_t stream, int n, const Vec3f* rest, const BoneWeights* weights, const Mat3x4f* matrices,
+                  Vec3f* vertices, Vec3f* displacement);
 void instanceBoxes(hipStream_t stream, int n, const InstanceShade* shades, const InstanceRec* recs, Aabb* out);
 
 // ---------------------------------------------------------------- raytrace (raytrace.hip)

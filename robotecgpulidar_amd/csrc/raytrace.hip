@@ -461,6 +461,14 @@ __device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float
 		if (is.hasPrev) {
 			disp = hitWorld - is.prevObjectToWorld * hitObject;
 		}
+		if (is.displacement != nullptr) {
+			// Vertex animation: the vertex displacements rotated to world space, interpolated at the hit.
+			const Mat3x4f rot = is.objectToWorld.rotation();
+			const Vec3f dA = rot * is.displacement[tri.x];
+			const Vec3f dB = rot * is.displacement[tri.y];
+			const Vec3f dC = rot * is.displacement[tri.z];
+			disp = disp + ((1.0f - h.u - h.v) * dA + h.u * dB + h.v * dC);
+		}
 		const Vec3f absVel = disp / p.sceneDeltaTime;
 		const Mat3x4f rotInv = p.rayOriginToWorld.rotation().inverse();
 		const Vec3f absInSensor = rotInv * absVel;

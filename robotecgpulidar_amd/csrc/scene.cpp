@@ -75,12 +75,69 @@ void Mesh::setTexCoords(const Vec2f* uv, int count)
 	hasUVs = true;
 }
 
+void Mesh::setBoneWeights(const BoneWeights* w, int count)
+{
+	if (count != vertexCount) {
+		throw std::invalid_argument("Cannot set bone weights because vertex count do not match with bone weights count: "
+		                            "vertexCount=" +
+		                            std::to_string(vertexCount) + ", boneWeightsCount=" + std::to_string(count));
+	}
+	int maxBone = -1;
+	for (int i = 0; i < count; ++i) {
+		for (int k = 0; k < 4; ++k) {
+			if (w[i].bones[k] < 0) {
+				throw std::invalid_argument("bone index must not be negative (vertex " + std::to_string(i) + ")");
+			}
+			maxBone = std::max(maxBone, w[i].bones[k]);
+		}
+	}
+	RGL_HIP(hipMemcpy(boneWeights.ensure(sizeof(BoneWeights) * count), w, sizeof(BoneWeights) * count,
+	                  hipMemcpyHostToDevice));
+	hasBoneWeights = true;
+	maxBoneIndex = maxBone;
+}
+
+void Mesh::setRestposes(const Mat3x4f* r, int count) { restposes.assign(r, r + count); }
+
 void Mesh::ensureBlas(hipStream_t s)
 {
-	if (blasBuilt) {
-		return;
+	if (!blas.built) {
+		blas.build(s, triCount, vertices.as<Vec3f>(), indices.as<Vec3i>(), false);
 	}
-	const int n = triCount;
+}
+
+const void* Mesh::nodesFor(hipStream_t s, int format)
+{
+	ensureBlas(s);
+	return blas.nodesFor(s, format);
+}
+
+int64_t Mesh::accelBytes(int format) const { return blas.built ? blas.bytes() : estimatedBlasBytes(triCount, format); }
+
+void Mesh::releaseAccel() { blas.release(); }
+
+// ------------------------------------------------------------------------------------------------ Blas
+namespace {
+size_t nodeBytes(int format)
+{
+	return format == kBvh4F ? sizeof(Bvh4FNode) : (format == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+}
+} // namespace
+
+int64_t estimatedBlasBytes(int64_t t, int format)
+{
+	// Binary nodes 64 B, leaf triangles 48 B, leaf order 4 B, parent links 8 B, fit counters and heights 8 B,
+	// primitive boxes 24 B per triangle; at most one 4-wide node per binary internal node (an upper bound: about half
+	// of them are kept in practice).
+	const int64_t wideBytes = format == kBvh4F ? 128 * t : (format == kBvh4Q ? 64 * t : 0);
+	return t * (64 + 48 + 4 + 8 + 8 + 24) + wideBytes;
+}
+
+void Blas::build(hipStream_t s, int n, const Vec3f* v, const Vec3i* idx, bool refittable)
+{
+	release();
+	triCount = n;
+	keepScratch = refittable;
 	const int inner = std::max(n - 1, 1);
 	boxes.ensure(sizeof(Aabb) * n);
 	nodes.ensure(sizeof(BvhNode) * inner);
@@ -90,29 +147,36 @@ void Mesh::ensureBlas(hipStream_t s)
 	counters.ensure(sizeof(int32_t) * inner);
 	height.ensure(sizeof(int32_t) * inner);
 	scratch.ensure(gpu::lbvhScratchBytes(n));
-	gpu::triangleBoxes(s, n, vertices.as<Vec3f>(), indices.as<Vec3i>(), boxes.as<Aabb>());
+	gpu::triangleBoxes(s, n, v, idx, boxes.as<Aabb>());
 	gpu::buildLbvh(s, n, boxes.as<Aabb>(), false, nodes.as<BvhNode>(), leafPrim.as<int32_t>(), parentSlot.as<int32_t>(),
 	               counters.as<int32_t>(), height.as<int32_t>(), scratch.get());
-	gpu::gatherTris(s, n, leafPrim.as<int32_t>(), vertices.as<Vec3f>(), indices.as<Vec3i>(), tris.as<LeafTri>());
-	int32_t depth = 0;
-	RGL_HIP(hipMemcpyAsync(&depth, height.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+	gpu::gatherTris(s, n, leafPrim.as<int32_t>(), v, idx, tris.as<LeafTri>());
+	int32_t d = 0;
+	RGL_HIP(hipMemcpyAsync(&d, height.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
 	RGL_HIP(hipStreamSynchronize(s));
 	RGL_HIP(hipGetLastError());
-	blasDepth = depth;
+	depth = d;
 	scratch.release();
-	blasBuilt = true;
+	built = true;
 }
 
-namespace {
-size_t nodeBytes(int format)
+void Blas::refit(hipStream_t s, const Vec3f* v, const Vec3i* idx)
 {
-	return format == kBvh4F ? sizeof(Bvh4FNode) : (format == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+	const int n = triCount;
+	gpu::triangleBoxes(s, n, v, idx, boxes.as<Aabb>());
+	gpu::refitLbvh(s, n, boxes.as<Aabb>(), leafPrim.as<int32_t>(), parentSlot.as<int32_t>(), nodes.as<BvhNode>(),
+	               counters.as<int32_t>(), height.as<int32_t>());
+	gpu::gatherTris(s, n, leafPrim.as<int32_t>(), v, idx, tris.as<LeafTri>());
+	for (int f = kBvh4F; f < kBvhFormats; ++f) {
+		if (wideBuilt[f]) {
+			gpu::bvh4Collapse(s, n, nodes.as<BvhNode>(), wideRanks.get(), f, wide[f].get());
+		}
+	}
+	RGL_HIP(hipGetLastError());
 }
-} // namespace
 
-const void* Mesh::nodesFor(hipStream_t s, int format)
+const void* Blas::nodesFor(hipStream_t s, int format)
 {
-	ensureBlas(s);
 	if (format == kBvh2) {
 		return nodes.get();
 	}
@@ -121,52 +185,50 @@ const void* Mesh::nodesFor(hipStream_t s, int format)
 	}
 	if (!wideBuilt[format]) {
 		const int n = triCount;
-		DeviceBuffer tmp, countDev;
-		tmp.ensure(gpu::bvh4ScratchBytes(n));
-		countDev.ensure(sizeof(int32_t));
-		gpu::bvh4Ranks(s, n, parentSlot.as<int32_t>(), tmp.get(), countDev.as<int32_t>());
-		int32_t m = 0;
-		RGL_HIP(hipMemcpyAsync(&m, countDev.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-		RGL_HIP(hipStreamSynchronize(s));
-		gpu::bvh4Collapse(s, n, nodes.as<BvhNode>(), tmp.get(), format, wide[format].ensure(nodeBytes(format) * m));
+		if (wideRanks.get() == nullptr) {
+			DeviceBuffer countDev;
+			wideRanks.ensure(gpu::bvh4ScratchBytes(n));
+			countDev.ensure(sizeof(int32_t));
+			gpu::bvh4Ranks(s, n, parentSlot.as<int32_t>(), wideRanks.get(), countDev.as<int32_t>());
+			RGL_HIP(hipMemcpyAsync(&wideCount, countDev.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+			RGL_HIP(hipStreamSynchronize(s));
+		}
+		const size_t m = static_cast<size_t>(wideCount);
+		gpu::bvh4Collapse(s, n, nodes.as<BvhNode>(), wideRanks.get(), format, wide[format].ensure(nodeBytes(format) * m));
 		RGL_HIP(hipGetLastError());
 		RGL_HIP(hipStreamSynchronize(s));
 		wideBuilt[format] = true;
+		if (!keepScratch) {
+			wideRanks.release();
+		}
 	}
 	return wide[format].get();
 }
 
-int64_t Mesh::accelBytes(int format) const
+int64_t Blas::bytes() const
 {
-	if (blasBuilt) {
-		int64_t b = 0;
-		for (const DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch}) {
-			b += static_cast<int64_t>(d->capacity());
-		}
-		for (const DeviceBuffer& w : wide) {
-			b += static_cast<int64_t>(w.capacity());
-		}
-		return b;
+	int64_t b = 0;
+	for (const DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch, &wideRanks}) {
+		b += static_cast<int64_t>(d->capacity());
 	}
-	// Binary nodes 64 B, leaf triangles 48 B, leaf order 4 B, parent links 8 B, fit counters and heights 8 B,
-	// primitive boxes 24 B per triangle; at most one 4-wide node per binary internal node (an upper bound: about half
-	// of them are kept in practice).
-	const int64_t t = triCount;
-	const int64_t wideBytes = format == kBvh4F ? 128 * t : (format == kBvh4Q ? 64 * t : 0);
-	return t * (64 + 48 + 4 + 8 + 8 + 24) + wideBytes;
+	for (const DeviceBuffer& w : wide) {
+		b += static_cast<int64_t>(w.capacity());
+	}
+	return b;
 }
 
-void Mesh::releaseAccel()
+void Blas::release()
 {
-	for (DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch}) {
+	for (DeviceBuffer* d : {&nodes, &tris, &leafPrim, &parentSlot, &counters, &height, &boxes, &scratch, &wideRanks}) {
 		d->release();
 	}
 	for (int f = 0; f < kBvhFormats; ++f) {
 		wide[f].release();
 		wideBuilt[f] = false;
 	}
-	blasBuilt = false;
-	blasDepth = 0;
+	built = false;
+	depth = 0;
+	wideCount = 0;
 }
 
 // ------------------------------------------------------------------------------------------------ Texture
@@ -200,6 +262,106 @@ std::optional<Mat3x4f> Entity::previousFrameTransform() const
 		return std::nullopt;
 	}
 	return formerTransform;
+}
+
+Animation& Entity::animationOf(Animation::Kind kind)
+{
+	// A different kind of animation starts again from the mesh's own vertices (src/scene/Entity.cpp:88-101).
+	if (!animation || animation->kind != kind) {
+		auto a = std::make_unique<Animation>();
+		a->kind = kind;
+		const size_t bytes = sizeof(Vec3f) * static_cast<size_t>(mesh->vertexCount);
+		RGL_HIP(hipMemcpy(a->vertices.ensure(bytes), mesh->vertices.get(), bytes, hipMemcpyDeviceToDevice));
+		RGL_HIP(hipMemset(a->displacement.ensure(bytes), 0, bytes));
+		animation = std::move(a);
+	}
+	return *animation;
+}
+
+void Entity::animated()
+{
+	formerAnimationTime = animationTime;
+	animationTime = scene->getTime();
+	animation->blasStale = true;
+	scene->markDirty(false);
+}
+
+// src/scene/animator/ExternalAnimator.cpp:23-40
+void Entity::applyExternalAnimation(const Vec3f* v, int count)
+{
+	if (count != mesh->vertexCount) {
+		throw std::invalid_argument("Invalid argument: cannot apply external animation because vertex counts do not match: "
+		                            "old=" +
+		                            std::to_string(mesh->vertexCount) + ", new=" + std::to_string(count));
+	}
+	Animation& a = animationOf(Animation::External);
+	const size_t bytes = sizeof(Vec3f) * static_cast<size_t>(count);
+	RGL_HIP(hipMemcpy(a.staging.ensure(bytes), v, bytes, hipMemcpyHostToDevice));
+	gpu::updateVertices(nullptr, count, a.staging.as<Vec3f>(), a.vertices.as<Vec3f>(), a.displacement.as<Vec3f>());
+	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipStreamSynchronize(nullptr));
+	animated();
+}
+
+// src/scene/animator/SkeletonAnimator.cpp:19-58
+void Entity::setPoseAndAnimate(const Mat3x4f* pose, int bones)
+{
+	if (!mesh->hasBoneWeights) {
+		throw std::invalid_argument("Cannot create SkeletonAnimator because mesh has no bone weights defined.");
+	}
+	if (mesh->restposes.empty()) {
+		throw std::invalid_argument("Cannot create SkeletonAnimator because mesh has no restposes defined.");
+	}
+	if (static_cast<size_t>(bones) != mesh->restposes.size()) {
+		throw std::invalid_argument("Cannot perform skeleton animation because bones count do not match restposes count: "
+		                            "bones=" +
+		                            std::to_string(bones) + ", restposes=" + std::to_string(mesh->restposes.size()));
+	}
+	if (mesh->maxBoneIndex >= bones) {
+		throw std::invalid_argument("bone weights refer to bone " + std::to_string(mesh->maxBoneIndex) + " of " +
+		                            std::to_string(bones));
+	}
+	Animation& a = animationOf(Animation::Skeleton);
+	std::vector<Mat3x4f> m(static_cast<size_t>(bones));
+	for (int b = 0; b < bones; ++b) {
+		m[b] = pose[b] * mesh->restposes[b];
+	}
+	const size_t mb = sizeof(Mat3x4f) * m.size();
+	RGL_HIP(hipMemcpy(a.matrices.ensure(mb), m.data(), mb, hipMemcpyHostToDevice));
+	gpu::skinVertices(nullptr, mesh->vertexCount, mesh->vertices.as<Vec3f>(), mesh->boneWeights.as<BoneWeights>(),
+	                  a.matrices.as<Mat3x4f>(), a.vertices.as<Vec3f>(), a.displacement.as<Vec3f>());
+	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipStreamSynchronize(nullptr));
+	animated();
+}
+
+const Vec3f* Entity::displacementSincePrevFrame() const
+{
+	if (!animation || !formerAnimationTime.has_value() || formerAnimationTime != scene->getPrevTime()) {
+		return nullptr;
+	}
+	return animation->displacement.as<Vec3f>();
+}
+
+const Vec3f* Entity::tracedVertices() const
+{
+	return animation ? animation->vertices.as<Vec3f>() : mesh->vertices.as<Vec3f>();
+}
+
+Blas& Entity::tracedBlas(hipStream_t s)
+{
+	if (!animation) {
+		mesh->ensureBlas(s);
+		return mesh->blas;
+	}
+	Animation& a = *animation;
+	if (!a.blas.built) {
+		a.blas.build(s, mesh->triCount, a.vertices.as<Vec3f>(), mesh->indices.as<Vec3i>(), true);
+	} else if (a.blasStale) {
+		a.blas.refit(s, a.vertices.as<Vec3f>(), mesh->indices.as<Vec3i>());
+	}
+	a.blasStale = false;
+	return a.blas;
 }
 
 // ------------------------------------------------------------------------------------------------ Scene
@@ -326,8 +488,24 @@ int64_t Scene::accelBytes(int format) const
 	for (Mesh* m : meshes()) {
 		b += m->accelBytes(format);
 	}
+	for (auto& e : entities) {
+		if (e->animation) {
+			const Blas& a = e->animation->blas;
+			b += a.built ? a.bytes() : estimatedBlasBytes(e->mesh->triCount, format);
+		}
+	}
 	const int64_t n = static_cast<int64_t>(entities.size());
 	return b + n * (64 + 160 + 24 + 64 + 4 + 8 + 8) + n * 128;
+}
+
+void Scene::releaseEntityAccel()
+{
+	for (auto& e : entities) {
+		if (e->animation) {
+			e->animation->blas.release();
+		}
+	}
+	markDirty(false);
 }
 
 std::vector<int32_t> Scene::instanceTriCounts() const
@@ -351,8 +529,7 @@ void Scene::rebuild(hipStream_t s)
 	const int fmt = bvhFormat();
 	maxBlasDepth = 0;
 	for (auto& e : entities) {
-		e->mesh->ensureBlas(s);
-		maxBlasDepth = std::max(maxBlasDepth, e->mesh->blasDepth);
+		maxBlasDepth = std::max(maxBlasDepth, e->tracedBlas(s).depth);
 	}
 	if (n == 0) {
 		view = SceneView{};
@@ -364,20 +541,22 @@ void Scene::rebuild(hipStream_t s)
 	auto* shade = static_cast<InstanceShade*>(hostShade.ensure(sizeof(InstanceShade) * n));
 	for (int i = 0; i < n; ++i) {
 		const Entity& e = *entities[i];
+		Blas& blas = entities[i]->tracedBlas(s);
 		inst[i].worldToObject = inverseFp64(e.transform);
-		inst[i].blasNodes = e.mesh->nodesFor(s, fmt);
-		inst[i].blasTris = e.mesh->tris.as<LeafTri>();
+		inst[i].blasNodes = blas.nodesFor(s, fmt);
+		inst[i].blasTris = blas.tris.as<LeafTri>();
 		InstanceShade& sh = shade[i];
 		std::memset(&sh, 0, sizeof(sh));
 		sh.objectToWorld = e.transform;
 		const auto prev = e.previousFrameTransform();
 		sh.prevObjectToWorld = prev.value_or(Mat3x4f::identity());
 		sh.hasPrev = prev.has_value() ? 1 : 0;
-		sh.vertices = e.mesh->vertices.as<Vec3f>();
+		sh.vertices = e.tracedVertices();
+		sh.displacement = e.displacementSincePrevFrame();
 		sh.indices = e.mesh->indices.as<Vec3i>();
 		sh.uvs = e.mesh->hasUVs ? e.mesh->uvs.as<Vec2f>() : nullptr;
 		sh.texels = e.texture ? e.texture->texels.as<uint8_t>() : nullptr;
-		sh.blas2 = e.mesh->nodes.as<BvhNode>();
+		sh.blas2 = blas.nodes.as<BvhNode>();
 		sh.texWidth = e.texture ? e.texture->width : 0;
 		sh.texHeight = e.texture ? e.texture->height : 0;
 		sh.entityId = e.id;

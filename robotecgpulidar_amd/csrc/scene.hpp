@@ -19,10 +19,38 @@ namespace rgl {
 
 class Scene;
 
+// Bottom-level acceleration structure over one triangle set: a mesh's own vertices, or the vertices of an animated
+// entity (which the reference gives its own refittable GAS, src/scene/Scene.cpp:195-233).
+struct Blas
+{
+	// Full LBVH build over triangles `idx[0..triCount)` of `v`; waits for it to finish.
+	void build(hipStream_t stream, int triCount, const Vec3f* v, const Vec3i* idx, bool refittable);
+	// Same topology, new vertex positions: refits boxes bottom-up and regathers leaf triangles, and re-collapses the
+	// 4-wide trees already built; asynchronous on `stream`.
+	void refit(hipStream_t stream, const Vec3f* v, const Vec3i* idx);
+	// Nodes in `format` (4-wide trees are collapsed from the binary one on first use; waits for that).
+	const void* nodesFor(hipStream_t stream, int format);
+	int64_t bytes() const;
+	void release();
+
+	bool built = false;
+	bool keepScratch = false; // refittable: keeps the 4-wide collapse ranks for refits
+	int depth = 0;
+	int triCount = 0;
+	DeviceBuffer nodes, tris, leafPrim, parentSlot, counters, height, boxes, scratch;
+	DeviceBuffer wide[kBvhFormats]; // 4-wide BLAS per format (index kBvh4F, kBvh4Q)
+	DeviceBuffer wideRanks;   // 4-wide collapse ranks (topology only)
+	int32_t wideCount = 0;    // 4-wide node count (topology only)
+	bool wideBuilt[kBvhFormats] = {};
+};
+
 struct Mesh
 {
 	Mesh(const Vec3f* vertices, int vertexCount, const Vec3i* indices, int indexCount);
 	void setTexCoords(const Vec2f* uvs, int count);
+	// Skinning data (src/scene/Mesh.cpp:43-68): one weight record per vertex; rest poses per bone.
+	void setBoneWeights(const BoneWeights* weights, int count);
+	void setRestposes(const Mat3x4f* restposes, int count);
 	// Builds the BLAS on `stream` if needed (first use); waits for it to finish.
 	void ensureBlas(hipStream_t stream);
 	// BLAS nodes in `format` (built on first use from the binary BLAS; waits for it to finish).
@@ -36,13 +64,15 @@ struct Mesh
 	int triCount = 0;
 	bool hasUVs = false;
 	DeviceBuffer vertices, indices, uvs;
-	// BLAS
-	bool blasBuilt = false;
-	int blasDepth = 0;
-	DeviceBuffer nodes, tris, leafPrim, parentSlot, counters, height, boxes, scratch;
-	DeviceBuffer wide[kBvhFormats]; // 4-wide BLAS per format (index kBvh4F, kBvh4Q)
-	bool wideBuilt[kBvhFormats] = {};
+	Blas blas;
+	DeviceBuffer boneWeights; // BoneWeights per vertex
+	bool hasBoneWeights = false;
+	int maxBoneIndex = -1;
+	std::vector<Mat3x4f> restposes;
 };
+
+// Estimated device bytes of a BLAS over `tris` triangles in `format`.
+int64_t estimatedBlasBytes(int64_t tris, int format);
 
 struct Texture
 {
@@ -56,11 +86,36 @@ struct Time
 	uint64_t ns = 0;
 };
 
+// Vertices of an animated entity (src/scene/animator/ExternalAnimator.cpp, SkeletonAnimator.cpp): the current
+// vertices, their displacement since the previous animation step, and the entity's own BLAS over them.
+struct Animation
+{
+	enum Kind
+	{
+		External,
+		Skeleton,
+	};
+	Kind kind;
+	DeviceBuffer vertices, displacement, staging, matrices;
+	Blas blas;
+	bool blasStale = true;
+};
+
 struct Entity
 {
 	Entity(std::shared_ptr<Mesh> m, Scene* owner) : mesh(std::move(m)), scene(owner) {}
 	void setTransform(const Mat3x4f& tf);
 	std::optional<Mat3x4f> previousFrameTransform() const;
+	// Replaces the entity's vertices with `vertices` (object space, one per mesh vertex).
+	void applyExternalAnimation(const Vec3f* vertices, int count);
+	// Skins the mesh's rest vertices with `pose[b] * restpose[b]` per bone (src/gpu/sceneKernels.cu:18-37).
+	void setPoseAndAnimate(const Mat3x4f* pose, int bones);
+	// Per-vertex displacement of the last animation step, if that step happened at the scene's previous time
+	// (src/scene/Entity.cpp:103-117); nullptr otherwise.
+	const Vec3f* displacementSincePrevFrame() const;
+	// Vertices and BLAS traced for this entity: the animated ones if animated, else the mesh's.
+	const Vec3f* tracedVertices() const;
+	Blas& tracedBlas(hipStream_t stream);
 
 	std::shared_ptr<Mesh> mesh;
 	Scene* scene;
@@ -73,6 +128,12 @@ struct Entity
 	int32_t id = RGL_DEFAULT_ENTITY_ID;
 	float laserRetro = 0.0f;
 	uint64_t order = 0;                          // creation order: instance index order in the TLAS
+	std::unique_ptr<Animation> animation;
+	std::optional<uint64_t> animationTime, formerAnimationTime;
+
+private:
+	Animation& animationOf(Animation::Kind kind);
+	void animated();
 };
 
 // How the TLAS follows entity motion when no entity was added or removed.
@@ -124,8 +185,11 @@ public:
 	const std::vector<std::shared_ptr<Entity>>& getEntities() const { return entities; }
 	// Distinct meshes of the scene's entities.
 	std::vector<Mesh*> meshes() const;
-	// Device bytes of the scene's acceleration structures (BLASes of its meshes plus TLAS), built or estimated.
+	// Device bytes of the scene's acceleration structures (BLASes of its meshes and animated entities plus TLAS),
+	// built or estimated.
 	int64_t accelBytes(int format) const;
+	// Frees the BLASes of the scene's animated entities (rebuilt on next use).
+	void releaseEntityAccel();
 	std::vector<int32_t> instanceTriCounts() const;
 
 	TlasUpdate tlasUpdate = TlasUpdate::Rebuild;

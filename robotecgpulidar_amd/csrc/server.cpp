@@ -157,6 +157,7 @@ int Server::admitScenes(const Request& r, const std::set<Scene*>& pinned)
 						m->releaseAccel();
 					}
 				}
+				v->releaseEntityAccel();
 				residentBytes -= resident[v];
 				resident.erase(v);
 				lru.remove(v);

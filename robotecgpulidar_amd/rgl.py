@@ -298,6 +298,34 @@ def entity_set_transform(entity, m) -> None:
     check("rgl_entity_set_transform", entity, _ptr(a))
 
 
+def entity_apply_external_animation(entity, vertices) -> None:
+    """Replaces the entity's object-space vertices (one per mesh vertex)."""
+    v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(-1, 3))
+    check("rgl_entity_apply_external_animation", entity, _ptr(v), len(v))
+
+
+def mesh_set_bone_weights(mesh, weights, bones) -> None:
+    """Per-vertex skinning weights (N, 4) float32 and bone indices (N, 4) int32."""
+    w = np.asarray(weights, np.float32).reshape(-1, 4)
+    b = np.asarray(bones, np.int32).reshape(-1, 4)
+    rec = np.ascontiguousarray(np.concatenate([w.view(np.uint32), b.view(np.uint32)], axis=1))
+    check("rgl_mesh_set_bone_weights", mesh, _ptr(rec), len(rec))
+
+
+def mesh_set_restposes(mesh, poses) -> None:
+    a = np.ascontiguousarray(np.asarray(poses, np.float32).reshape(-1, 3, 4))
+    check("rgl_mesh_set_restposes", mesh, _ptr(a), len(a))
+
+
+def entity_set_pose_world(entity, poses) -> None:
+    a = np.ascontiguousarray(np.asarray(poses, np.float32).reshape(-1, 3, 4))
+    check("rgl_entity_set_pose_world", entity, _ptr(a), len(a))
+
+
+def scene_set_time(scene, nanoseconds: int) -> None:
+    check("rgl_scene_set_time", scene, C.c_uint64(int(nanoseconds)))
+
+
 def entity_set_id(entity, i: int) -> None:
     check("rgl_entity_set_id", entity, int(i))
 

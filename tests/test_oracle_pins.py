@@ -95,3 +95,38 @@ def test_planted_mistake_is_caught(oracle):
     planted = r[DIST].copy()
     planted[0] = np.nextafter(np.float32(9.0), np.float32(10.0))
     assert planted[0] != 9.0
+
+
+def test_vertex_displacement_velocity_closed_form(oracle):
+    """Vertex-animation velocity (src/gpu/optixPrograms.cu:186-196): for per-vertex displacements of an affine field
+    d(v) = A v + b, barycentric interpolation is exact, so a hit at object point p moves at M3 (A p + b) / dt, where
+    M3 is the linear part of the entity transform (scale included). Catches a wrong vertex index, a missing rotation
+    or an interpolation weight on the wrong vertex."""
+    rng = np.random.default_rng(4)
+    tf = S.trs((1.0, 2.0, 3.0), (10.0, 20.0, 30.0), (2.0, 1.5, 1.0))
+    spec = cube_scene(tf)
+    A = rng.uniform(-0.2, 0.2, (3, 3))
+    b = rng.uniform(-0.1, 0.1, 3)
+    disp = S.CUBE_VERTICES.astype(np.float64) @ A.T + b
+    dirs = rng.normal(size=(64, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = np.zeros((64, 3, 4), np.float32)
+    for k, d in enumerate(dirs):  # rays from the entity's centre; local +Z of the ray frame along d
+        z = d
+        x = np.cross(z, [0.0, 0.0, 1.0] if abs(z[2]) < 0.9 else [1.0, 0.0, 0.0])
+        x /= np.linalg.norm(x)
+        y = np.cross(z, x)
+        rays[k, :, :3] = np.stack([x, y, z], axis=1)
+        rays[k, :, 3] = (1.0, 2.0, 3.0)
+    dt = 0.1
+    fields = [1, IS_HIT, 15]
+    orc = oracle.raytrace(spec, rays, scene_dt=dt, prev_transforms=np.stack([tf]), displacements={0: disp},
+                          fields=fields)
+    assert orc[IS_HIT].all()
+    m3, t = np.asarray(tf, np.float64)[:, :3], np.asarray(tf, np.float64)[:, 3]
+    p = np.linalg.solve(m3, (orc[1].astype(np.float64) - t).T).T
+    expected = (p @ A.T + b) @ m3.T / dt
+    np.testing.assert_allclose(orc[15], expected, atol=2e-5)
+    # Without displacements (and an unchanged transform) the hit points do not move.
+    still = oracle.raytrace(spec, rays, scene_dt=dt, prev_transforms=np.stack([tf]), fields=fields)
+    assert np.abs(still[15]).max() < 1e-5
