@@ -20,6 +20,7 @@
 #include <cstring>
 
 #include "graph.hpp"
+#include "hostcopy.hpp"
 #include "server.hpp"
 #include "scene.hpp"
 
@@ -1013,9 +1014,9 @@ RGL_API rgl_status_t rgl_graph_get_result_data(rgl_node_t node, rgl_field_t fiel
 			auto yield = std::dynamic_pointer_cast<YieldPointsNode>(pn);
 			const void* prefetched = yield ? yield->hostData(field) : nullptr;
 			if (prefetched != nullptr) {
-				std::memcpy(dst, prefetched, bytes); // written by the run's own copy kernel
+				hostCopy(dst, prefetched, bytes); // written by the run's own copy kernel
 			} else {
-				RGL_HIP(hipMemcpy(dst, pn->fieldData(field), bytes, hipMemcpyDeviceToHost));
+				deviceToHost(dst, pn->fieldData(field), bytes);
 			}
 		}
 	});
