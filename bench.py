@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -504,6 +505,83 @@ def run_multireturn(args, moving_fraction: float = 0.2, divergence: float = 0.13
     return out
 
 
+RADAR_SCOPES = [(0.0, 50.0, 0.5, 0.5, math.radians(1.0)), (50.0, 200.0, 1.0, 1.0, math.radians(1.5))]
+
+
+def build_radar_post_graph(lidar: S.Lidar, scene, seed: int):
+    """rays -> pose -> range -> raytrace(scene, sensor moving at 10 m/s) -> compact -> radar postprocess (the
+    reference's clustering, RCS and link budget; 77 GHz, 31 dBm, 27 dBi) -> yield."""
+    n = rgl.Node
+    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
+    pose = rgl.node_rays_transform(n(), lidar.pose)
+    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
+    rt = rgl.node_raytrace(n(), scene)
+    lin, ang = np.float32([10.0, 0.0, 0.0]), np.zeros(3, np.float32)
+    rgl.check("rgl_node_raytrace_configure_velocity", rt.h, lin.ctypes.data, ang.ctypes.data)
+    comp = rgl.node_points_compact_by_field(n(), rgl.IS_HIT_I32)
+    radar = rgl.node_points_radar_postprocess(n(), RADAR_SCOPES, math.radians(120.0 / 256), math.radians(20.0 / 63),
+                                              77e9, 31.0, 27.0, -90.0, 1.0)
+    rgl.check("rgl_node_gaussian_noise_set_seed", radar.h, C.c_uint64(seed))
+    out = rgl.node_points_yield(n(), [rgl.XYZ_VEC3_F32, rgl.DISTANCE_F32, rgl.RCS_F32, rgl.POWER_F32, rgl.SNR_F32])
+    rgl.chain(rays, pose, rng, rt, comp, radar, out)
+    return rays, comp, out
+
+
+def run_radar(args, radars: int = 8):
+    """Radar postprocessing with the reference's semantics: 8 radars of 256 x 64 rays (+-60 deg x +-10 deg) on the
+    urban scene, moving entities so radial speeds are defined, one frame of all radars per step."""
+    spec = S.config2_scene()
+    sc = rgl.scene_create()
+    _, ents, _ = S.load_into_rgl(spec, scene=sc)
+    rng = np.random.default_rng(0xC0F5)
+    sensors = []
+    for k in range(radars):
+        yaw = 360.0 * k / radars
+        pose = S.f32(S.trs((rng.uniform(-40, 40), rng.uniform(-40, 40), 1.2), (0.0, 0.0, yaw)))
+        sensors.append(S.Lidar(S.radar_rays(256, 64), pose, (0.5, 200.0)))
+    graphs = [build_radar_post_graph(x, sc, 0xAADA0 + k) for k, x in enumerate(sensors)]
+    entries = [g[0] for g in graphs]
+    movers = list(range(1, len(ents), 7))
+    t_ns = 0
+
+    def move():
+        nonlocal t_ns
+        t_ns += 50_000_000
+        rgl.check("rgl_scene_set_time", sc, C.c_uint64(t_ns))
+        for j in movers:
+            m = spec.instances[j].transform.astype(np.float64).copy()
+            m[0, 3] += 0.5 * (t_ns / 50_000_000) * (1 if j % 2 else -1)
+            rgl.entity_set_transform(ents[j], S.f32(m))
+
+    def frame(batched):
+        if batched:
+            rgl.graph_run_batch(entries)
+        else:
+            for e in entries:
+                rgl.check("rgl_graph_run", e.h)
+        clusters = sum(kept_count(g[2]) for g in graphs)
+        detections = sum(kept_count(g[1]) for g in graphs)
+        return clusters, detections
+
+    out = {"radars": radars, "rays_per_radar": 256 * 64, "triangles": spec.triangle_count,
+           "scopes": [list(map(float, s)) for s in RADAR_SCOPES]}
+    for mode in ("batched", "plain"):
+        for _ in range(args.warmup):
+            move()
+            frame(mode == "batched")
+        lat = []
+        for _ in range(args.steps):
+            move()
+            f0 = time.perf_counter()
+            clusters, detections = frame(mode == "batched")
+            lat.append(time.perf_counter() - f0)
+        p50, p99 = latency_stats(lat)
+        out[mode] = {"p50_ms": p50, "p99_ms": p99, "frames_per_s": round(len(lat) / sum(lat), 2),
+                     "mrays_per_s": round(radars * 256 * 64 * len(lat) / sum(lat) / 1e6, 3),
+                     "detections_per_step": detections, "clusters_per_step": clusters}
+    return out
+
+
 def run_urban(args):
     spec, lidar = S.config2_scene(), S.config2_lidar()
     S.load_into_rgl(spec)
@@ -563,7 +641,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=8, help="untimed frames (the first six also settle the autotuner)")
-    ap.add_argument("--workload", choices=["batch64", "urban", "serve", "multireturn"], default="batch64")
+    ap.add_argument("--workload", choices=["batch64", "urban", "serve", "multireturn", "radar"], default="batch64")
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the serving-loop measurement")
     ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",
                     help="node format: binary FP32, 4-wide FP32, or 4-wide 8-bit quantized")
@@ -589,8 +667,9 @@ def main():
     rgl.check("rgl_configure_logging", 6, None, False)
     fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8}[args.bvh_format]
     rgl.set_bvh_format(fmt)
-    if args.workload in ("serve", "multireturn"):
-        print(json.dumps(run_serve(args) if args.workload == "serve" else run_multireturn(args)))
+    if args.workload in ("serve", "multireturn", "radar"):
+        run = {"serve": run_serve, "multireturn": run_multireturn, "radar": run_radar}[args.workload]
+        print(json.dumps(run(args)))
         rgl.cleanup()
         return
     result = run_batch64(args) if args.workload == "batch64" else run_urban(args)

@@ -310,7 +310,7 @@ RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const r
                                                        int32_t radar_scopes_count, float ray_azimuth_step,
                                                        float ray_elevation_step, float frequency, float power_transmitted,
                                                        float cumulative_device_gain, float received_noise_mean,
-                                                       float received_noise_st_dev); // core.h:1020 (not implemented)
+                                                       float received_noise_st_dev); // core.h:1020
 RGL_API rgl_status_t rgl_node_points_radar_track_objects(rgl_node_t* node, float object_distance_threshold,
                                                          float object_azimuth_threshold, float object_elevation_threshold,
                                                          float object_radial_speed_threshold, float max_matching_distance,

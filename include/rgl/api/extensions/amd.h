@@ -9,6 +9,8 @@
 
 // Sets the seed of a Gaussian noise node's stream and restarts its run counter. The noise drawn for element i in
 // run r is a pure function of (seed, r, node kind, i), so the same seed replays the same noise on CPU and GPU.
+// Radar postprocess nodes (rgl_node_points_radar_postprocess) draw their per-cluster noise from such a stream too
+// and accept both calls.
 RGL_API rgl_status_t rgl_node_gaussian_noise_set_seed(rgl_node_t node, uint64_t seed);
 
 // Reports the stream identity (seed, next run index, node-kind tag) of a Gaussian noise node.

@@ -821,7 +821,38 @@ RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const r
                                                        float cumulative_device_gain, float received_noise_mean,
                                                        float received_noise_st_dev)
 {
-	return safeCall([&]() { throw NotImplementedCall("radar postprocessing is not implemented in this library yet"); });
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(radar_scopes != nullptr);
+		CHECK_ARG(radar_scopes_count > 0);
+		CHECK_ARG(ray_azimuth_step > 0);
+		CHECK_ARG(ray_elevation_step > 0);
+		CHECK_ARG(frequency > 0);
+		CHECK_ARG(power_transmitted > 0);
+		CHECK_ARG(cumulative_device_gain > 0);
+		CHECK_ARG(received_noise_st_dev > 0);
+		for (int i = 0; i < radar_scopes_count; ++i) {
+			CHECK_ARG(radar_scopes[i].begin_distance >= 0);
+			CHECK_ARG(radar_scopes[i].end_distance >= 0);
+			CHECK_ARG(radar_scopes[i].distance_separation_threshold >= 0);
+			CHECK_ARG(radar_scopes[i].radial_speed_separation_threshold >= 0);
+			CHECK_ARG(radar_scopes[i].azimuth_separation_threshold >= 0);
+			CHECK_ARG(radar_scopes[i].end_distance >= radar_scopes[i].begin_distance);
+		}
+		std::vector<RadarScope> scopes;
+		for (int i = 0; i < radar_scopes_count; ++i) {
+			const rgl_radar_scope_t& r = radar_scopes[i];
+			scopes.push_back(RadarScope{r.begin_distance, r.end_distance, r.distance_separation_threshold,
+			                            r.radial_speed_separation_threshold, r.azimuth_separation_threshold});
+		}
+		createOrUpdateNode<RadarPostprocessPointsNode>(node, "RadarPostprocessPointsNode", false,
+		                                               [&](RadarPostprocessPointsNode& n) {
+			                                               n.setParameters(scopes, ray_azimuth_step, ray_elevation_step,
+			                                                               frequency, power_transmitted,
+			                                                               cumulative_device_gain, received_noise_mean,
+			                                                               received_noise_st_dev);
+		                                               });
+	});
 }
 
 RGL_API rgl_status_t rgl_node_points_radar_track_objects(rgl_node_t* node, float object_distance_threshold,
@@ -1074,6 +1105,8 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_set_seed(rgl_node_t node, uint64_t 
 			s = &b->rng;
 		} else if (auto c = std::dynamic_pointer_cast<GaussianNoiseDistanceNode>(n)) {
 			s = &c->rng;
+		} else if (auto d = std::dynamic_pointer_cast<RadarPostprocessPointsNode>(n)) {
+			s = &d->rng;
 		}
 		if (s == nullptr) {
 			throw InvalidAPIObject("RGL API Error: Node type mismatch: expected a Gaussian noise node, got " + n->getName());
@@ -1097,6 +1130,8 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_get_stream(rgl_node_t node, uint64_
 			s = &b->rng;
 		} else if (auto c = std::dynamic_pointer_cast<GaussianNoiseDistanceNode>(n)) {
 			s = &c->rng;
+		} else if (auto d = std::dynamic_pointer_cast<RadarPostprocessPointsNode>(n)) {
+			s = &d->rng;
 		}
 		if (s == nullptr) {
 			throw InvalidAPIObject("RGL API Error: Node type mismatch: expected a Gaussian noise node, got " + n->getName());

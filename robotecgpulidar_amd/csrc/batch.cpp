@@ -36,9 +36,12 @@ struct GraphBatch::Plan
 		std::vector<CompactItem> cp;
 		std::vector<int> cpList;
 		std::vector<CompactByFieldPointsNode*> cpNodes;
+		std::vector<RadarItem> rp;
+		std::vector<int> rpList;
+		std::vector<RadarPostprocessPointsNode*> rpNodes;
 		std::vector<Node*> other;
-		int rtBlocks = 0, rtmBlocks = 0, opBlocks = 0, cpTiles = 0;
-		size_t rtOff = 0, rtmOff = 0, opOff = 0, cpOff = 0;
+		int rtBlocks = 0, rtmBlocks = 0, opBlocks = 0, cpTiles = 0, rpBlocks = 0;
+		size_t rtOff = 0, rtmOff = 0, opOff = 0, cpOff = 0, rpOff = 0;
 		void clear()
 		{
 			rt.clear();
@@ -50,12 +53,15 @@ struct GraphBatch::Plan
 			cp.clear();
 			cpList.clear();
 			cpNodes.clear();
+			rp.clear();
+			rpList.clear();
+			rpNodes.clear();
 			other.clear();
-			rtBlocks = rtmBlocks = opBlocks = cpTiles = 0;
+			rtBlocks = rtmBlocks = opBlocks = cpTiles = rpBlocks = 0;
 		}
 	};
 	std::vector<Level> levels; // steps, in launch order
-	std::vector<Node*> readyOp, readyRt, readyCp, passive;
+	std::vector<Node*> readyOp, readyRt, readyCp, readyRp, passive;
 	std::vector<FieldCopyList> lists;
 	std::vector<const int32_t*> gather;
 	std::vector<std::shared_ptr<GraphRunCtx>> ctxs;
@@ -329,6 +335,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		pl.readyOp.clear();
 		pl.readyRt.clear();
 		pl.readyCp.clear();
+		pl.readyRp.clear();
 		pl.passive.clear();
 		int steps = 0;
 		auto newStep = [&]() -> Plan::Level& {
@@ -346,6 +353,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				case BatchStage::PointOp: pl.readyOp.push_back(n); break;
 				case BatchStage::Compact: pl.readyCp.push_back(n); break;
 				case BatchStage::Raytrace: pl.readyRt.push_back(n); break;
+				case BatchStage::RadarPost: pl.readyRp.push_back(n); break;
 			}
 		};
 		auto release = [&](Node* n) {
@@ -379,6 +387,8 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				group.swap(pl.readyOp);
 			} else if (!pl.readyCp.empty()) {
 				group.swap(pl.readyCp);
+			} else if (!pl.readyRp.empty()) {
+				group.swap(pl.readyRp);
 			} else if (!pl.readyRt.empty()) {
 				group.swap(pl.readyRt);
 			} else {
@@ -434,6 +444,18 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						pl.lists.push_back(list);
 						break;
 					}
+					case BatchStage::RadarPost: {
+						auto* rn = dynamic_cast<RadarPostprocessPointsNode*>(n);
+						FieldCopyList list;
+						RadarItem it = rn->radarItem(list);
+						it.firstBlock = L.rpBlocks;
+						L.rpBlocks += gpu::radarBlocks(it.capacity);
+						L.rp.push_back(it);
+						L.rpList.push_back(static_cast<int>(pl.lists.size()));
+						L.rpNodes.push_back(rn);
+						pl.lists.push_back(list);
+						break;
+					}
 					default: throw std::logic_error("batch: unexpected stage kind");
 				}
 			}
@@ -448,10 +470,17 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		}
 		const int maxLevel = steps - 1;
 
-		// Host slots: kept counts of compaction nodes, then overflow flags of raytrace nodes.
+		// Host slots: kept counts of compaction nodes, cluster counts and error bits of radar postprocess nodes, then
+		// overflow flags of raytrace nodes.
 		for (int l = 0; l <= maxLevel; ++l) {
 			for (auto& it : pl.levels[l].cp) {
 				pl.gather.push_back(it.countOut);
+			}
+		}
+		for (int l = 0; l <= maxLevel; ++l) {
+			for (auto& it : pl.levels[l].rp) {
+				pl.gather.push_back(it.info);
+				pl.gather.push_back(it.info + 1);
 			}
 		}
 		for (int l = 0; l <= maxLevel; ++l) {
@@ -478,6 +507,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			L.rtmOff = reserve(sizeof(RaytraceBatchItem) * L.rtm.size());
 			L.opOff = reserve(sizeof(PointOpItem) * L.op.size());
 			L.cpOff = reserve(sizeof(CompactItem) * L.cp.size());
+			L.rpOff = reserve(sizeof(RadarItem) * L.rp.size());
 		}
 		const size_t total = std::max<size_t>(off, 1);
 		if (total > items.capacity()) {
@@ -502,6 +532,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			for (size_t i = 0; i < L.cp.size(); ++i) {
 				L.cp[i].fields = devLists + L.cpList[i];
 			}
+			for (size_t i = 0; i < L.rp.size(); ++i) {
+				L.rp[i].gather = devLists + L.rpList[i];
+			}
 			if (!L.rt.empty()) {
 				std::memcpy(host + L.rtOff, L.rt.data(), sizeof(RaytraceBatchItem) * L.rt.size());
 			}
@@ -513,6 +546,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			}
 			if (!L.cp.empty()) {
 				std::memcpy(host + L.cpOff, L.cp.data(), sizeof(CompactItem) * L.cp.size());
+			}
+			if (!L.rp.empty()) {
+				std::memcpy(host + L.rpOff, L.rp.data(), sizeof(RadarItem) * L.rp.size());
 			}
 		}
 		// Kept counts and overflow flags of every graph come back through one gather and one copy.
@@ -612,6 +648,11 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						pending.push_back(pt);
 					}
 				}
+				if (!L.rp.empty()) {
+					gpu::radarPostprocessBatch(s, reinterpret_cast<const RadarItem*>(dev + L.rpOff),
+					                           static_cast<int>(L.rp.size()), L.rpBlocks);
+					RGL_HIP(hipGetLastError());
+				}
 				for (Node* n : L.other) {
 					n->enqueue();
 				}
@@ -638,7 +679,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				                  static_cast<int64_t>(L.rtmBlocks), static_cast<int64_t>(L.rtmOff),
 				                  static_cast<int64_t>(L.op.size()), static_cast<int64_t>(L.opBlocks),
 				                  static_cast<int64_t>(L.opOff), static_cast<int64_t>(L.cp.size()),
-				                  static_cast<int64_t>(L.cpTiles), static_cast<int64_t>(L.cpOff)}) {
+				                  static_cast<int64_t>(L.cpTiles), static_cast<int64_t>(L.cpOff),
+				                  static_cast<int64_t>(L.rp.size()), static_cast<int64_t>(L.rpBlocks),
+				                  static_cast<int64_t>(L.rpOff)}) {
 					sig.push_back(v);
 				}
 			}
@@ -706,6 +749,12 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			for (int l = 0; l <= maxLevel; ++l) {
 				for (auto* cn : pl.levels[l].cpNodes) {
 					cn->setCountHost(hostBuf, k++);
+				}
+			}
+			for (int l = 0; l <= maxLevel; ++l) {
+				for (auto* rn : pl.levels[l].rpNodes) {
+					rn->setInfoHost(hostBuf, k);
+					k += 2;
 				}
 			}
 			for (int l = 0; l <= maxLevel; ++l) {

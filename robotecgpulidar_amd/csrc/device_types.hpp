@@ -205,6 +205,52 @@ struct RadarBinParams
 	int32_t rangeCells;
 	int32_t azCells;
 };
+// Radar postprocessing with the reference's semantics (src/graph/RadarPostprocessPointsNode.cpp): distance scopes
+// with their separation thresholds (the layout of rgl_radar_scope_t), energy and link-budget parameters, and the
+// node's noise stream.
+struct RadarScope
+{
+	float begin, end, distanceThreshold, radialSpeedThreshold, azimuthThreshold;
+};
+constexpr int kMaxRadarScopes = 16;
+constexpr int kRadarMaxClusters = 1024; // clusters held in LDS by the clustering wave
+struct RadarPostParams
+{
+	RadarScope scopes[kMaxRadarScopes];
+	int32_t scopeCount;
+	float azimuthStep, elevationStep, frequency;
+	float powerTransmitted, deviceGain, noiseMean, noiseStDev;
+	Mat3x4f lookAt; // point frame -> sensor frame
+	uint64_t seed;
+	uint32_t run, tag;
+};
+// One radar postprocess node's launch description (alone or inside a batched launch).
+struct RadarItem
+{
+	RadarPostParams p;
+	const Mat3x4f* rayPose;
+	const float* distance;
+	const float* azimuth;
+	const float* elevation;
+	const float* radialSpeed;
+	const Vec3f* normal;
+	const int32_t* count; // live input count on the device, nullptr = capacity
+	float* terms;         // 4 floats per point
+	int32_t* work;        // 3 ints per point
+	int32_t* centres;
+	float* power;
+	float* rcs;
+	float* noise;
+	float* snr;
+	int32_t* info;                // {cluster count, error bits}
+	const FieldCopyList* gather;  // centre-point fields to gather (device memory)
+	int32_t capacity;
+	int32_t firstBlock;           // energy / gather launches: blocks [firstBlock, firstBlock + blocks(capacity))
+};
+// Error bits reported by the clustering kernel.
+constexpr int32_t kRadarErrRcsNan = 1;
+constexpr int32_t kRadarErrTooManyClusters = 2;
+
 // Fixed-point scale of the RCS sums: contributions are rounded to multiples of 2^-24 m^2 and summed as 64-bit
 // integers, so a cell's sum does not depend on the order in which hits are added.
 constexpr double kRcsFixedScale = 16777216.0;

@@ -1109,6 +1109,148 @@ const void* RadarBinPointsNode::fieldData(rgl_field_t f)
 	return it->second.get();
 }
 
+// ------------------------------------------------------------------------------------------------ radar postprocess
+namespace {
+const rgl_field_t kRadarPostFields[] = {RGL_FIELD_POWER_F32, RGL_FIELD_RCS_F32, RGL_FIELD_NOISE_F32, RGL_FIELD_SNR_F32};
+bool isRadarPostField(rgl_field_t f)
+{
+	return std::find(std::begin(kRadarPostFields), std::end(kRadarPostFields), f) != std::end(kRadarPostFields);
+}
+} // namespace
+
+void RadarPostprocessPointsNode::setParameters(const std::vector<RadarScope>& scopes, float azimuthStep,
+                                               float elevationStep, float frequency, float powerTransmitted,
+                                               float deviceGain, float noiseMean, float noiseStDev)
+{
+	if (scopes.size() > static_cast<size_t>(kMaxRadarScopes)) {
+		throw std::invalid_argument("at most " + std::to_string(kMaxRadarScopes) + " radar scopes are supported");
+	}
+	RadarPostParams p{};
+	for (size_t i = 0; i < scopes.size(); ++i) {
+		p.scopes[i] = scopes[i];
+	}
+	p.scopeCount = static_cast<int32_t>(scopes.size());
+	p.azimuthStep = azimuthStep;
+	p.elevationStep = elevationStep;
+	p.frequency = frequency;
+	p.powerTransmitted = powerTransmitted;
+	p.deviceGain = deviceGain;
+	p.noiseMean = noiseMean;
+	p.noiseStDev = noiseStDev;
+	params = p;
+}
+
+// src/graph/RadarPostprocessPointsNode.cpp:37-49
+void RadarPostprocessPointsNode::validateImpl()
+{
+	IPointsNodeSingleInput::validateImpl();
+	if (!input->isDense()) {
+		throw InvalidPipeline("RadarComputeEnergyPointsNode requires dense input");
+	}
+}
+
+bool RadarPostprocessPointsNode::hasField(rgl_field_t f) const { return isRadarPostField(f) || input->hasField(f); }
+
+std::vector<rgl_field_t> RadarPostprocessPointsNode::fieldList() const
+{
+	std::vector<rgl_field_t> out = input->fieldList();
+	for (rgl_field_t f : kRadarPostFields) {
+		if (std::find(out.begin(), out.end(), f) == out.end()) {
+			out.push_back(f);
+		}
+	}
+	return out;
+}
+
+RadarItem RadarPostprocessPointsNode::radarItem(FieldCopyList& list)
+{
+	const int cap = input->capacity();
+	const size_t n = static_cast<size_t>(std::max(cap, 1));
+	RadarItem it{};
+	it.p = params;
+	it.p.lookAt = input->lookAtOriginTransform();
+	it.p.seed = rng.seed;
+	it.p.run = rng.run++;
+	it.p.tag = rng.tag;
+	it.rayPose = static_cast<const Mat3x4f*>(input->fieldData(RGL_FIELD_RAY_POSE_MAT3x4_F32));
+	it.distance = static_cast<const float*>(input->fieldData(RGL_FIELD_DISTANCE_F32));
+	it.azimuth = static_cast<const float*>(input->fieldData(RGL_FIELD_AZIMUTH_F32));
+	it.elevation = static_cast<const float*>(input->fieldData(RGL_FIELD_ELEVATION_F32));
+	it.radialSpeed = static_cast<const float*>(input->fieldData(RGL_FIELD_RADIAL_SPEED_F32));
+	it.normal = static_cast<const Vec3f*>(input->fieldData(RGL_FIELD_NORMAL_VEC3_F32));
+	it.count = input->countDevice();
+	it.terms = static_cast<float*>(terms.ensure(16 * n));
+	it.work = work.ensureT<int32_t>(3 * n);
+	it.centres = centres.ensureT<int32_t>(n);
+	it.power = power.ensureT<float>(n);
+	it.rcs = rcs.ensureT<float>(n);
+	it.noise = noise.ensureT<float>(n);
+	it.snr = snr.ensureT<float>(n);
+	it.info = info.ensureT<int32_t>(2);
+	it.capacity = cap;
+	// The centre point's fields of every cluster (src/graph/RadarPostprocessPointsNode.cpp:215-257 gathers them on
+	// request; here every input field is gathered with the clusters).
+	list = FieldCopyList{};
+	for (rgl_field_t f : input->fieldList()) {
+		if (isRadarPostField(f) || isPadding(f)) {
+			continue;
+		}
+		if (list.count == kMaxCompactFields) {
+			throw InvalidPipeline("RadarPostprocessPointsNode: too many input fields");
+		}
+		const size_t sz = fieldSizeChecked(f);
+		list.f[list.count++] = FieldCopy{static_cast<const char*>(input->fieldData(f)),
+		                                 static_cast<char*>(gathered[f].ensure(sz * n)), static_cast<int32_t>(sz), 0};
+	}
+	infoSlots.reset();
+	return it;
+}
+
+void RadarPostprocessPointsNode::enqueueImpl()
+{
+	FieldCopyList list{};
+	RadarItem it = radarItem(list);
+	// From pageable memory: the copy has read `list` when the call returns.
+	RGL_HIP(hipMemcpyAsync(gatherList.ensure(sizeof(FieldCopyList)), &list, sizeof(FieldCopyList),
+	                       hipMemcpyHostToDevice, stream()));
+	it.gather = gatherList.as<FieldCopyList>();
+	gpu::radarPostprocess(stream(), it);
+	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipMemcpyAsync(infoHost.ensure(2 * sizeof(int32_t)), info.get(), 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+	                       stream()));
+}
+
+int RadarPostprocessPointsNode::hostCount()
+{
+	waitForResults();
+	const auto* h = infoSlots ? static_cast<const int32_t*>(infoSlots->get()) + infoSlot
+	                          : static_cast<const int32_t*>(infoHost.get());
+	if (h[1] & kRadarErrRcsNan) {
+		throw InvalidPipeline("RCS is NaN");
+	}
+	if (h[1] & kRadarErrTooManyClusters) {
+		throw InvalidPipeline("radar postprocessing supports at most " + std::to_string(kRadarMaxClusters) +
+		                      " clusters per frame");
+	}
+	return h[0];
+}
+
+const void* RadarPostprocessPointsNode::fieldData(rgl_field_t f)
+{
+	switch (f) {
+		case RGL_FIELD_POWER_F32: return power.get();
+		case RGL_FIELD_RCS_F32: return rcs.get();
+		case RGL_FIELD_NOISE_F32: return noise.get();
+		case RGL_FIELD_SNR_F32: return snr.get();
+		default: break;
+	}
+	auto it = gathered.find(f);
+	if (it == gathered.end()) {
+		throw InvalidPipeline("RadarPostprocessPointsNode does not provide field " + fieldName(f));
+	}
+	return it->second.get();
+}
+
 // src/graph/FromArrayPointsNode.cpp:18-46
 void FromArrayPointsNode::setParameters(const void* points, int n, const std::vector<rgl_field_t>& f)
 {

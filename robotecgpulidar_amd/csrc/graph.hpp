@@ -30,6 +30,7 @@ enum class BatchStage
 {
 	Passive,  // nothing to launch (configuration or view nodes)
 	Raytrace, // one item of the batched raytrace launch
+	RadarPost, // one item of the batched radar postprocess launches
 	PointOp,  // one item of the batched elementwise launch
 	Compact,  // one item of the batched compaction launches
 	Other,    // enqueued on its own, on the batch stream
@@ -655,6 +656,52 @@ private:
 	DeviceBuffer sums, counts, keep, count, tileCounts;
 	std::map<rgl_field_t, DeviceBuffer> dense, out;
 	PinnedBuffer countHost;
+};
+
+// Radar detections with the reference's semantics (src/graph/RadarPostprocessPointsNode.cpp): the dense input
+// points are clustered by distance scope and separation thresholds; one output point per cluster, carrying the
+// fields of the cluster's directional centre point plus RCS (dBsm), POWER (dBm, with noise), NOISE and SNR (dB).
+// The noise is this library's seeded stream (tag 6), not the reference's unseeded generator.
+struct RadarPostprocessPointsNode : IPointsNodeSingleInput
+{
+	RadarPostprocessPointsNode() { rng.init(6); }
+	void setParameters(const std::vector<RadarScope>& scopes, float azimuthStep, float elevationStep, float frequency,
+	                   float powerTransmitted, float deviceGain, float noiseMean, float noiseStDev);
+	std::string getName() const override { return "RadarPostprocessPointsNode"; }
+	std::vector<rgl_field_t> requiredFields() const override
+	{
+		return {RGL_FIELD_DISTANCE_F32, RGL_FIELD_AZIMUTH_F32,     RGL_FIELD_ELEVATION_F32,  RGL_FIELD_RADIAL_SPEED_F32,
+		        RGL_FIELD_RAY_POSE_MAT3x4_F32, RGL_FIELD_NORMAL_VEC3_F32, RGL_FIELD_XYZ_VEC3_F32};
+	}
+	bool isDense() const override { return true; }
+	bool hasField(rgl_field_t f) const override;
+	std::vector<rgl_field_t> fieldList() const override;
+	const int32_t* countDevice() const override { return info.as<int32_t>(); }
+	int hostCount() override;
+	const void* fieldData(rgl_field_t f) override;
+	BatchStage batchStage() const override { return BatchStage::RadarPost; }
+	// Launch description of this run (advances the noise stream); `gather` receives the centre-point field list,
+	// which it.gather must point to in device memory.
+	RadarItem radarItem(FieldCopyList& gather);
+	// Batched runs: the cluster count and error bits come back in a batch's host slots (index, index + 1).
+	void setInfoHost(std::shared_ptr<PinnedBuffer> slots, int index)
+	{
+		infoSlots = std::move(slots);
+		infoSlot = index;
+	}
+	NoiseStream rng;
+
+protected:
+	void validateImpl() override;
+	void enqueueImpl() override;
+
+private:
+	RadarPostParams params{};
+	DeviceBuffer terms, work, centres, power, rcs, noise, snr, info, gatherList;
+	std::map<rgl_field_t, DeviceBuffer> gathered;
+	PinnedBuffer infoHost;
+	std::shared_ptr<PinnedBuffer> infoSlots;
+	int infoSlot = 0;
 };
 
 struct FromArrayPointsNode : IPointsNode

@@ -95,5 +95,14 @@ void radarBin(hipStream_t stream, int capacity, const int32_t* count, const Vec3
               unsigned long long* sums, uint32_t* counts, Vec3f* cellXyz, float* cellDistance, float* cellAzimuth,
               float* cellRcs, int32_t* cellHit, int32_t* keep);
 
+// Radar postprocessing (src/gpu/nodeKernels.cu:82-101,209-320; src/graph/RadarPostprocessPointsNode.cpp:74-174):
+// per-point scattered-field terms, then one wave per node clusters the points in order, merges clusters, and writes
+// per cluster its centre point, RCS, noise, power and SNR (info = {cluster count, error bits}); then the centre
+// points' fields are gathered. `it.gather` must point to device memory. The batched form runs several nodes in
+// one set of launches (items in device memory, energy and gather blocks laid out by firstBlock).
+int radarBlocks(int capacity);
+void radarPostprocess(hipStream_t stream, const RadarItem& it);
+void radarPostprocessBatch(hipStream_t stream, const RadarItem* items, int count, int totalBlocks);
+
 } // namespace gpu
 } // namespace rgl

@@ -413,6 +413,17 @@ def node_points_radar_bin(node: Node, range_step: float, range_max: float, azimu
     return node
 
 
+def node_points_radar_postprocess(node: Node, scopes, azimuth_step: float, elevation_step: float, frequency: float,
+                                  power_transmitted: float, device_gain: float, noise_mean: float,
+                                  noise_st_dev: float) -> Node:
+    """`scopes`: rows of (begin_distance, end_distance, distance_threshold, radial_speed_threshold,
+    azimuth_threshold), the layout of rgl_radar_scope_t."""
+    sc = np.ascontiguousarray(np.asarray(scopes, np.float32).reshape(-1, 5))
+    check("rgl_node_points_radar_postprocess", node.ref, _ptr(sc), len(sc), azimuth_step, elevation_step, frequency,
+          power_transmitted, device_gain, noise_mean, noise_st_dev)
+    return node
+
+
 def node_points_from_array(node: Node, records: np.ndarray, fields) -> Node:
     f = np.ascontiguousarray(np.asarray(fields, dtype=np.int32))
     buf = np.ascontiguousarray(records)

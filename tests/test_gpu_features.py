@@ -271,7 +271,8 @@ def test_api_error_semantics(rgl_lib):
     r = np.array([[2.0, 1.0]], np.float32)
     assert rgl.call("rgl_node_rays_set_range", rgl.Node().ref, r.ctypes.data, 1) == rgl.RGL_INVALID_ARGUMENT
     assert "minimum range must be lower" in rgl.last_error()
-    assert rgl.call("rgl_node_points_radar_postprocess", rgl.Node().ref, None, 0, *([1.0] * 7)) == rgl.RGL_NOT_IMPLEMENTED
+    assert rgl.call("rgl_node_points_radar_postprocess", rgl.Node().ref, None, 0, *([1.0] * 7)) == rgl.RGL_INVALID_ARGUMENT
+    assert rgl.call("rgl_node_points_radar_track_objects", rgl.Node().ref, *([1.0] * 7)) == rgl.RGL_NOT_IMPLEMENTED
     # Results of a node that never ran.
     y = rgl.node_points_yield(rgl.Node(), [XYZ])
     assert rgl.call("rgl_graph_get_result_size", y.h, XYZ, None, None) == rgl.RGL_INVALID_PIPELINE
