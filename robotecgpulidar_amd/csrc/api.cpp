@@ -7,6 +7,7 @@
 
 #include <rgl/api/core.h>
 #include <rgl/api/extensions/amd.h>
+#include <rgl/api/extensions/tape.h>
 
 #include <cstdio>
 #include <fstream>
@@ -22,6 +23,7 @@
 #include "graph.hpp"
 #include "hostcopy.hpp"
 #include "server.hpp"
+#include "tape.hpp"
 #include "scene.hpp"
 
 using namespace rgl;
@@ -129,6 +131,10 @@ rgl_status_t safeCall(Fn&& fn)
 		fn();
 	} catch (NotImplementedCall& e) {
 		return updateState(RGL_NOT_IMPLEMENTED, e.what());
+	} catch (tape::TapeError& e) {
+		return updateState(RGL_TAPE_ERROR, e.what());
+	} catch (tape::InvalidFilePath& e) {
+		return updateState(RGL_INVALID_FILE_PATH, e.what());
 	} catch (InvalidAPIObject& e) {
 		return updateState(RGL_INVALID_API_OBJECT, e.what());
 	} catch (InvalidPipeline& e) {
@@ -268,9 +274,79 @@ void throwHip(hipError_t e, const char* expr, const char* file, int line)
 } // namespace rgl
 
 // ============================================================================================ general
+// ============================================================================================ tape recording
+// Arguments of a recorded call as YAML scalars (src/tape/TapeRecorder.hpp:69-128): handles by address, out-handles
+// by the handle written, int32_t out-parameters by value, enums as integers, arrays and single structs by their
+// offset in the binary file.
+namespace tapeargs {
+using rgl::tape::handle;
+using rgl::tape::scalar;
+inline std::string A(int32_t v) { return scalar(int64_t{v}); }
+inline std::string A(uint64_t v) { return rgl::tape::scalarU(v); }
+inline std::string A(float v) { return scalar(static_cast<double>(v), true); }
+inline std::string A(bool v) { return scalar(v); }
+inline std::string A(const char* s) { return scalar(s); }
+inline std::string A(const void* p) { return handle(p); }
+inline std::string A(rgl_mesh_t h) { return handle(h); }
+inline std::string A(rgl_entity_t h) { return handle(h); }
+inline std::string A(rgl_texture_t h) { return handle(h); }
+inline std::string A(rgl_node_t h) { return handle(h); }
+inline std::string A(rgl_scene_t h) { return handle(h); }
+template<typename E>
+std::enable_if_t<std::is_enum_v<E>, std::string> A(E e)
+{
+	return scalar(static_cast<int64_t>(e));
+}
+template<typename H>
+std::string D(H* out)
+{
+	return out != nullptr ? A(*out) : std::string("0");
+}
+inline std::string V(const int32_t* p) { return p != nullptr ? A(*p) : std::string("0"); }
+template<typename T>
+std::string R(const T* p, int64_t count)
+{
+	return rgl::tape::active()->bytes(rgl::tape::array(p, count));
+}
+} // namespace tapeargs
+
+// Bytes of one record of `fields` (0 if any field is unknown).
+int64_t recordBytes(const rgl_field_t* fields, int32_t count)
+{
+	int64_t b = 0;
+	for (int32_t i = 0; fields != nullptr && i < count; ++i) {
+		try {
+			b += static_cast<int64_t>(fieldSizeChecked(fields[i]));
+		} catch (...) {
+			return 0;
+		}
+	}
+	return b;
+}
+
+template<typename MakeArgs>
+void tapeRecord(rgl_status_t& status, const char* fn, MakeArgs makeArgs)
+{
+	auto* rec = tape::active();
+	if (rec == nullptr) {
+		return;
+	}
+	try {
+		rec->call(fn, makeArgs());
+	} catch (std::exception& e) {
+		status = updateState(RGL_TAPE_ERROR, e.what());
+	}
+}
+
+#define TAPE_RECORD(fn, ...)                                                                                           \
+	tapeRecord(status, #fn, [&]() {                                                                                    \
+		using namespace tapeargs;                                                                                      \
+		return std::vector<std::string>{__VA_ARGS__};                                                                  \
+	})
+
 RGL_API rgl_status_t rgl_get_version_info(int32_t* out_major, int32_t* out_minor, int32_t* out_patch)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(out_major != nullptr);
 		CHECK_ARG(out_minor != nullptr);
 		CHECK_ARG(out_patch != nullptr);
@@ -278,20 +354,24 @@ RGL_API rgl_status_t rgl_get_version_info(int32_t* out_major, int32_t* out_minor
 		*out_minor = RGL_VERSION_MINOR;
 		*out_patch = RGL_VERSION_PATCH;
 	});
+	TAPE_RECORD(rgl_get_version_info, V(out_major), V(out_minor), V(out_patch));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_get_extension_info(rgl_extension_t extension, int32_t* out_available)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(0 <= extension && extension < RGL_EXTENSION_COUNT);
 		CHECK_ARG(out_available != nullptr);
 		*out_available = 0;
 	});
+	TAPE_RECORD(rgl_get_extension_info, A(extension), V(out_available));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_configure_logging(rgl_log_level_t log_level, const char* log_file_path, bool use_stdout)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(0 <= log_level && log_level <= 6);
 		logLevel = log_level;
 		logStdout = use_stdout;
@@ -304,6 +384,8 @@ RGL_API rgl_status_t rgl_configure_logging(rgl_log_level_t log_level, const char
 			}
 		}
 	});
+	TAPE_RECORD(rgl_configure_logging, A(log_level), A(log_file_path), A(use_stdout));
+	return status;
 }
 
 RGL_API void rgl_get_last_error_string(const char** out_error_string)
@@ -316,7 +398,7 @@ RGL_API void rgl_get_last_error_string(const char** out_error_string)
 
 RGL_API rgl_status_t rgl_cleanup(void)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		for (auto& [h, sv] : Registry<rgl::Server>::map()) {
 			sv->drain();
 		}
@@ -340,13 +422,15 @@ RGL_API rgl_status_t rgl_cleanup(void)
 		Registry<rgl::Scene>::map().clear();
 		rgl::Scene::instance().clear();
 	});
+	TAPE_RECORD(rgl_cleanup);
+	return status;
 }
 
 // ============================================================================================ mesh
 RGL_API rgl_status_t rgl_mesh_create(rgl_mesh_t* out_mesh, const rgl_vec3f* vertices, int32_t vertex_count,
                                      const rgl_vec3i* indices, int32_t index_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(out_mesh != nullptr);
 		CHECK_ARG(vertices != nullptr);
 		CHECK_ARG(vertex_count > 0);
@@ -358,11 +442,13 @@ RGL_API rgl_status_t rgl_mesh_create(rgl_mesh_t* out_mesh, const rgl_vec3f* vert
 		Registry<rgl::Mesh>::add(m);
 		*out_mesh = reinterpret_cast<rgl_mesh_t>(m.get());
 	});
+	TAPE_RECORD(rgl_mesh_create, D(out_mesh), R(vertices, vertex_count), A(vertex_count), R(indices, index_count), A(index_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_mesh_set_texture_coords(rgl_mesh_t mesh, const rgl_vec2f* uvs, int32_t uv_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(mesh != nullptr);
 		CHECK_ARG(uvs != nullptr);
 		CHECK_ARG(uv_count > 0);
@@ -370,12 +456,14 @@ RGL_API rgl_status_t rgl_mesh_set_texture_coords(rgl_mesh_t mesh, const rgl_vec2
 		Registry<rgl::Mesh>::get(mesh, "Mesh")->setTexCoords(reinterpret_cast<const Vec2f*>(uvs), uv_count);
 		rgl::Scene::markAllDirty(false);
 	});
+	TAPE_RECORD(rgl_mesh_set_texture_coords, A(mesh), R(uvs, uv_count), A(uv_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_mesh_set_bone_weights(rgl_mesh_t mesh, const rgl_bone_weights_t* bone_weights,
                                                int32_t bone_weights_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(mesh != nullptr);
 		CHECK_ARG(bone_weights != nullptr);
 		CHECK_ARG(bone_weights_count > 0);
@@ -383,26 +471,32 @@ RGL_API rgl_status_t rgl_mesh_set_bone_weights(rgl_mesh_t mesh, const rgl_bone_w
 		Registry<rgl::Mesh>::get(mesh, "Mesh")->setBoneWeights(reinterpret_cast<const BoneWeights*>(bone_weights),
 		                                                      bone_weights_count);
 	});
+	TAPE_RECORD(rgl_mesh_set_bone_weights, A(mesh), R(bone_weights, bone_weights_count), A(bone_weights_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_mesh_set_restposes(rgl_mesh_t mesh, const rgl_mat3x4f* restposes, int32_t bones_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(mesh != nullptr);
 		CHECK_ARG(restposes != nullptr);
 		CHECK_ARG(bones_count > 0);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Mesh>::get(mesh, "Mesh")->setRestposes(reinterpret_cast<const Mat3x4f*>(restposes), bones_count);
 	});
+	TAPE_RECORD(rgl_mesh_set_restposes, A(mesh), R(restposes, bones_count), A(bones_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_mesh_destroy(rgl_mesh_t mesh)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(mesh != nullptr);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Mesh>::release(mesh, "Mesh");
 	});
+	TAPE_RECORD(rgl_mesh_destroy, A(mesh));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_mesh_is_alive(rgl_mesh_t mesh, bool* out_alive)
@@ -416,7 +510,7 @@ RGL_API rgl_status_t rgl_mesh_is_alive(rgl_mesh_t mesh, bool* out_alive)
 // ============================================================================================ entity
 RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t scene, rgl_mesh_t mesh)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(out_entity != nullptr);
 		CHECK_ARG(mesh != nullptr);
 		GraphRunCtx::synchronizeAll();
@@ -426,43 +520,51 @@ RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t sce
 		sc->addEntity(e);
 		*out_entity = reinterpret_cast<rgl_entity_t>(e.get());
 	});
+	TAPE_RECORD(rgl_entity_create, D(out_entity), A(scene), A(mesh));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_destroy(rgl_entity_t entity)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		e->scene->removeEntity(e);
 		Registry<rgl::Entity>::release(entity, "Entity");
 	});
+	TAPE_RECORD(rgl_entity_destroy, A(entity));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_set_transform(rgl_entity_t entity, const rgl_mat3x4f* transform)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(transform != nullptr);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Entity>::get(entity, "Entity")->setTransform(toMat(transform));
 	});
+	TAPE_RECORD(rgl_entity_set_transform, A(entity), R(transform, 1));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_set_pose_world(rgl_entity_t entity, const rgl_mat3x4f* pose, int32_t bones_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(pose != nullptr);
 		CHECK_ARG(bones_count > 0);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Entity>::get(entity, "Entity")->setPoseAndAnimate(reinterpret_cast<const Mat3x4f*>(pose), bones_count);
 	});
+	TAPE_RECORD(rgl_entity_set_pose_world, A(entity), R(pose, bones_count), A(bones_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(id != RGL_ENTITY_INVALID_ID);
 		GraphRunCtx::synchronizeAll();
@@ -474,11 +576,13 @@ RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id)
 		e->id = id;
 		e->scene->markDirty(false);
 	});
+	TAPE_RECORD(rgl_entity_set_id, A(entity), A(id));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_texture_t texture)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(texture != nullptr);
 		GraphRunCtx::synchronizeAll();
@@ -486,23 +590,27 @@ RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_t
 		e->texture = Registry<rgl::Texture>::get(texture, "Texture");
 		e->scene->markDirty(false);
 	});
+	TAPE_RECORD(rgl_entity_set_intensity_texture, A(entity), A(texture));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_set_laser_retro(rgl_entity_t entity, float retro)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		e->laserRetro = retro;
 		e->scene->markDirty(false);
 	});
+	TAPE_RECORD(rgl_entity_set_laser_retro, A(entity), A(retro));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_apply_external_animation(rgl_entity_t entity, const rgl_vec3f* vertices,
                                                          int32_t vertex_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(vertices != nullptr);
 		CHECK_ARG(vertex_count > 0);
@@ -510,6 +618,8 @@ RGL_API rgl_status_t rgl_entity_apply_external_animation(rgl_entity_t entity, co
 		Registry<rgl::Entity>::get(entity, "Entity")->applyExternalAnimation(reinterpret_cast<const Vec3f*>(vertices),
 		                                                                    vertex_count);
 	});
+	TAPE_RECORD(rgl_entity_apply_external_animation, A(entity), R(vertices, vertex_count), A(vertex_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_entity_is_alive(rgl_entity_t entity, bool* out_alive)
@@ -523,7 +633,7 @@ RGL_API rgl_status_t rgl_entity_is_alive(rgl_entity_t entity, bool* out_alive)
 // ============================================================================================ texture
 RGL_API rgl_status_t rgl_texture_create(rgl_texture_t* out_texture, const void* texels, int32_t width, int32_t height)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(out_texture != nullptr);
 		CHECK_ARG(texels != nullptr);
 		CHECK_ARG(width > 0);
@@ -533,15 +643,19 @@ RGL_API rgl_status_t rgl_texture_create(rgl_texture_t* out_texture, const void* 
 		Registry<rgl::Texture>::add(t);
 		*out_texture = reinterpret_cast<rgl_texture_t>(t.get());
 	});
+	TAPE_RECORD(rgl_texture_create, D(out_texture), R(static_cast<const uint8_t*>(texels), static_cast<int64_t>(width) * height), A(width), A(height));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_texture_destroy(rgl_texture_t texture)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(texture != nullptr);
 		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Texture>::release(texture, "Texture");
 	});
+	TAPE_RECORD(rgl_texture_destroy, A(texture));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_texture_is_alive(rgl_texture_t texture, bool* out_alive)
@@ -555,18 +669,20 @@ RGL_API rgl_status_t rgl_texture_is_alive(rgl_texture_t texture, bool* out_alive
 // ============================================================================================ scene
 RGL_API rgl_status_t rgl_scene_set_time(rgl_scene_t scene, uint64_t nanoseconds)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		GraphRunCtx::synchronizeAll();
 		auto sc = sceneOf(scene);
 		sc->setTime(nanoseconds);
 		sc->markDirty(false);
 	});
+	TAPE_RECORD(rgl_scene_set_time, A(scene), A(nanoseconds));
+	return status;
 }
 
 // ============================================================================================ rays nodes
 RGL_API rgl_status_t rgl_node_rays_from_mat3x4f(rgl_node_t* node, const rgl_mat3x4f* rays, int32_t ray_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(rays != nullptr);
 		CHECK_ARG(ray_count > 0);
@@ -574,22 +690,26 @@ RGL_API rgl_status_t rgl_node_rays_from_mat3x4f(rgl_node_t* node, const rgl_mat3
 			n.setParameters(reinterpret_cast<const Mat3x4f*>(rays), ray_count);
 		});
 	});
+	TAPE_RECORD(rgl_node_rays_from_mat3x4f, D(node), R(rays, ray_count), A(ray_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_rays_set_ring_ids(rgl_node_t* node, const int32_t* ring_ids, int32_t ring_ids_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(ring_ids != nullptr);
 		CHECK_ARG(ring_ids_count > 0);
 		createOrUpdateNode<SetRingIdsRaysNode>(node, "SetRingIdsRaysNode", false,
 		                                       [&](SetRingIdsRaysNode& n) { n.setParameters(ring_ids, ring_ids_count); });
 	});
+	TAPE_RECORD(rgl_node_rays_set_ring_ids, D(node), R(ring_ids, ring_ids_count), A(ring_ids_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_rays_set_range(rgl_node_t* node, const rgl_vec2f* ranges, int32_t ranges_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(ranges != nullptr);
 		CHECK_ARG(ranges_count > 0);
@@ -597,53 +717,63 @@ RGL_API rgl_status_t rgl_node_rays_set_range(rgl_node_t* node, const rgl_vec2f* 
 			n.setParameters(reinterpret_cast<const Vec2f*>(ranges), ranges_count);
 		});
 	});
+	TAPE_RECORD(rgl_node_rays_set_range, D(node), R(ranges, ranges_count), A(ranges_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_rays_set_time_offsets(rgl_node_t* node, const float* offsets, int32_t offsets_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(offsets != nullptr);
 		CHECK_ARG(offsets_count > 0);
 		createOrUpdateNode<SetTimeOffsetsRaysNode>(node, "SetTimeOffsetsRaysNode", false,
 		                                           [&](SetTimeOffsetsRaysNode& n) { n.setParameters(offsets, offsets_count); });
 	});
+	TAPE_RECORD(rgl_node_rays_set_time_offsets, D(node), R(offsets, offsets_count), A(offsets_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_rays_transform(rgl_node_t* node, const rgl_mat3x4f* transform)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(transform != nullptr);
 		createOrUpdateNode<TransformRaysNode>(node, "TransformRaysNode", false,
 		                                      [&](TransformRaysNode& n) { n.setParameters(toMat(transform)); });
 	});
+	TAPE_RECORD(rgl_node_rays_transform, D(node), R(transform, 1));
+	return status;
 }
 
 // ============================================================================================ points nodes
 RGL_API rgl_status_t rgl_node_points_transform(rgl_node_t* node, const rgl_mat3x4f* transform)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(transform != nullptr);
 		createOrUpdateNode<TransformPointsNode>(node, "TransformPointsNode", false,
 		                                        [&](TransformPointsNode& n) { n.setParameters(toMat(transform)); });
 	});
+	TAPE_RECORD(rgl_node_points_transform, D(node), R(transform, 1));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace(rgl_node_t* node, rgl_scene_t scene)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto sc = sceneOf(scene);
 		createOrUpdateNode<RaytraceNode>(node, "RaytraceNode", false, [&](RaytraceNode& n) { n.setParameters(sc); });
 	});
+	TAPE_RECORD(rgl_node_raytrace, D(node), A(scene));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_velocity(rgl_node_t node, const rgl_vec3f* linear_velocity,
                                                           const rgl_vec3f* angular_velocity)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(linear_velocity != nullptr);
 		CHECK_ARG(angular_velocity != nullptr);
@@ -651,32 +781,38 @@ RGL_API rgl_status_t rgl_node_raytrace_configure_velocity(rgl_node_t node, const
 		syncNodeGraph(n);
 		n->setVelocity(*reinterpret_cast<const Vec3f*>(linear_velocity), *reinterpret_cast<const Vec3f*>(angular_velocity));
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_velocity, A(node), R(linear_velocity, 1), R(angular_velocity, 1));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_distortion(rgl_node_t node, bool enable)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
 		syncNodeGraph(n);
 		n->setDistortion(enable);
 		n->markDirty();
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_distortion, A(node), A(enable));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_non_hits(rgl_node_t node, float nearDistance, float farDistance)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
 		syncNodeGraph(n);
 		n->setNonHits(nearDistance, farDistance);
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_non_hits, A(node), A(nearDistance), A(farDistance));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_mask(rgl_node_t node, const int8_t* rays_mask, int32_t rays_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(rays_mask != nullptr);
 		CHECK_ARG(rays_count > 0);
@@ -685,12 +821,14 @@ RGL_API rgl_status_t rgl_node_raytrace_configure_mask(rgl_node_t node, const int
 		n->setMask(rays_mask, rays_count);
 		n->markDirty();
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_mask, A(node), R(rays_mask, rays_count), A(rays_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_beam_divergence(rgl_node_t node, float horizontal_beam_divergence,
                                                                  float vertical_beam_divergence)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG((horizontal_beam_divergence > 0.0f && vertical_beam_divergence > 0.0f) ||
 		          (horizontal_beam_divergence == 0.0f && vertical_beam_divergence == 0.0f));
@@ -698,22 +836,26 @@ RGL_API rgl_status_t rgl_node_raytrace_configure_beam_divergence(rgl_node_t node
 		syncNodeGraph(n);
 		n->setBeamDivergence(horizontal_beam_divergence, vertical_beam_divergence);
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_beam_divergence, A(node), A(horizontal_beam_divergence), A(vertical_beam_divergence));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_default_intensity(rgl_node_t node, float default_intensity)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(default_intensity >= 0.0f);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
 		syncNodeGraph(n);
 		n->setDefaultIntensity(default_intensity);
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_default_intensity, A(node), A(default_intensity));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_raytrace_configure_return_mode(rgl_node_t node, rgl_return_mode_t return_mode)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(return_mode != RGL_RETURN_UNKNOWN);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
@@ -723,38 +865,46 @@ RGL_API rgl_status_t rgl_node_raytrace_configure_return_mode(rgl_node_t node, rg
 			n->getCtx()->markNodesDirty();
 		}
 	});
+	TAPE_RECORD(rgl_node_raytrace_configure_return_mode, A(node), A(return_mode));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_format(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(fields != nullptr);
 		CHECK_ARG(field_count > 0);
 		std::vector<rgl_field_t> f(fields, fields + field_count);
 		createOrUpdateNode<FormatPointsNode>(node, "FormatPointsNode", true, [&](FormatPointsNode& n) { n.setParameters(f); });
 	});
+	TAPE_RECORD(rgl_node_points_format, D(node), R(fields, field_count), A(field_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_yield(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(fields != nullptr);
 		CHECK_ARG(field_count > 0);
 		std::vector<rgl_field_t> f(fields, fields + field_count);
 		createOrUpdateNode<YieldPointsNode>(node, "YieldPointsNode", true, [&](YieldPointsNode& n) { n.setParameters(f); });
 	});
+	TAPE_RECORD(rgl_node_points_yield, D(node), R(fields, field_count), A(field_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_compact_by_field(rgl_node_t* node, rgl_field_t field)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(field == RGL_FIELD_IS_HIT_I32 || field == RGL_FIELD_IS_GROUND_I32);
 		createOrUpdateNode<CompactByFieldPointsNode>(node, "CompactByFieldPointsNode", false,
 		                                             [&](CompactByFieldPointsNode& n) { n.setParameters(field); });
 	});
+	TAPE_RECORD(rgl_node_points_compact_by_field, D(node), A(field));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_compact(rgl_node_t* node) { return rgl_node_points_compact_by_field(node, RGL_FIELD_IS_HIT_I32); }
@@ -782,28 +932,32 @@ RGL_API rgl_status_t rgl_node_points_radar_bin(rgl_node_t* node, float range_ste
 
 RGL_API rgl_status_t rgl_node_points_spatial_merge(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(fields != nullptr);
 		CHECK_ARG(field_count > 0);
 		throw NotImplementedCall("spatial merge is outside the raytrace-and-point-cloud path");
 	});
+	TAPE_RECORD(rgl_node_points_spatial_merge, D(node), R(fields, field_count), A(field_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_temporal_merge(rgl_node_t* node, const rgl_field_t* fields, int32_t field_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(fields != nullptr);
 		CHECK_ARG(field_count > 0);
 		throw NotImplementedCall("temporal merge is outside the raytrace-and-point-cloud path");
 	});
+	TAPE_RECORD(rgl_node_points_temporal_merge, D(node), R(fields, field_count), A(field_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_from_array(rgl_node_t* node, const void* points, int32_t points_count,
                                                 const rgl_field_t* fields, int32_t field_count)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(points != nullptr);
 		CHECK_ARG(points_count > 0);
@@ -813,6 +967,8 @@ RGL_API rgl_status_t rgl_node_points_from_array(rgl_node_t* node, const void* po
 		createOrUpdateNode<FromArrayPointsNode>(node, "FromArrayPointsNode", true,
 		                                        [&](FromArrayPointsNode& n) { n.setParameters(points, points_count, f); });
 	});
+	TAPE_RECORD(rgl_node_points_from_array, D(node), R(static_cast<const uint8_t*>(points), static_cast<int64_t>(points_count) * recordBytes(fields, field_count)), A(points_count), R(fields, field_count), A(field_count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const rgl_radar_scope_t* radar_scopes,
@@ -821,7 +977,7 @@ RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const r
                                                        float cumulative_device_gain, float received_noise_mean,
                                                        float received_noise_st_dev)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(radar_scopes != nullptr);
 		CHECK_ARG(radar_scopes_count > 0);
@@ -853,6 +1009,8 @@ RGL_API rgl_status_t rgl_node_points_radar_postprocess(rgl_node_t* node, const r
 			                                                               received_noise_st_dev);
 		                                               });
 	});
+	TAPE_RECORD(rgl_node_points_radar_postprocess, D(node), R(radar_scopes, radar_scopes_count), A(radar_scopes_count), A(ray_azimuth_step), A(ray_elevation_step), A(frequency), A(power_transmitted), A(cumulative_device_gain), A(received_noise_mean), A(received_noise_st_dev));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_radar_track_objects(rgl_node_t* node, float object_distance_threshold,
@@ -860,25 +1018,31 @@ RGL_API rgl_status_t rgl_node_points_radar_track_objects(rgl_node_t* node, float
                                                          float object_radial_speed_threshold, float max_matching_distance,
                                                          float max_prediction_time_frame, float movement_sensitivity)
 {
-	return safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+	rgl_status_t status = safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+	TAPE_RECORD(rgl_node_points_radar_track_objects, D(node), A(object_distance_threshold), A(object_azimuth_threshold), A(object_elevation_threshold), A(object_radial_speed_threshold), A(max_matching_distance), A(max_prediction_time_frame), A(movement_sensitivity));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_radar_set_classes(rgl_node_t node, const int32_t* entity_ids,
                                                        const rgl_radar_object_class_t* object_classes, int32_t count)
 {
-	return safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+	rgl_status_t status = safeCall([&]() { throw NotImplementedCall("radar object tracking is outside this library"); });
+	TAPE_RECORD(rgl_node_points_radar_set_classes, A(node), R(entity_ids, count), R(object_classes, count), A(count));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_points_filter_ground(rgl_node_t* node, const rgl_vec3f* sensor_up_vector,
                                                    float ground_angle_threshold)
 {
-	return safeCall([&]() { throw NotImplementedCall("ground filtering is outside the raytrace-and-point-cloud path"); });
+	rgl_status_t status = safeCall([&]() { throw NotImplementedCall("ground filtering is outside the raytrace-and-point-cloud path"); });
+	TAPE_RECORD(rgl_node_points_filter_ground, D(node), R(sensor_up_vector, 1), A(ground_angle_threshold));
+	return status;
 }
 
 // ============================================================================================ noise nodes
 RGL_API rgl_status_t rgl_node_gaussian_noise_angular_ray(rgl_node_t* node, float mean, float st_dev, rgl_axis_t rotation_axis)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(st_dev >= 0);
 		CHECK_ARG((rotation_axis == RGL_AXIS_X) || (rotation_axis == RGL_AXIS_Y) || (rotation_axis == RGL_AXIS_Z));
@@ -887,12 +1051,14 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_angular_ray(rgl_node_t* node, float
 			                                                 n.setParameters(mean, st_dev, rotation_axis);
 		                                                 });
 	});
+	TAPE_RECORD(rgl_node_gaussian_noise_angular_ray, D(node), A(mean), A(st_dev), A(rotation_axis));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_gaussian_noise_angular_hitpoint(rgl_node_t* node, float mean, float st_dev,
                                                               rgl_axis_t rotation_axis)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(st_dev >= 0);
 		CHECK_ARG((rotation_axis == RGL_AXIS_X) || (rotation_axis == RGL_AXIS_Y) || (rotation_axis == RGL_AXIS_Z));
@@ -901,12 +1067,14 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_angular_hitpoint(rgl_node_t* node, 
 			                                                     n.setParameters(mean, st_dev, rotation_axis);
 		                                                     });
 	});
+	TAPE_RECORD(rgl_node_gaussian_noise_angular_hitpoint, D(node), A(mean), A(st_dev), A(rotation_axis));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_node_gaussian_noise_distance(rgl_node_t* node, float mean, float st_dev_base,
                                                       float st_dev_rise_per_meter)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(st_dev_base >= 0);
 		CHECK_ARG(st_dev_rise_per_meter >= 0);
@@ -914,6 +1082,8 @@ RGL_API rgl_status_t rgl_node_gaussian_noise_distance(rgl_node_t* node, float me
 		    node, "GaussianNoiseDistanceNode", false,
 		    [&](GaussianNoiseDistanceNode& n) { n.setParameters(mean, st_dev_base, st_dev_rise_per_meter); });
 	});
+	TAPE_RECORD(rgl_node_gaussian_noise_distance, D(node), A(mean), A(st_dev_base), A(st_dev_rise_per_meter));
+	return status;
 }
 
 // ============================================================================================ graph
@@ -927,7 +1097,7 @@ RGL_API rgl_status_t rgl_node_is_alive(rgl_node_t node, bool* out_alive)
 
 RGL_API rgl_status_t rgl_graph_run(rgl_node_t raw_node)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(raw_node != nullptr);
 		auto node = lookupNode(raw_node);
 		if (!node->hasCtx()) {
@@ -935,6 +1105,8 @@ RGL_API rgl_status_t rgl_graph_run(rgl_node_t raw_node)
 		}
 		node->getCtx()->executeAsync();
 	});
+	TAPE_RECORD(rgl_graph_run, A(raw_node));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* raw_nodes, int32_t node_count)
@@ -999,7 +1171,7 @@ RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant,
 
 RGL_API rgl_status_t rgl_graph_destroy(rgl_node_t raw_node)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(raw_node != nullptr);
 		auto node = lookupNode(raw_node);
 		if (node->hasCtx()) {
@@ -1009,11 +1181,13 @@ RGL_API rgl_status_t rgl_graph_destroy(rgl_node_t raw_node)
 			releaseNode(n.get());
 		}
 	});
+	TAPE_RECORD(rgl_graph_destroy, A(raw_node));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_get_result_size(rgl_node_t node, rgl_field_t field, int32_t* out_count, int32_t* out_size_of)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto pn = nodeAs<IPointsNode>(node, "IPointsNode");
 		pn->waitForResults();
@@ -1027,11 +1201,13 @@ RGL_API rgl_status_t rgl_graph_get_result_size(rgl_node_t node, rgl_field_t fiel
 			*out_size_of = size;
 		}
 	});
+	TAPE_RECORD(rgl_graph_get_result_size, A(node), A(field), V(out_count), V(out_size_of));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_get_result_data(rgl_node_t node, rgl_field_t field, void* dst)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(dst != nullptr);
 		auto pn = nodeAs<IPointsNode>(node, "IPointsNode");
@@ -1051,44 +1227,54 @@ RGL_API rgl_status_t rgl_graph_get_result_data(rgl_node_t node, rgl_field_t fiel
 			}
 		}
 	});
+	TAPE_RECORD(rgl_graph_get_result_data, A(node), A(field), A(static_cast<const void*>(dst)));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_node_add_child(rgl_node_t parent, rgl_node_t child)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(parent != nullptr);
 		CHECK_ARG(child != nullptr);
 		CHECK_ARG(parent != child);
 		lookupNode(parent)->addChild(lookupNode(child));
 	});
+	TAPE_RECORD(rgl_graph_node_add_child, A(parent), A(child));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_node_remove_child(rgl_node_t parent, rgl_node_t child)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(parent != nullptr);
 		CHECK_ARG(child != nullptr);
 		lookupNode(parent)->removeChild(lookupNode(child));
 	});
+	TAPE_RECORD(rgl_graph_node_remove_child, A(parent), A(child));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_node_set_priority(rgl_node_t node, int32_t priority)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto n = lookupNode(node);
 		syncNodeGraph(n);
 		n->setPriority(priority);
 	});
+	TAPE_RECORD(rgl_graph_node_set_priority, A(node), A(priority));
+	return status;
 }
 
 RGL_API rgl_status_t rgl_graph_node_get_priority(rgl_node_t node, int32_t* out_priority)
 {
-	return safeCall([&]() {
+	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		CHECK_ARG(out_priority != nullptr);
 		*out_priority = lookupNode(node)->getPriority();
 	});
+	TAPE_RECORD(rgl_graph_node_get_priority, A(node), V(out_priority));
+	return status;
 }
 
 // ============================================================================================ extensions
@@ -1339,5 +1525,41 @@ RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag
 		RGL_HIP(hipGetLastError());
 		RGL_HIP(hipDeviceSynchronize());
 		RGL_HIP(hipMemcpy(out, d.get(), sizeof(float) * count, hipMemcpyDeviceToHost));
+	});
+}
+
+// ============================================================================================ tape
+// src/api/apiCore.cpp:1456-1530
+RGL_API rgl_status_t rgl_tape_record_begin(const char* path)
+{
+	return safeCall([&]() {
+		CHECK_ARG(path != nullptr);
+		CHECK_ARG(path[0] != '\0');
+		tape::begin(path);
+	});
+}
+
+RGL_API rgl_status_t rgl_tape_record_end()
+{
+	return safeCall([&]() { tape::end(); });
+}
+
+RGL_API rgl_status_t rgl_tape_record_is_active(bool* is_active)
+{
+	return safeCall([&]() {
+		CHECK_ARG(is_active != nullptr);
+		*is_active = tape::active() != nullptr;
+	});
+}
+
+RGL_API rgl_status_t rgl_tape_play(const char* path)
+{
+	return safeCall([&]() {
+		CHECK_ARG(path != nullptr);
+		CHECK_ARG(path[0] != '\0');
+		if (tape::active() != nullptr) {
+			throw tape::TapeError("rgl_tape_play: recording active");
+		}
+		tape::play(path);
 	});
 }

@@ -188,6 +188,10 @@ _PROTOTYPES = [
     ("rgl_graph_batch_configure_autotune", [C.c_bool]),
     ("rgl_graph_batch_force_variants", [C.c_int32, C.c_int32]),
     ("rgl_graph_batch_get_variants", [i32p, i32p]),
+    ("rgl_tape_record_begin", [C.c_char_p]),
+    ("rgl_tape_record_end", []),
+    ("rgl_tape_record_is_active", [P(C.c_bool)]),
+    ("rgl_tape_play", [C.c_char_p]),
     ("rgl_graph_batch_configure_capture", [C.c_bool]),
     ("rgl_graph_batch_get_capture_stats", [P(C.c_int64), P(C.c_int64)]),
     ("rgl_configure_bvh_format", [C.c_int32]),

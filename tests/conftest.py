@@ -1,3 +1,4 @@
+import ctypes
 import sys
 from pathlib import Path
 
@@ -28,6 +29,9 @@ def rgl_lib():
     lib = rgl.load()
     rgl.check("rgl_configure_logging", 6, None, False)
     yield rgl
+    active = ctypes.c_bool(False)
+    if rgl.call("rgl_tape_record_is_active", ctypes.byref(active)) == rgl.RGL_SUCCESS and active.value:
+        rgl.call("rgl_tape_record_end")  # a failed tape test must not leave a recording running
     assert rgl.call("rgl_cleanup") == rgl.RGL_SUCCESS, rgl.last_error()
 
 
