@@ -608,8 +608,8 @@ __global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags
 		Bvh4FNode o;
 		for (int c = 0; c < 4; ++c) {
 			for (int a = 0; a < 3; ++a) {
-				o.lo[a][c] = c < cnt ? clo[c][a] : FLT_MAX;
-				o.hi[a][c] = c < cnt ? chi[c][a] : -FLT_MAX;
+				o.lo[a][c] = c < cnt ? clo[c][a] : INFINITY;
+				o.hi[a][c] = c < cnt ? chi[c][a] : INFINITY;
 			}
 			o.child[c] = c < cnt ? ref[c] : ref[0];
 		}

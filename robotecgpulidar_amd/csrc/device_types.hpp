@@ -35,7 +35,8 @@ enum BvhFormat : int32_t
 constexpr int kBvhFormats = 3;
 
 // 4-wide node with FP32 child boxes stored per axis ([axis][child]) so the four slab tests load contiguous lanes.
-// Children occupy slots [0, count); child references follow the BvhNode convention.
+// Children occupy slots [0, count); child references follow the BvhNode convention. An empty slot holds the point
+// box lo = hi = (+inf, +inf, +inf), which no ray with a finite t bound can hit, so traversal needs no count test.
 struct alignas(16) Bvh4FNode
 {
 	float lo[3][4];

@@ -43,49 +43,68 @@ constexpr int kRtBlock = RGL_RT_BLOCK; // threads per raytrace block (the LDS sh
 #endif
 constexpr int kLdsStack = 24;
 constexpr int kOvfStack = 104;
-constexpr int kSentinel = 0x7fffffff;
 constexpr int kPop = INT_MIN;
 constexpr int kBeamSamples = 58;  // 1 primary + 3 layers x 19 vertices (src/gpu/MultiReturn.hpp:19-21)
 constexpr int kBeamLayers = 3;
 constexpr int kBeamVertices = 19;
-// Slab-test widening: |rounding error| of (bound - origin) * invDir is below 3 ulp; widen by 2^-20.
+// Slab-test widening: a computed slab distance (bound - origin) * invDir is within 3 units of roundoff
+// (u = 2^-24) of the exact one and keeps its sign, so comparing the computed near distance against the far
+// distance (or the closest hit) grown by 2^-20 = 16u, with the growth itself rounded (one more u), keeps every box
+// the exact test enters: (1 + 3u) / (1 - 3u) < (1 + 16u)(1 - u).
 constexpr float kSlabGrow = 1.0f + 9.5367431640625e-7f;
-constexpr float kSlabShrink = 1.0f - 9.5367431640625e-7f;
 
+// Component selection by a one-hot vector e: e.x*v.x + e.y*v.y + e.z*v.z is exactly the selected component (the
+// products are exact, the other terms are zeros). Selecting by a per-lane index otherwise compiles to branches or
+// to an indexed scratch array.
+struct OneHot3
+{
+	float x, y, z;
+};
+__device__ __forceinline__ OneHot3 oneHot(int k)
+{
+	return OneHot3{k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f};
+}
+__device__ __forceinline__ float pick(const Vec3f& v, const OneHot3& e) { return fmaf(e.x, v.x, fmaf(e.y, v.y, e.z * v.z)); }
+
+// Ray of the traversal: origin, reciprocal direction for the slab tests, and the shear and axis permutation of
+// the watertight triangle test (axes kx, ky, kz as one-hot selectors).
 struct TravRay
 {
 	Vec3f o;
 	Vec3f inv;
 	float sx, sy, sz;
-	int kx, ky, kz;
+	OneHot3 ex, ey, ez;
 };
 
-__device__ __forceinline__ float sel3(const Vec3f& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+// Per-axis reciprocal direction of the slab tests; a zero component is replaced by 1e-30.
+__device__ __forceinline__ Vec3f invDir(Vec3f d)
+{
+	const float tiny = 1e-30f;
+	return Vec3f{1.0f / (d.x != 0.0f ? d.x : tiny), 1.0f / (d.y != 0.0f ? d.y : tiny),
+	             1.0f / (d.z != 0.0f ? d.z : tiny)};
+}
 
 __device__ __forceinline__ TravRay makeTravRay(Vec3f o, Vec3f d)
 {
 	TravRay r;
 	r.o = o;
-	const float tiny = 1e-30f;
-	r.inv.x = 1.0f / (d.x != 0.0f ? d.x : tiny);
-	r.inv.y = 1.0f / (d.y != 0.0f ? d.y : tiny);
-	r.inv.z = 1.0f / (d.z != 0.0f ? d.z : tiny);
+	r.inv = invDir(d);
 	const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
 	int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
 	int kx = kz == 2 ? 0 : kz + 1;
 	int ky = kx == 2 ? 0 : kx + 1;
-	if (sel3(d, kz) < 0.0f) {
+	r.ez = oneHot(kz);
+	const float dz = pick(d, r.ez);
+	if (dz < 0.0f) {
 		const int t = kx;
 		kx = ky;
 		ky = t;
 	}
-	const float dz = sel3(d, kz);
-	r.sx = sel3(d, kx) / dz;
-	r.sy = sel3(d, ky) / dz;
+	r.ex = oneHot(kx);
+	r.ey = oneHot(ky);
+	r.sx = pick(d, r.ex) / dz;
+	r.sy = pick(d, r.ey) / dz;
 	r.sz = 1.0f / dz;
-	r.kx = kx;
-	r.ky = ky;
-	r.kz = kz;
 	return r;
 }
 
@@ -98,19 +117,21 @@ struct HitRec
 	float u, v;
 };
 
-// Watertight test; on success writes t, u (weight of B), v (weight of C).
-__device__ __forceinline__ bool intersectTri(const TravRay& r, const LeafTri& tri, float& tOut, float& uOut, float& vOut)
+// Watertight test; on success writes t and the barycentric numerators: u = vNum / det is the weight of B and
+// v = wNum / det the weight of C (divided only for the hits that are kept).
+__device__ __forceinline__ bool intersectTri(const TravRay& r, const LeafTri& tri, float& tOut, float& vNum, float& wNum,
+                                             float& detOut)
 {
 	const Vec3f A{tri.ax - r.o.x, tri.ay - r.o.y, tri.az - r.o.z};
 	const Vec3f B{tri.bx - r.o.x, tri.by - r.o.y, tri.bz - r.o.z};
 	const Vec3f C{tri.cx - r.o.x, tri.cy - r.o.y, tri.cz - r.o.z};
-	const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
-	const float Ax = sel3(A, r.kx) - r.sx * Akz;
-	const float Ay = sel3(A, r.ky) - r.sy * Akz;
-	const float Bx = sel3(B, r.kx) - r.sx * Bkz;
-	const float By = sel3(B, r.ky) - r.sy * Bkz;
-	const float Cx = sel3(C, r.kx) - r.sx * Ckz;
-	const float Cy = sel3(C, r.ky) - r.sy * Ckz;
+	const float Akz = pick(A, r.ez), Bkz = pick(B, r.ez), Ckz = pick(C, r.ez);
+	const float Ax = pick(A, r.ex) - r.sx * Akz;
+	const float Ay = pick(A, r.ey) - r.sy * Akz;
+	const float Bx = pick(B, r.ex) - r.sx * Bkz;
+	const float By = pick(B, r.ey) - r.sy * Bkz;
+	const float Cx = pick(C, r.ex) - r.sx * Ckz;
+	const float Cy = pick(C, r.ey) - r.sy * Ckz;
 	float U = Cx * By - Cy * Bx;
 	float V = Ax * Cy - Ay * Cx;
 	float W = Bx * Ay - By * Ax;
@@ -132,8 +153,9 @@ __device__ __forceinline__ bool intersectTri(const TravRay& r, const LeafTri& tr
 	const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
 	const float T = U * Az + V * Bz + W * Cz;
 	tOut = T / det;
-	uOut = V / det;
-	vOut = W / det;
+	vNum = V;
+	wNum = W;
+	detOut = det;
 	return true;
 }
 
@@ -147,25 +169,64 @@ struct TravStats
 	int nodes, tris, insts;
 };
 
-__device__ __forceinline__ void slab(const TravRay& r, const float* f, float tBest, bool& hit, float& tNear)
+// Upper t bound of the slab tests while the closest hit so far is at tBest: widened like the far plane and kept
+// finite, so that a box whose near distance is +inf (the empty-slot box of Bvh4FNode) is never entered.
+__device__ __forceinline__ float slabBound(float tBest) { return fminf(tBest * kSlabGrow, FLT_MAX); }
+
+// Conservative slab test: the box is entered if [max(tn, 0), min(tf, tBest)] is non-empty after the far distance
+// and tBest are grown by kSlabGrow. tBound = slabBound(tBest).
+__device__ __forceinline__ void slab(const Vec3f& ro, const Vec3f& ri, const float* f, float tBound, bool& hit,
+                                     float& tNear)
 {
-	const float tx0 = (f[0] - r.o.x) * r.inv.x, tx1 = (f[3] - r.o.x) * r.inv.x;
-	const float ty0 = (f[1] - r.o.y) * r.inv.y, ty1 = (f[4] - r.o.y) * r.inv.y;
-	const float tz0 = (f[2] - r.o.z) * r.inv.z, tz1 = (f[5] - r.o.z) * r.inv.z;
+	const float tx0 = (f[0] - ro.x) * ri.x, tx1 = (f[3] - ro.x) * ri.x;
+	const float ty0 = (f[1] - ro.y) * ri.y, ty1 = (f[4] - ro.y) * ri.y;
+	const float tz0 = (f[2] - ro.z) * ri.z, tz1 = (f[5] - ro.z) * ri.z;
 	const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
 	const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-	const float tnC = tn > 0.0f ? tn * kSlabShrink : 0.0f;
-	const float tfC = tf * kSlabGrow;
-	hit = tnC <= tfC && tnC <= tBest * kSlabGrow && tfC >= 0.0f;
+	hit = fmaxf(tn, 0.0f) <= fminf(tf * kSlabGrow, tBound);
 	tNear = tn;
 }
 
 // Slab test of a decoded child box.
-__device__ __forceinline__ void slab6(const TravRay& r, float lx, float ly, float lz, float hx, float hy, float hz,
-                                      float tBest, bool& hit, float& tNear)
+__device__ __forceinline__ void slab6(const Vec3f& ro, const Vec3f& ri, float lx, float ly, float lz, float hx, float hy,
+                                      float hz, float tBound, bool& hit, float& tNear)
 {
 	const float f[6] = {lx, ly, lz, hx, hy, hz};
-	slab(r, f, tBest, hit, tNear);
+	slab(ro, ri, f, tBound, hit, tNear);
+}
+
+// Node and triangle records are read through global-address-space pointers, so the loads are global (not flat)
+// loads and only wait on the vector memory counter.
+typedef float RglF4 __attribute__((ext_vector_type(4)));
+typedef int RglI4 __attribute__((ext_vector_type(4)));
+typedef unsigned RglU4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) RglF4 GlobalFloat4;
+typedef const __attribute__((address_space(1))) RglI4 GlobalInt4;
+typedef const __attribute__((address_space(1))) RglU4 GlobalUint4;
+template<typename T>
+__device__ __forceinline__ T* globalPtr(const void* p)
+{
+	return (T*)(p);
+}
+
+__device__ __forceinline__ LeafTri loadLeafTri(const LeafTri* tris, int leaf)
+{
+	GlobalFloat4* q = globalPtr<GlobalFloat4>(tris + leaf);
+	const RglF4 a = q[0], b = q[1], c = q[2];
+	LeafTri t;
+	t.ax = a.x;
+	t.ay = a.y;
+	t.az = a.z;
+	t.prim = __float_as_int(a.w);
+	t.bx = b.x;
+	t.by = b.y;
+	t.bz = b.z;
+	t.pad0 = 0.0f;
+	t.cx = c.x;
+	t.cy = c.y;
+	t.cz = c.z;
+	t.pad1 = 0.0f;
+	return t;
 }
 
 __device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<float>((w >> (8 * c)) & 0xffu); }
@@ -173,6 +234,11 @@ __device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<f
 // Closest hit of one ray against the two-level BVH; returns false if nothing is hit in (0, tmax].
 // kFmt selects the node format of the TLAS and BLASes (BvhFormat). With kStats the canonical binary trees are
 // traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted.
+//
+// One stack serves both levels. Entering an instance records the stack depth (instBase); when pops bring the depth
+// back to it, the BLAS is finished and traversal continues in the TLAS. The world-space ray is never copied: node
+// steps select the world or the object-space origin and reciprocal direction by the current level, so the loop
+// carries one ray record that changes only on instance entry.
 template<bool kStats, int kFmt, int kBlock = kRtBlock>
 __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
 {
@@ -208,39 +274,63 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 #define RGL_POP() (--sp, sp < kLdsStack ? ldsStack[sp * kBlock + tid] : __float_as_int(ovf[sp - kLdsStack]))
 #define RGL_CSWAP(a, b)                                                                                                \
 	do {                                                                                                               \
-		if (k##b < k##a) {                                                                                             \
-			const float tk = k##a;                                                                                     \
-			k##a = k##b;                                                                                               \
-			k##b = tk;                                                                                                 \
-			const int tr = r##a;                                                                                       \
-			r##a = r##b;                                                                                               \
-			r##b = tr;                                                                                                 \
-		}                                                                                                              \
+		const bool sw = k##b < k##a;                                                                                   \
+		const float ka_ = sw ? k##b : k##a, kb_ = sw ? k##a : k##b;                                                    \
+		const int ra_ = sw ? r##b : r##a, rb_ = sw ? r##a : r##b;                                                      \
+		k##a = ka_;                                                                                                    \
+		k##b = kb_;                                                                                                    \
+		r##a = ra_;                                                                                                    \
+		r##b = rb_;                                                                                                    \
 	} while (0)
 
-	const TravRay world = makeTravRay(o, d);
-	TravRay cur = world;
+	const Vec3f worldInv = invDir(d);
+	TravRay obj;  // object-space ray of the entered instance (valid while inst >= 0)
+	obj.o = o;
+	obj.inv = worldInv;
+	obj.sx = obj.sy = obj.sz = 0.0f;
+	obj.ex = obj.ey = obj.ez = oneHot(0);
 	const void* tlas = kStats ? static_cast<const void*>(sc.tlas2) : sc.tlasNodes;
-	const void* nodes = tlas;
+	const void* blas = tlas;
 	const LeafTri* tris = nullptr;
 	int inst = -1;
+	int instBase = -1;
 	bool found = false;
 	int node = 0;
 	while (true) {
+		if (node < 0) {
+			// An instance leaf of the TLAS: continue at the root of its BLAS with the object-space ray. Entry is
+			// handled here, outside the pop loop, so the ray record is only written at this point of the loop.
+			inst = ~node;
+			instBase = sp;
+			const InstanceRec& ir = sc.instances[inst];
+			obj = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
+			blas = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
+			tris = ir.blasTris;
+			if (kStats) {
+				st->insts++;
+			}
+			node = 0;
+		}
 		if (kStats) {
 			st->nodes++;
 		}
 		int next;
+		const float tBound = slabBound(best.t);
+		const bool inTlas = inst < 0;
+		const Vec3f ro{inTlas ? o.x : obj.o.x, inTlas ? o.y : obj.o.y, inTlas ? o.z : obj.o.z};
+		const Vec3f ri{inTlas ? worldInv.x : obj.inv.x, inTlas ? worldInv.y : obj.inv.y,
+		               inTlas ? worldInv.z : obj.inv.z};
+		const void* nodes = inTlas ? tlas : blas;
 		if constexpr (kFmt == kBvh2) {
-			const float4* np = reinterpret_cast<const float4*>(static_cast<const BvhNode*>(nodes) + node);
-			const float4 q0 = np[0], q1 = np[1], q2 = np[2];
-			const int4 q3 = reinterpret_cast<const int4*>(np)[3];
+			GlobalFloat4* np = globalPtr<GlobalFloat4>(static_cast<const BvhNode*>(nodes) + node);
+			const RglF4 q0 = np[0], q1 = np[1], q2 = np[2];
+			const RglI4 q3 = globalPtr<GlobalInt4>((const void*)np)[3];
 			const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
 			const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 			bool h0, h1;
 			float n0, n1;
-			slab(cur, b0, best.t, h0, n0);
-			slab(cur, b1, best.t, h1, n1);
+			slab(ro, ri, b0, tBound, h0, n0);
+			slab(ro, ri, b1, tBound, h1, n1);
 			if (h0 && h1) {
 				const bool firstNear = n0 <= n1;
 				RGL_PUSH(firstNear ? q3.y : q3.x);
@@ -256,38 +346,42 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			// Four children: slab-test each, order the hit ones near to far, descend into the nearest.
 			bool h0, h1, h2, h3;
 			float k0, k1, k2, k3;
-			int r0, r1, r2, r3, cnt;
+			int r0, r1, r2, r3;
 			if constexpr (kFmt == kBvh4F) {
-				const float4* np = reinterpret_cast<const float4*>(static_cast<const Bvh4FNode*>(nodes) + node);
-				const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-				const int4 ch = reinterpret_cast<const int4*>(np)[6];
-				cnt = reinterpret_cast<const int4*>(np)[7].x;
-				slab6(cur, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, best.t, h0, k0);
-				slab6(cur, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, best.t, h1, k1);
-				slab6(cur, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, best.t, h2, k2);
-				slab6(cur, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, best.t, h3, k3);
+				// Empty slots hold boxes no ray enters, so the child count is not read.
+				GlobalFloat4* np = globalPtr<GlobalFloat4>(static_cast<const Bvh4FNode*>(nodes) + node);
+				const RglF4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+				const RglI4 ch = globalPtr<GlobalInt4>((const void*)np)[6];
+				slab6(ro, ri, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBound, h0, k0);
+				slab6(ro, ri, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBound, h1, k1);
+				slab6(ro, ri, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBound, h2, k2);
+				slab6(ro, ri, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBound, h3, k3);
 				r0 = ch.x;
 				r1 = ch.y;
 				r2 = ch.z;
 				r3 = ch.w;
 			} else {
-				const uint4* up = reinterpret_cast<const uint4*>(static_cast<const Bvh4QNode*>(nodes) + node);
-				const uint4 u0 = up[0], u1 = up[1], u2 = up[2], u3 = up[3];
+				GlobalUint4* up = globalPtr<GlobalUint4>(static_cast<const Bvh4QNode*>(nodes) + node);
+				const RglU4 u0 = up[0], u1 = up[1], u2 = up[2], u3 = up[3];
 				const float ox = __uint_as_float(u0.x), oy = __uint_as_float(u0.y), oz = __uint_as_float(u0.z);
 				// exp byte = e + 127, so (byte << 23) is the bit pattern of 2^e.
 				const float sx = __uint_as_float((u0.w & 0xffu) << 23);
 				const float sy = __uint_as_float(((u0.w >> 8) & 0xffu) << 23);
 				const float sz = __uint_as_float(((u0.w >> 16) & 0xffu) << 23);
-				cnt = static_cast<int>(u0.w >> 24);
+				const int cnt = static_cast<int>(u0.w >> 24);
 #define RGL_QBOX(c, h, k)                                                                                              \
-	slab6(cur, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                                \
+	slab6(ro, ri, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                             \
 	      __fadd_rn(oz, ubyte(u1.z, c) * sz), __fadd_rn(ox, ubyte(u1.w, c) * sx), __fadd_rn(oy, ubyte(u2.x, c) * sy),  \
-	      __fadd_rn(oz, ubyte(u2.y, c) * sz), best.t, h, k)
+	      __fadd_rn(oz, ubyte(u2.y, c) * sz), tBound, h, k)
 				RGL_QBOX(0, h0, k0);
 				RGL_QBOX(1, h1, k1);
 				RGL_QBOX(2, h2, k2);
 				RGL_QBOX(3, h3, k3);
 #undef RGL_QBOX
+				// Empty slots decode to inverted boxes, which the slab test does not reject.
+				h1 = h1 && cnt > 1;
+				h2 = h2 && cnt > 2;
+				h3 = h3 && cnt > 3;
 				r0 = static_cast<int>(u2.z);
 				r1 = static_cast<int>(u2.w);
 				r2 = static_cast<int>(u3.x);
@@ -295,77 +389,89 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			}
 			const float kMiss = INFINITY;
 			k0 = h0 ? k0 : kMiss;
-			k1 = (h1 && cnt > 1) ? k1 : kMiss;
-			k2 = (h2 && cnt > 2) ? k2 : kMiss;
-			k3 = (h3 && cnt > 3) ? k3 : kMiss;
+			k1 = h1 ? k1 : kMiss;
+			k2 = h2 ? k2 : kMiss;
+			k3 = h3 ? k3 : kMiss;
 			RGL_CSWAP(0, 1);
 			RGL_CSWAP(2, 3);
 			RGL_CSWAP(0, 2);
 			RGL_CSWAP(1, 3);
 			RGL_CSWAP(1, 2);
-			if (k3 != kMiss) {
-				RGL_PUSH(r3);
-			}
-			if (k2 != kMiss) {
-				RGL_PUSH(r2);
-			}
-			if (k1 != kMiss) {
-				RGL_PUSH(r1);
+			if (__all(sp <= kLdsStack - 3)) {
+				// Every lane has room for three LDS entries: write all three slots unconditionally and advance the
+				// stack pointer past the hit ones only (children are sorted, so misses come last and are
+				// overwritten or left above the top).
+				int t = sp;
+				ldsStack[t * kBlock + tid] = r3;
+				t += k3 != kMiss ? 1 : 0;
+				ldsStack[t * kBlock + tid] = r2;
+				t += k2 != kMiss ? 1 : 0;
+				ldsStack[t * kBlock + tid] = r1;
+				t += k1 != kMiss ? 1 : 0;
+				sp = t;
+			} else {
+				if (k3 != kMiss) {
+					RGL_PUSH(r3);
+				}
+				if (k2 != kMiss) {
+					RGL_PUSH(r2);
+				}
+				if (k1 != kMiss) {
+					RGL_PUSH(r1);
+				}
 			}
 			next = k0 != kMiss ? r0 : kPop;
 		}
-		while (true) {
+		// Pops and triangle leaves until the next inner node or instance leaf (one exit, so the loop-carried
+		// values need no copies at divergent exits).
+		bool done = false;
+		bool tri;
+		do {
 			if (next == kPop) {
-				if (sp == 0 || overflow) {
-					if (overflow) {
-						*sc.overflow = 1;
+				if (sp == instBase) {
+					inst = -1; // the entered instance's BLAS is finished
+					instBase = -1;
+				}
+				done = sp == 0 || overflow;
+				if (!done) {
+					if (__all(sp <= kLdsStack)) {
+						--sp;
+						next = ldsStack[sp * kBlock + tid];
+					} else {
+						next = RGL_POP();
 					}
-					return found;
-				}
-				next = RGL_POP();
-				if (next == kSentinel) {
-					cur = world;
-					nodes = tlas;
-					inst = -1;
-					next = kPop;
-					continue;
 				}
 			}
-			if (next >= 0) {
-				break;
-			}
-			const int leaf = ~next;
-			if (inst < 0) {
-				inst = leaf;
-				const InstanceRec& ir = sc.instances[inst];
-				cur = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
-				nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
-				tris = ir.blasTris;
+			tri = !done && next < 0 && inst >= 0;
+			if (tri) {
+				const int leaf = ~next;
 				if (kStats) {
-					st->insts++;
+					st->tris++;
 				}
-				RGL_PUSH(kSentinel);
-				next = 0;
-				continue;
+				const LeafTri lt = loadLeafTri(tris, leaf);
+				float t, vNum, wNum, det;
+				if (intersectTri(obj, lt, t, vNum, wNum, det) && t > 0.0f && t <= tmax &&
+				    better(t, inst, lt.prim, best)) {
+					best.t = t;
+					best.inst = inst;
+					best.prim = lt.prim;
+					best.leaf = leaf;
+					best.u = vNum / det;
+					best.v = wNum / det;
+					found = true;
+				}
+				next = kPop;
 			}
-			if (kStats) {
-				st->tris++;
-			}
-			const LeafTri tri = tris[leaf];
-			float t, u, v;
-			if (intersectTri(cur, tri, t, u, v) && t > 0.0f && t <= tmax && better(t, inst, tri.prim, best)) {
-				best.t = t;
-				best.inst = inst;
-				best.prim = tri.prim;
-				best.leaf = leaf;
-				best.u = u;
-				best.v = v;
-				found = true;
-			}
-			next = kPop;
+		} while (tri);
+		if (done) {
+			break;
 		}
 		node = next;
 	}
+	if (overflow) {
+		*sc.overflow = 1;
+	}
+	return found;
 #undef RGL_PUSH
 #undef RGL_POP
 #undef RGL_CSWAP
@@ -823,8 +929,8 @@ __global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytracePara
 		const TravRay r = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
 		for (int k = 0; k < triCounts[i]; ++k) {
 			const LeafTri tri = ir.blasTris[k];
-			float t, u, v;
-			if (intersectTri(r, tri, t, u, v) && t > 0.0f && t <= range.y && better(t, i, tri.prim, best)) {
+			float t, vNum, wNum, det;
+			if (intersectTri(r, tri, t, vNum, wNum, det) && t > 0.0f && t <= range.y && better(t, i, tri.prim, best)) {
 				best.t = t;
 				best.inst = i;
 				best.prim = tri.prim;
