@@ -150,10 +150,13 @@ struct RaytraceOutputs
 	float* incidentAngle;
 };
 
+
 struct RaytraceParams
 {
-	const Mat3x4f* rays;          // world-frame rays (after the rays-node chain)
+	const Mat3x4f* rays;          // rays of the rays-node chain (world frame unless rayPreTransformed)
 	int32_t rayCount;
+	int32_t rayPreTransformed;    // 1: the world-frame ray is rayPreTransform * rays[i]
+	Mat3x4f rayPreTransform;
 	Mat3x4f rayOriginToWorld;     // cumulative ray transform of the rays chain
 	Mat3x4f worldToRayOrigin;     // its inverse, computed once on the host
 	const Vec2f* ranges;          // 1 or rayCount entries

@@ -468,8 +468,36 @@ void TransformRaysNode::pointOp(PointOpItem& it, FieldCopyList&)
 	it.out0 = out.ensureT<Mat3x4f>(std::max(n, 1));
 	it.m0 = transform;
 }
+bool TransformRaysNode::deferToRaytrace() const
+{
+	if (outputs.empty() || input->deferredRays().hasPre) {
+		return false;
+	}
+	std::function<bool(const Node&)> accepts = [&](const Node& n) -> bool {
+		if (dynamic_cast<const SetRangeRaysNode*>(&n) != nullptr || dynamic_cast<const SetRingIdsRaysNode*>(&n) != nullptr ||
+		    dynamic_cast<const SetTimeOffsetsRaysNode*>(&n) != nullptr) {
+			const auto& outs = n.getOutputs();
+			return !outs.empty() && std::all_of(outs.begin(), outs.end(), [&](const Node::Ptr& o) { return accepts(*o); });
+		}
+		const auto* rt = dynamic_cast<const RaytraceNode*>(&n);
+		return rt != nullptr && !rt->hasField(RGL_FIELD_RAY_POSE_MAT3x4_F32);
+	};
+	return std::all_of(outputs.begin(), outputs.end(), [&](const Node::Ptr& o) { return accepts(*o); });
+}
+
+IRaysNode::DeferredRays TransformRaysNode::deferredRays() const
+{
+	if (deferToRaytrace()) {
+		return {input->rays(), transform, true};
+	}
+	return {out.as<Mat3x4f>(), Mat3x4f::identity(), false};
+}
+
 void TransformRaysNode::enqueueImpl()
 {
+	if (deferToRaytrace()) {
+		return;
+	}
 	PointOpItem it{};
 	FieldCopyList unused;
 	pointOp(it, unused);
@@ -577,7 +605,10 @@ void RaytraceNode::setMask(const int8_t* m, int n)
 RaytraceParams RaytraceNode::buildParams(const SceneView&) const
 {
 	RaytraceParams p{};
-	p.rays = raysNode->rays();
+	const IRaysNode::DeferredRays dr = raysNode->deferredRays();
+	p.rays = dr.rays;
+	p.rayPreTransformed = dr.hasPre ? 1 : 0;
+	p.rayPreTransform = dr.pre;
 	p.rayCount = raysNode->rayCount();
 	p.rayOriginToWorld = raysNode->cumulativeRayTransform();
 	p.worldToRayOrigin = p.rayOriginToWorld.inverse();

@@ -106,6 +106,15 @@ struct IRaysNode : virtual Node
 	virtual const float* timeOffsets() const { return nullptr; }
 	virtual int timeOffsetsCount() const { return 0; }
 	virtual Mat3x4f cumulativeRayTransform() const { return Mat3x4f::identity(); }
+	// The rays as a raytrace launch consumes them: `rays`, with `pre` still to be applied when `hasPre` (a transform
+	// the chain left to the raytrace kernel).
+	struct DeferredRays
+	{
+		const Mat3x4f* rays;
+		Mat3x4f pre;
+		bool hasPre;
+	};
+	virtual DeferredRays deferredRays() const { return {rays(), Mat3x4f::identity(), false}; }
 };
 
 struct IRaysNodeSingleInput : IRaysNode
@@ -311,6 +320,7 @@ struct SetRingIdsRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const int32_t* ids, int count);
 	std::string getName() const override { return "SetRingIdsRaysNode"; }
+	DeferredRays deferredRays() const override { return input->deferredRays(); }
 	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const int32_t* ringIds() const override { return buf.as<int32_t>(); }
 	int ringIdsCount() const override { return count; }
@@ -328,6 +338,7 @@ struct SetRangeRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const Vec2f* r, int count);
 	std::string getName() const override { return "SetRangeRaysNode"; }
+	DeferredRays deferredRays() const override { return input->deferredRays(); }
 	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const Vec2f* ranges() const override { return buf.as<Vec2f>(); }
 	int rangesCount() const override { return count; }
@@ -345,6 +356,7 @@ struct SetTimeOffsetsRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const float* offsets, int count);
 	std::string getName() const override { return "SetTimeOffsetsRaysNode"; }
+	DeferredRays deferredRays() const override { return input->deferredRays(); }
 	BatchStage batchStage() const override { return BatchStage::Passive; }
 	const float* timeOffsets() const override { return buf.as<float>(); }
 	int timeOffsetsCount() const override { return count; }
@@ -358,14 +370,20 @@ private:
 	int count = 0;
 };
 
+// When every consumer of the transformed rays is a raytrace node that does not output ray poses (possibly behind
+// range / ring id / time offset nodes), the transform is not materialized: the raytrace kernel applies it to each
+// ray it loads (the device matrix product rounds identically in every kernel, so the poses are bit-identical),
+// which saves a pass over the rays.
 struct TransformRaysNode : IRaysNodeSingleInput
 {
 	void setParameters(const Mat3x4f& tf) { transform = tf; }
 	std::string getName() const override { return "TransformRaysNode"; }
 	const Mat3x4f* rays() const override { return out.as<Mat3x4f>(); }
+	DeferredRays deferredRays() const override;
 	Mat3x4f cumulativeRayTransform() const override { return transform * input->cumulativeRayTransform(); }
-	BatchStage batchStage() const override { return BatchStage::PointOp; }
+	BatchStage batchStage() const override { return deferToRaytrace() ? BatchStage::Passive : BatchStage::PointOp; }
 	void pointOp(PointOpItem& it, FieldCopyList& layout) override;
+	bool deferToRaytrace() const;
 
 protected:
 	void enqueueImpl() override;

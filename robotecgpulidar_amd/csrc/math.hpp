@@ -155,31 +155,52 @@ struct Mat3x4f
 	}
 };
 
+// Three-term dot product a0*b0 + a1*b1 + a2*b2. In device code every rounding is explicit (two fused multiply-adds
+// in a fixed order), so a matrix product gives the same bits in every kernel whatever the compiler would contract
+// around it: batched and single launches of the same ray produce identical poses, angles and hits.
+RGL_HD float dot3(float a0, float b0, float a1, float b1, float a2, float b2)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __fmaf_rn(a2, b2, __fmaf_rn(a1, b1, __fmul_rn(a0, b0)));
+#else
+	return (a0 * b0) + (a1 * b1) + (a2 * b2);
+#endif
+}
+
+RGL_HD float add1(float a, float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __fadd_rn(a, b);
+#else
+	return a + b;
+#endif
+}
+
 RGL_HD Mat3x4f operator*(const Mat3x4f& a, const Mat3x4f& b)
 {
 	Mat3x4f r;
 	for (int y = 0; y < 3; ++y) {
 		for (int x = 0; x < 4; ++x) {
-			r.rc[y][x] = (a.rc[y][0] * b.rc[0][x]) + (a.rc[y][1] * b.rc[1][x]) + (a.rc[y][2] * b.rc[2][x]);
+			r.rc[y][x] = dot3(a.rc[y][0], b.rc[0][x], a.rc[y][1], b.rc[1][x], a.rc[y][2], b.rc[2][x]);
 		}
-		r.rc[y][3] += a.rc[y][3];
+		r.rc[y][3] = add1(r.rc[y][3], a.rc[y][3]);
 	}
 	return r;
 }
 
 RGL_HD Vec3f operator*(const Mat3x4f& m, Vec3f v)
 {
-	return {(m.rc[0][0] * v.x) + (m.rc[0][1] * v.y) + (m.rc[0][2] * v.z) + m.rc[0][3],
-	        (m.rc[1][0] * v.x) + (m.rc[1][1] * v.y) + (m.rc[1][2] * v.z) + m.rc[1][3],
-	        (m.rc[2][0] * v.x) + (m.rc[2][1] * v.y) + (m.rc[2][2] * v.z) + m.rc[2][3]};
+	return {add1(dot3(m.rc[0][0], v.x, m.rc[0][1], v.y, m.rc[0][2], v.z), m.rc[0][3]),
+	        add1(dot3(m.rc[1][0], v.x, m.rc[1][1], v.y, m.rc[1][2], v.z), m.rc[1][3]),
+	        add1(dot3(m.rc[2][0], v.x, m.rc[2][1], v.y, m.rc[2][2], v.z), m.rc[2][3])};
 }
 
 // Linear part only (direction transform).
 RGL_HD Vec3f mulDir(const Mat3x4f& m, Vec3f v)
 {
-	return {(m.rc[0][0] * v.x) + (m.rc[0][1] * v.y) + (m.rc[0][2] * v.z),
-	        (m.rc[1][0] * v.x) + (m.rc[1][1] * v.y) + (m.rc[1][2] * v.z),
-	        (m.rc[2][0] * v.x) + (m.rc[2][1] * v.y) + (m.rc[2][2] * v.z)};
+	return {dot3(m.rc[0][0], v.x, m.rc[0][1], v.y, m.rc[0][2], v.z),
+	        dot3(m.rc[1][0], v.x, m.rc[1][1], v.y, m.rc[1][2], v.z),
+	        dot3(m.rc[2][0], v.x, m.rc[2][1], v.y, m.rc[2][2], v.z)};
 }
 
 // TRS = T * R(deg) * S (src/math/Mat3x4f.hpp:91-97).

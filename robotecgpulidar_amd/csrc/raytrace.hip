@@ -490,6 +490,13 @@ __device__ Mat3x4f beamSampleTransform(float hHalf, float vHalf, int layer, int 
 	return Mat3x4f::rotationRad(vAngle, 0.0f, 0.0f) * Mat3x4f::rotationRad(0.0f, hAngle, 0.0f);
 }
 
+// World-frame pose of ray i of the rays chain; a transform the chain deferred to this kernel is applied here (the
+// device matrix product rounds the same way in every kernel, so the pose equals the transform node's output).
+__device__ __forceinline__ Mat3x4f rayWorldPose(const RaytraceParams& p, int i)
+{
+	return p.rayPreTransformed ? p.rayPreTransform * p.rays[i] : p.rays[i];
+}
+
 // Ray origin = M*(0,0,0); direction = M*(0,0,1) - origin, evaluated as the reference does (not column 2 verbatim).
 __device__ __forceinline__ void rayOriginDir(const Mat3x4f& m, Vec3f& o, Vec3f& d)
 {
@@ -588,12 +595,10 @@ __device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float
 	return s;
 }
 
-__device__ void writeReturnHit(const RaytraceOutputs& out, const RaytraceParams& p, int beam, int rp, uint8_t type,
-                               const Shade& s)
+// o, d: origin and direction of the beam's world-frame ray (before distortion).
+__device__ void writeReturnHit(const RaytraceOutputs& out, Vec3f o, Vec3f d, int rp, uint8_t type, const Shade& s)
 {
 	if (out.xyz) {
-		Vec3f o, d;
-		rayOriginDir(p.rays[beam], o, d);
 		out.xyz[rp] = o + normalized(d) * s.distance;
 	}
 	if (out.isHit) out.isHit[rp] = 1;
@@ -612,13 +617,10 @@ __device__ void writeReturnHit(const RaytraceOutputs& out, const RaytraceParams&
 	if (out.laserRetro) out.laserRetro[rp] = s.retro;
 }
 
-__device__ void writeReturnNonHit(const RaytraceOutputs& out, const RaytraceParams& p, int beam, int rp, uint8_t type,
-                                  float d0)
+__device__ void writeReturnNonHit(const RaytraceOutputs& out, Vec3f o, Vec3f d, int rp, uint8_t type, float d0)
 {
 	const float qnan = __int_as_float(0x7fc00000);
 	if (out.xyz) {
-		Vec3f o, d;
-		rayOriginDir(p.rays[beam], o, d);
 		Vec3f disp = normalized(d) * d0;
 		disp.x = isnan(disp.x) ? 0.0f : disp.x;
 		disp.y = isnan(disp.y) ? 0.0f : disp.y;
@@ -684,13 +686,15 @@ template<int kFmt, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                              const int beam)
 {
-	const Mat3x4f rayWorld = p.rays[beam];
+	const Mat3x4f rayWorld = rayWorldPose(p, beam);
 	const int R = p.returnCount;
 	writeBeamShared(out, p, beam, rayWorld);
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
 	if (p.mask != nullptr && p.mask[beam] == 0) {
+		Vec3f wo, wd;
+		rayOriginDir(rayWorld, wo, wd);
 		for (int r = 0; r < R; ++r) {
-			writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
+			writeReturnNonHit(out, wo, wd, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
 		}
 		return;
 	}
@@ -699,13 +703,20 @@ __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const Raytrace
 	rayOriginDir(ray, so, sd);
 	HitRec h;
 	float d0 = p.farNonHitDistance;
-	if (traceClosest<false, kFmt, kBlock>(sc, so, sd, range.y, h, nullptr)) {
+	const bool hit = traceClosest<false, kFmt, kBlock>(sc, so, sd, range.y, h, nullptr);
+	// Points are placed along the undistorted world ray; without distortion that is the traced ray, otherwise it is
+	// reloaded here rather than kept in registers through the traversal.
+	Vec3f wo = so, wd = sd;
+	if (p.doDistortion) {
+		rayOriginDir(rayWorldPose(p, beam), wo, wd);
+	}
+	if (hit) {
 		const Shade sh = shadeSample(sc, p, range.x, so, sd, h, p.velocityRequested != 0);
 		if (sh.isHit) {
 			for (int r = 0; r < R; ++r) {
 				const uint8_t type = returnTypeOf(p.returnMode, r);
 				if (type >= 1 && type <= 5) {
-					writeReturnHit(out, p, beam, beam * R + r, type, sh);
+					writeReturnHit(out, wo, wd, beam * R + r, type, sh);
 				}
 			}
 			return;
@@ -713,7 +724,7 @@ __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const Raytrace
 		d0 = sh.distance;
 	}
 	for (int r = 0; r < R; ++r) {
-		writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), d0);
+		writeReturnNonHit(out, wo, wd, beam * R + r, returnTypeOf(p.returnMode, r), d0);
 	}
 }
 
@@ -727,16 +738,18 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
                                                  const int beam)
 {
 	const int lane = threadIdx.x & 63;
-	const Mat3x4f rayWorld = p.rays[beam];
+	const Mat3x4f rayWorld = rayWorldPose(p, beam);
 	const int R = p.returnCount;
 	if (lane == 0) {
 		writeBeamShared(out, p, beam, rayWorld);
 	}
+	Vec3f wo, wd;
+	rayOriginDir(rayWorld, wo, wd);
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
 	if (p.mask != nullptr && p.mask[beam] == 0) {
 		if (lane == 0) {
 			for (int r = 0; r < R; ++r) {
-				writeReturnNonHit(out, p, beam, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
+				writeReturnNonHit(out, wo, wd, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
 			}
 		}
 		return;
@@ -809,7 +822,7 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		const int rp = beam * R + r;
 		if (fS < 0) {
 			if (lane == 0) {
-				writeReturnNonHit(out, p, beam, rp, type, d0);
+				writeReturnNonHit(out, wo, wd, rp, type, d0);
 			}
 			continue;
 		}
@@ -819,7 +832,7 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		const int sel = type == 3 ? sS : (type == 2 ? lS : (type == 1 ? stS : (type == 5 ? ssS : fS)));
 		if (lane == sel) {
 			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
-			writeReturnHit(out, p, beam, rp, type, sh);
+			writeReturnHit(out, wo, wd, rp, type, sh);
 		}
 	}
 }
@@ -897,7 +910,7 @@ __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, Raytra
 	}
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
 	Vec3f so, sd;
-	rayOriginDir(p.rays[beam], so, sd);
+	rayOriginDir(rayWorldPose(p, beam), so, sd);
 	HitRec h;
 	TravStats st{0, 0, 0};
 	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
@@ -918,7 +931,7 @@ __global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytracePara
 	}
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
 	Vec3f o, d;
-	rayOriginDir(p.rays[beam], o, d);
+	rayOriginDir(rayWorldPose(p, beam), o, d);
 	HitRec best;
 	best.t = range.y;
 	best.inst = INT_MAX;

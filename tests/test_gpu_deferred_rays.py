@@ -1,0 +1,97 @@
+"""Deferred ray transforms: when the only consumers of a rays-transform node are raytrace nodes that do not output
+ray poses, the transform is applied inside the raytrace kernel instead of in a pass of its own. The results must
+be bit-identical to the same graph with the transform materialized (forced here by also yielding RAY_POSE), run
+alone and in batches, with divergent beams and with velocity distortion."""
+import numpy as np
+import pytest
+
+from robotecgpulidar_amd import scenes as S
+from tests.test_gpu_batch import assert_same, build_world
+
+pytestmark = pytest.mark.gpu
+
+XYZ, IS_HIT, ENT, DIST, NORMAL, RAY_POSE = 1, 4, 7, 8, 22, 24
+FIELDS = [XYZ, DIST, ENT, IS_HIT, NORMAL]
+
+
+class Graph:
+    """rays -> transform(pose) -> range -> [time offsets] -> raytrace -> yield (+ RAY_POSE if `with_pose`)."""
+
+    def __init__(self, rgl, scene, ch, az, seed, with_pose, divergent=False, distortion=False):
+        self.rgl = rgl
+        lidar = S.spinning_lidar(ch, az, -25.0, 12.0)
+        pose = S.f32(S.trs((0.7 + 0.3 * seed, -1.3, 1.9), (3.0 * seed, -2.0, 17.0 * seed), (1, 1, 1)))
+        self.rays = rgl.node_rays_from_mat3x4f(rgl.Node(), lidar)
+        chain = [self.rays, rgl.node_rays_transform(rgl.Node(), pose),
+                 rgl.node_rays_set_range(rgl.Node(), np.array([[0.3, 120.0]], np.float32))]
+        if distortion:
+            chain.append(rgl.node_rays_set_time_offsets(rgl.Node(), np.linspace(0.0, 50.0, len(lidar), dtype=np.float32)))
+        rt = rgl.node_raytrace(rgl.Node(), scene)
+        rgl.check("rgl_node_raytrace_configure_non_hits", rt.h, 0.0, 120.0)
+        if divergent:
+            rgl.check("rgl_node_raytrace_configure_beam_divergence", rt.h, 2e-3, 2e-3)
+            rgl.check("rgl_node_raytrace_configure_return_mode", rt.h, rgl.RGL_RETURN_FIRST_LAST)
+        if distortion:
+            lin = np.asarray([12.0, -3.0, 0.5], np.float32)
+            ang = np.asarray([0.0, 0.1, 0.8], np.float32)
+            rgl.check("rgl_node_raytrace_configure_velocity", rt.h, lin.ctypes.data, ang.ctypes.data)
+            rgl.check("rgl_node_raytrace_configure_distortion", rt.h, True)
+        chain.append(rt)
+        self.fields = FIELDS + ([RAY_POSE] if with_pose else [])
+        self.out = rgl.node_points_yield(rgl.Node(), self.fields)
+        chain.append(self.out)
+        rgl.chain(*chain)
+
+    def results(self):
+        return {f: self.rgl.result(self.out, f) for f in self.fields if f != RAY_POSE}
+
+
+CONFIGS = [
+    dict(ch=32, az=700),
+    dict(ch=8, az=150, divergent=True),
+    dict(ch=16, az=400, distortion=True),
+]
+
+
+def pairs(rgl, scenes):
+    out = []
+    for k, cfg in enumerate(CONFIGS):
+        scene = scenes[k % 2]
+        out.append((Graph(rgl, scene, seed=k, with_pose=False, **cfg), Graph(rgl, scene, seed=k, with_pose=True, **cfg)))
+    return out
+
+
+def test_deferred_transform_equals_materialized(rgl_lib):
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    graphs = pairs(rgl, scenes)
+    for k, (deferred, materialized) in enumerate(graphs):
+        rgl.graph_run(deferred.rays)
+        rgl.graph_run(materialized.rays)
+        a, b = deferred.results(), materialized.results()
+        assert_same(a, b, f"solo config {k}")
+        assert int(a[IS_HIT].sum()) > 0, k
+    for frame in range(3):
+        rgl.graph_run_batch([g.rays for pair in graphs for g in pair])
+        for k, (deferred, materialized) in enumerate(graphs):
+            assert_same(deferred.results(), materialized.results(), f"batch frame {frame} config {k}")
+
+
+def test_ray_poses_follow_the_transform_when_requested(rgl_lib):
+    """With RAY_POSE requested the transform is materialized; the poses are the transform applied to the input
+    rays (float64 reference, within FP32 rounding of a 3x4 product)."""
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    lidar = S.spinning_lidar(8, 64, -10.0, 10.0)
+    pose = S.f32(S.trs((1.0, 2.0, 3.0), (10.0, 20.0, 30.0), (1, 1, 1)))
+    rays = rgl.node_rays_from_mat3x4f(rgl.Node(), lidar)
+    tf = rgl.node_rays_transform(rgl.Node(), pose)
+    rt = rgl.node_raytrace(rgl.Node(), scenes[1])
+    out = rgl.node_points_yield(rgl.Node(), [XYZ, RAY_POSE])
+    rgl.chain(rays, tf, rt, out)
+    rgl.graph_run(rays)
+    got = rgl.result(out, RAY_POSE).reshape(-1, 3, 4).astype(np.float64)
+    p = pose.astype(np.float64)
+    want = np.einsum("ij,njk->nik", p[:, :3], lidar.astype(np.float64))
+    want[:, :, 3] += p[:, 3]
+    np.testing.assert_allclose(got, want, rtol=0, atol=4e-6 * max(1.0, float(np.abs(want).max())))
