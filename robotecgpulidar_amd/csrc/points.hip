@@ -81,6 +81,46 @@ __device__ __forceinline__ void copyElem(const FieldCopy& f, int src, int dst)
 	}
 }
 
+// Copies the kept elements of one field of a tile: element base + k*kBlock goes to pos[k] when its bit is set.
+template<typename T, int K>
+__device__ __forceinline__ void scatterField(const FieldCopy& f, int base, const uint64_t* masks, const int* pos, int lane)
+{
+	const T* src = reinterpret_cast<const T*>(f.src);
+	T* dst = reinterpret_cast<T*>(f.dst);
+	T v[K];
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		if ((masks[k] >> lane) & 1ull) {
+			v[k] = src[base + k * kBlock];
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		if ((masks[k] >> lane) & 1ull) {
+			dst[pos[k]] = v[k];
+		}
+	}
+}
+
+// One 4-byte word of records of f.size bytes (f.src / f.dst point at the word inside the first record).
+template<int K>
+__device__ __forceinline__ void scatterStrided(const FieldCopy& f, int base, const uint64_t* masks, const int* pos, int lane)
+{
+	uint32_t v[K];
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		if ((masks[k] >> lane) & 1ull) {
+			v[k] = *reinterpret_cast<const uint32_t*>(f.src + static_cast<size_t>(base + k * kBlock) * f.size);
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		if ((masks[k] >> lane) & 1ull) {
+			*reinterpret_cast<uint32_t*>(f.dst + static_cast<size_t>(pos[k]) * f.size) = v[k];
+		}
+	}
+}
+
 // Offset of tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so each of
 // the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave) counts in
 // LDS, which preserves input order.
@@ -112,6 +152,7 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	const int base = tile * kBlock * kItems_ + threadIdx.x;
 	uint64_t masks[kItems_];
+#pragma unroll
 	for (int k = 0; k < kItems_; ++k) {
 		const int i = base + k * kBlock;
 		masks[k] = __ballot(i < n && flags[i] != 0);
@@ -131,13 +172,34 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 		}
 	}
 	__syncthreads();
-	const int fieldCount = fields.count;
+	int pos[kItems_];
+#pragma unroll
 	for (int k = 0; k < kItems_; ++k) {
-		if ((masks[k] >> lane) & 1ull) {
-			const int pos = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
-			for (int f = 0; f < fieldCount; ++f) {
-				copyElem(fields.f[f], base + k * kBlock, pos);
-			}
+		pos[k] = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
+	}
+	// Field by field (the size switch is uniform), all of a field's loads issued before its stores.
+	const int fieldCount = fields.count;
+	for (int f = 0; f < fieldCount; ++f) {
+		const FieldCopy fc = fields.f[f];
+		switch (fc.size) {
+			case 1: scatterField<uint8_t, kItems_>(fc, base, masks, pos, lane); break;
+			case 2: scatterField<uint16_t, kItems_>(fc, base, masks, pos, lane); break;
+			case 4: scatterField<uint32_t, kItems_>(fc, base, masks, pos, lane); break;
+			case 8: scatterField<uint64_t, kItems_>(fc, base, masks, pos, lane); break;
+			default:
+				if (fc.size % 4 == 0) {
+					// Wider records (vectors, ray poses) one 4-byte word at a time.
+					for (int w = 0; w < fc.size / 4; ++w) {
+						FieldCopy part{fc.src + 4 * w, fc.dst + 4 * w, fc.size, 0};
+						scatterStrided<kItems_>(part, base, masks, pos, lane);
+					}
+				} else {
+					for (int k = 0; k < kItems_; ++k) {
+						if ((masks[k] >> lane) & 1ull) {
+							copyElem(fc, base + k * kBlock, pos[k]);
+						}
+					}
+				}
 		}
 	}
 }
