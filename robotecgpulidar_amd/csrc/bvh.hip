@@ -95,7 +95,11 @@ __device__ __forceinline__ uint64_t expandBits21(uint32_t v)
 	return x;
 }
 
-__global__ void kMorton(int n, const Aabb* boxes, const uint32_t* bounds, uint64_t* keys, int32_t* vals)
+// Sort keys: 63-bit Morton code of the box centroid in the centroid bounds. With separateLarge, boxes whose diagonal
+// is at least half the diagonal of the centroid bounds (ground planes, terrain, anything spanning the scene) get
+// bit 63 set, so the first split of the tree separates them from the rest and their boxes do not inflate every
+// node on the path of their centroid through the spatial subdivision of the other primitives.
+__global__ void kMorton(int n, const Aabb* boxes, const uint32_t* bounds, uint64_t* keys, int32_t* vals, int separateLarge)
 {
 	const int i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n) {
@@ -114,7 +118,15 @@ __global__ void kMorton(int n, const Aabb* boxes, const uint32_t* bounds, uint64
 		t = fminf(fmaxf(t, 0.0f), 1.0f);
 		q[a] = static_cast<uint32_t>(fminf(t * 2097152.0f, 2097151.0f));
 	}
-	keys[i] = (expandBits21(q[0]) << 2) | (expandBits21(q[1]) << 1) | expandBits21(q[2]);
+	uint64_t key = (expandBits21(q[0]) << 2) | (expandBits21(q[1]) << 1) | expandBits21(q[2]);
+	if (separateLarge) {
+		const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+		const float sceneDiag2 = ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2];
+		if (dx * dx + dy * dy + dz * dz >= 0.25f * sceneDiag2) {
+			key |= 1ull << 63;
+		}
+	}
+	keys[i] = key;
 	vals[i] = i;
 }
 
@@ -701,7 +713,8 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 	hipLaunchKernelGGL(kInitBounds, dim3(1), dim3(64), 0, stream, bounds);
 	const int rb = std::min(blocksFor(n), 1024);
 	hipLaunchKernelGGL(kCentroidBounds, dim3(rb), dim3(kBlock), 0, stream, n, boxes, bounds);
-	hipLaunchKernelGGL(kMorton, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, boxes, bounds, keys, leafPrim);
+	hipLaunchKernelGGL(kMorton, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, boxes, bounds, keys, leafPrim,
+	                   leafRefIsPrim ? 1 : 0);
 
 	uint64_t* kIn = keys;
 	uint64_t* kOut = keysAlt;
