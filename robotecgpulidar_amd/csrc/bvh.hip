@@ -670,6 +670,173 @@ __global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags
 	static_cast<Bvh4QNode*>(out)[slot] = o;
 }
 
+// ---------------------------------------------------------------------------------------- PLOC topology
+// Agglomerative clustering (PLOC: Meister and Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume
+// Hierarchy Construction", IEEE TVCG 2018) over the Morton-sorted leaves, in one workgroup. Each round every
+// cluster finds the neighbour within kPlocRadius positions whose union box has the smallest surface area; mutual
+// nearest neighbours merge (the lower position creates the node and keeps the position), and the survivors are
+// compacted in order. The result is a binary topology in the LBVH convention (root = node 0, child refs >= 0 are
+// nodes, < 0 leaves; parentSlot[leaf] and parentSlot[n + node] = parent << 1 | slot); boxes and heights are filled by
+// the same bottom-up fit as the LBVH. Surface-area-driven merging keeps scene-spanning boxes near the root and
+// groups the rest by locality, which the Morton radix tree does only along its fixed splits.
+constexpr int kPlocBlock = 1024;
+constexpr int kPlocRadius = 16;
+constexpr int kPlocMax = 16384;
+
+__device__ __forceinline__ float halfArea(const Aabb& b)
+{
+	const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+	return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ Aabb unite(const Aabb& a, const Aabb& b)
+{
+	Aabb r;
+	for (int k = 0; k < 3; ++k) {
+		r.lo[k] = fminf(a.lo[k], b.lo[k]);
+		r.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+	}
+	return r;
+}
+
+// Exclusive scan of v over the workgroup's items [0, count) (each thread owns a contiguous chunk); returns the total.
+__device__ int plocScan(int32_t* v, int count, int32_t* lds)
+{
+	const int per = (count + kPlocBlock - 1) / kPlocBlock;
+	const int b = threadIdx.x * per, e = min(b + per, count);
+	int sum = 0;
+	for (int i = b; i < e; ++i) {
+		sum += v[i];
+	}
+	// Inclusive scan of the per-thread sums: within each wave by shuffles, then over the 16 wave totals.
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int incl = sum;
+	for (int off = 1; off < 64; off <<= 1) {
+		const int y = __shfl_up(incl, off, 64);
+		incl += lane >= off ? y : 0;
+	}
+	if (lane == 63) {
+		lds[wave] = incl;
+	}
+	__syncthreads();
+	if (threadIdx.x < 64) {
+		const int w = threadIdx.x < kPlocBlock / 64 ? lds[threadIdx.x] : 0;
+		int x = w;
+		for (int off = 1; off < 64; off <<= 1) {
+			const int y = __shfl_up(x, off, 64);
+			x += threadIdx.x >= off ? y : 0;
+		}
+		if (threadIdx.x < kPlocBlock / 64) {
+			lds[kPlocBlock / 64 + threadIdx.x] = x - w; // exclusive wave offsets
+		}
+		if (threadIdx.x == kPlocBlock / 64 - 1) {
+			lds[2 * (kPlocBlock / 64)] = x; // total
+		}
+	}
+	__syncthreads();
+	const int total = lds[2 * (kPlocBlock / 64)];
+	int run = lds[kPlocBlock / 64 + wave] + incl - sum;
+	for (int i = b; i < e; ++i) {
+		const int x = v[i];
+		v[i] = run;
+		run += x;
+	}
+	__syncthreads();
+	return total;
+}
+
+// scratch: box[2][n], ref[2][n], nn[n], merge[n], keep[n]. Leaf clusters carry ref = ~leafIndex.
+__global__ void __launch_bounds__(kPlocBlock) kPloc(int n, const Aabb* boxes, const int32_t* leafPrim, int leafRefIsPrim,
+                                                    BvhNode* nodes, int32_t* parentSlot, Aabb* box0, Aabb* box1,
+                                                    int32_t* ref0, int32_t* ref1, int32_t* nn, int32_t* merge,
+                                                    int32_t* keep)
+{
+	__shared__ int32_t lds[kPlocBlock];
+	for (int i = threadIdx.x; i < n; i += kPlocBlock) {
+		box0[i] = boxes[leafPrim[i]];
+		ref0[i] = ~i;
+	}
+	__syncthreads();
+	Aabb* cb = box0;
+	Aabb* nb = box1;
+	int32_t* cr = ref0;
+	int32_t* nr = ref1;
+	int count = n;
+	int next = n - 2; // index of the next node created (nodes are numbered downwards, so the root is node 0)
+	while (count > 1) {
+		for (int i = threadIdx.x; i < count; i += kPlocBlock) {
+			const Aabb bi = cb[i];
+			float best = INFINITY;
+			int arg = -1;
+			const int lo = max(0, i - kPlocRadius), hi = min(count - 1, i + kPlocRadius);
+			for (int j = lo; j <= hi; ++j) {
+				if (j == i) {
+					continue;
+				}
+				float a = halfArea(unite(bi, cb[j]));
+				a = isnan(a) ? INFINITY : a;
+				if (a < best || arg < 0) { // ties keep the lower position, so some pair is always mutual
+					best = a;
+					arg = j;
+				}
+			}
+			nn[i] = arg;
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < count; i += kPlocBlock) {
+			const int j = nn[i];
+			const bool mutual = nn[j] == i;
+			merge[i] = (mutual && i < j) ? 1 : 0;
+			keep[i] = (mutual && i > j) ? 0 : 1;
+		}
+		__syncthreads();
+		const int merges = plocScan(merge, count, lds);
+		const int kept = plocScan(keep, count, lds);
+		for (int i = threadIdx.x; i < count; i += kPlocBlock) {
+			const int j = nn[i];
+			const bool mutual = nn[j] == i;
+			if (mutual && i > j) {
+				continue; // merged into j's node
+			}
+			const int pos = keep[i];
+			if (mutual) {
+				const int node = next - merge[i];
+				const int c[2] = {cr[i], cr[j]};
+				for (int slot = 0; slot < 2; ++slot) {
+					const int code = (node << 1) | slot;
+					if (c[slot] < 0) {
+						const int leaf = ~c[slot];
+						parentSlot[leaf] = code;
+						nodes[node].child[slot] = ~(leafRefIsPrim ? leafPrim[leaf] : leaf);
+					} else {
+						parentSlot[n + c[slot]] = code;
+						nodes[node].child[slot] = c[slot];
+					}
+				}
+				nodes[node].pad[0] = 0;
+				nodes[node].pad[1] = 0;
+				nb[pos] = unite(cb[i], cb[j]);
+				nr[pos] = node;
+			} else {
+				nb[pos] = cb[i];
+				nr[pos] = cr[i];
+			}
+		}
+		__syncthreads();
+		next -= merges;
+		count = kept;
+		Aabb* tb = cb;
+		cb = nb;
+		nb = tb;
+		int32_t* tr = cr;
+		cr = nr;
+		nr = tr;
+	}
+	if (threadIdx.x == 0) {
+		parentSlot[n] = -1;
+	}
+}
+
 inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 } // namespace
@@ -682,7 +849,10 @@ size_t lbvhScratchBytes(int n)
 	bytes += 2 * sizeof(int32_t) * n;      // vals, valsAlt
 	bytes += sizeof(uint32_t) * 256 * tiles;
 	bytes += sizeof(uint32_t) * 8;
-	bytes += 4096;                          // alignment slack
+	if (n <= kPlocMax) {
+		bytes += (2 * sizeof(Aabb) + 5 * sizeof(int32_t)) * n; // PLOC clusters (two buffers), nn, merge, keep
+	}
+	bytes += 8192;                          // alignment slack
 	return bytes;
 }
 
@@ -730,8 +900,21 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 		std::swap(vIn, vOut);
 	}
 	// 8 passes: sorted data is back in (keys, leafPrim).
-	hipLaunchKernelGGL(kKarras, dim3(blocksFor(n - 1)), dim3(kBlock), 0, stream, n, keys, leafPrim,
-	                   leafRefIsPrim ? 1 : 0, nodes, parentSlot);
+	if (leafRefIsPrim && n <= kPlocMax) {
+		// Instance trees are small and decide every ray's first steps: agglomerative topology.
+		Aabb* b0 = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * n));
+		Aabb* b1 = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * n));
+		int32_t* r0 = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+		int32_t* r1 = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+		int32_t* nn = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+		int32_t* merge = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+		int32_t* keep = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+		hipLaunchKernelGGL(kPloc, dim3(1), dim3(kPlocBlock), 0, stream, n, boxes, leafPrim, 1, nodes, parentSlot, b0,
+		                   b1, r0, r1, nn, merge, keep);
+	} else {
+		hipLaunchKernelGGL(kKarras, dim3(blocksFor(n - 1)), dim3(kBlock), 0, stream, n, keys, leafPrim,
+		                   leafRefIsPrim ? 1 : 0, nodes, parentSlot);
+	}
 	refitLbvh(stream, n, boxes, leafPrim, parentSlot, nodes, counters, height);
 }
 

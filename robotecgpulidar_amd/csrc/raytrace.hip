@@ -30,9 +30,10 @@ namespace {
 #define RGL_RT_BLOCK 128
 #endif
 constexpr int kRtBlock = RGL_RT_BLOCK; // threads per raytrace block (the LDS short stack is laid out per block)
-// Occupancy target (waves per SIMD) of the single-sample raytrace kernels. The traversal is latency-bound, so more
-// waves in flight pay until register spills start: on gfx950, 5 waves (96 VGPRs) spill 16 B per lane and beat both
-// the compiler's own choice (4 waves, ~106 VGPRs) and 6 waves (80 VGPRs, 96 B of spills). 0 = compiler's choice.
+// Occupancy target (waves per SIMD) of the raytrace kernels. More waves in flight hide the dependent node loads
+// until register spills enter the traversal loop: on gfx950 the single-sample kernels fit 5 waves (95 VGPRs, no
+// spills) and are slower at 6 (80 VGPRs, spills in the loop); the divergent-beam kernels reload a few beam values
+// from scratch at 5 waves and still run ~7% faster than at the compiler's 4. 0 = compiler's choice.
 #ifndef RGL_RT_WAVES
 #define RGL_RT_WAVES 5
 #endif
@@ -875,8 +876,7 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
 }
 
-// Single-sample beams (no divergence), one per thread: occupancy-tuned. Divergent beams, one per wave, keep the
-// compiler's register allocation.
+// Single-sample beams (no divergence), one per thread; divergent beams, one per wave.
 template<int kFmt>
 __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
@@ -884,7 +884,7 @@ __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytrace(SceneView
 }
 
 template<int kFmt>
-__global__ void __launch_bounds__(kRtBlock) kRaytraceMulti(SceneView sc, RaytraceParams p, RaytraceOutputs out)
+__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceMulti(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
 	raytraceOne<kFmt, true>(sc, p, out, blockIdx.x);
 }
@@ -896,7 +896,7 @@ __global__ void __launch_bounds__(kBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const 
 }
 
 template<int kFmt>
-__global__ void __launch_bounds__(kRtBlock) kRaytraceBatchMulti(const RaytraceBatchItem* items, int itemCount)
+__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceBatchMulti(const RaytraceBatchItem* items, int itemCount)
 {
 	raytraceBatchOne<kFmt, true>(items, itemCount);
 }
