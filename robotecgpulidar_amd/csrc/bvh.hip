@@ -900,8 +900,8 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 		std::swap(vIn, vOut);
 	}
 	// 8 passes: sorted data is back in (keys, leafPrim).
-	if (leafRefIsPrim && n <= kPlocMax) {
-		// Instance trees are small and decide every ray's first steps: agglomerative topology.
+	if (n <= kPlocMax) {
+		// Small trees (every TLAS, most meshes): agglomerative topology, built in one workgroup.
 		Aabb* b0 = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * n));
 		Aabb* b1 = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * n));
 		int32_t* r0 = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
@@ -909,7 +909,7 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 		int32_t* nn = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
 		int32_t* merge = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
 		int32_t* keep = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
-		hipLaunchKernelGGL(kPloc, dim3(1), dim3(kPlocBlock), 0, stream, n, boxes, leafPrim, 1, nodes, parentSlot, b0,
+		hipLaunchKernelGGL(kPloc, dim3(1), dim3(kPlocBlock), 0, stream, n, boxes, leafPrim, leafRefIsPrim ? 1 : 0, nodes, parentSlot, b0,
 		                   b1, r0, r1, nn, merge, keep);
 	} else {
 		hipLaunchKernelGGL(kKarras, dim3(blocksFor(n - 1)), dim3(kBlock), 0, stream, n, keys, leafPrim,

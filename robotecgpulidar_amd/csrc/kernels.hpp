@@ -23,7 +23,7 @@ size_t lbvhScratchBytes(int n);
 // Builds an LBVH over n primitive boxes. nodes: max(n-1, 1) entries; leafPrim: n (primitive of each leaf in
 // leaf order); parentSlot: 2n entries; counters: max(n-1,1); height: max(n-1,1) (height[0] = tree depth).
 // When leafRefIsPrim, leaf references in nodes hold the primitive index (TLAS); otherwise the leaf index
-// (BLAS, whose triangles are gathered in leaf order). The TLAS (up to 16384 instances) takes its topology from
+// (BLAS, whose triangles are gathered in leaf order). Trees of up to 16384 leaves take their topology from
 // agglomerative clustering over the Morton order instead of the radix tree.
 void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim, BvhNode* nodes, int32_t* leafPrim,
                int32_t* parentSlot, int32_t* counters, int32_t* height, void* scratch);
