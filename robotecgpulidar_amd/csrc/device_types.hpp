@@ -179,6 +179,25 @@ struct RaytraceParams
 	int32_t velocityRequested;
 };
 
+#if defined(__HIPCC__)
+// Item of a batched launch that block `block` belongs to: the last item whose first block (key) is <= block; items
+// are sorted by first block and the first one starts at 0. Each lane of a wave tests one item and a ballot counts the
+// matches, so the lookup costs one round of loads per 64 items rather than a chain of dependent loads (which
+// dominated short blocks). The result is wave-uniform. Every lane of the wave must be active.
+template<typename Item, typename Key>
+__device__ __forceinline__ int batchItemOf(const Item* items, int itemCount, int block, Key key)
+{
+	const int lane = static_cast<int>(threadIdx.x & 63);
+	int count = 0;
+	for (int base = 0; base < itemCount; base += 64) {
+		const int k = base + lane;
+		const bool le = k < itemCount && key(items[k]) <= block;
+		count += __popcll(__ballot(le));
+	}
+	return __builtin_amdgcn_readfirstlane(count - 1);
+}
+#endif
+
 // ------------------------------------------------------------------------------------------------ points
 constexpr int kMaxFormatFields = 32;
 constexpr int kMaxCompactFields = 32;

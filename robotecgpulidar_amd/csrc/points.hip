@@ -218,16 +218,7 @@ __global__ void kCompactScatter(int capacity, const int32_t* countIn, const int3
 template<typename Item, typename Key>
 __device__ __forceinline__ int findItem(const Item* items, int itemCount, Key key)
 {
-	int lo = 0, hi = itemCount - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (key(items[mid]) <= static_cast<int>(blockIdx.x)) {
-			lo = mid;
-		} else {
-			hi = mid - 1;
-		}
-	}
-	return __builtin_amdgcn_readfirstlane(lo);
+	return batchItemOf(items, itemCount, static_cast<int>(blockIdx.x), key);
 }
 
 template<int kItems_>

@@ -510,16 +510,7 @@ __global__ void kRadarGather(RadarItem it)
 
 __device__ __forceinline__ int findRadarItem(const RadarItem* items, int count)
 {
-	int lo = 0, hi = count - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (items[mid].firstBlock <= static_cast<int>(blockIdx.x)) {
-			lo = mid;
-		} else {
-			hi = mid - 1;
-		}
-	}
-	return __builtin_amdgcn_readfirstlane(lo);
+	return batchItemOf(items, count, static_cast<int>(blockIdx.x), [](const RadarItem& r) { return r.firstBlock; });
 }
 
 __global__ void kRadarEnergyBatch(const RadarItem* items, int count)

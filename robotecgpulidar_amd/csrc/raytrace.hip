@@ -862,16 +862,8 @@ __device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceP
 template<int kFmt, bool kMulti, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items, int itemCount)
 {
-	int lo = 0, hi = itemCount - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (items[mid].firstBlock <= static_cast<int>(blockIdx.x)) {
-			lo = mid;
-		} else {
-			hi = mid - 1;
-		}
-	}
-	const int item = __builtin_amdgcn_readfirstlane(lo);
+	const int item = batchItemOf(items, itemCount, static_cast<int>(blockIdx.x),
+	                             [](const RaytraceBatchItem& r) { return r.firstBlock; });
 	const RaytraceBatchItem& it = items[item];
 	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
 }
