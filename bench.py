@@ -135,22 +135,27 @@ def latency_stats(lat_s):
     return round(float(np.percentile(ms, 50)), 4), round(float(np.percentile(ms, 99)), 4)
 
 
-def roofline(b_alg: int, b_io: int, kernel_ms: float, traffic):
+def roofline(b_alg: int, b_io: int, kernel_ms: float, profile):
     """`achieved` counts algorithmic bytes (canonical binary-BVH node, triangle and instance fetches per ray, each
     once, plus ray and point I/O): an effective bandwidth that caches can push above the HBM peak. `traffic` is the
-    DRAM bytes rocprof measured for the same kernel, and `dram_frac` the fraction of peak they represent."""
+    DRAM bytes rocprof measured for the same kernel, and `dram_frac` the fraction of peak they represent;
+    `valu_issue_busy` is the fraction of SIMD cycles spent issuing VALU instructions in the committed counter run
+    (the traversal's actual limiter)."""
+    traffic = profile.get("bytes_per_launch") if profile else None
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes_per_launch": b_alg,
            "io_bytes_per_launch": b_io, "io_only_frac": round(b_io / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
     if traffic:
         out["dram_frac"] = round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if profile and "valu_issue_busy" in profile:
+        out["valu_issue_busy"] = profile["valu_issue_busy"]
     return out
 
 
-def traffic_bytes(name: str):
+def kernel_profile(name: str):
     tf = ROOT / "profiles" / name
-    return json.loads(tf.read_text()).get("bytes_per_launch") if tf.exists() else None
+    return json.loads(tf.read_text()) if tf.exists() else None
 
 
 def run_batch64(args):
@@ -309,7 +314,7 @@ def run_batch64(args):
         "raytrace_launches_per_step": launches_per_frame,
         "traversal_per_ray": {"nodes": round(sums[0] / rays_per_frame, 2), "triangles": round(sums[1] / rays_per_frame, 2),
                               "instances": round(sums[2] / rays_per_frame, 2)},
-        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, traffic_bytes("raytrace_batch64_traffic.json")),
+        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, kernel_profile("raytrace_batch64_traffic.json")),
         "cpu_baseline": cpu,
     }
 
@@ -630,7 +635,7 @@ def run_urban(args):
         "raytrace_kernel_ms": round(kernel_ms, 4),
         "traversal_per_ray": {"nodes": round(sums[0] / n_rays, 2), "triangles": round(sums[1] / n_rays, 2),
                               "instances": round(sums[2] / n_rays, 2)},
-        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, traffic_bytes("raytrace_urban_traffic.json")),
+        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, kernel_profile("raytrace_urban_traffic.json")),
         "cpu_baseline": cpu_baseline(spec, lidar, args.cpu_budget, True, "the same scene and lidar")
         if args.cpu_budget > 0 else None,
     }
