@@ -48,10 +48,11 @@ constexpr int kPop = INT_MIN;
 constexpr int kBeamSamples = 58;  // 1 primary + 3 layers x 19 vertices (src/gpu/MultiReturn.hpp:19-21)
 constexpr int kBeamLayers = 3;
 constexpr int kBeamVertices = 19;
-// Slab-test widening: a computed slab distance (bound - origin) * invDir is within 3 units of roundoff
-// (u = 2^-24) of the exact one and keeps its sign, so comparing the computed near distance against the far
-// distance (or the closest hit) grown by 2^-20 = 16u, with the growth itself rounded (one more u), keeps every box
-// the exact test enters: (1 + 3u) / (1 - 3u) < (1 + 16u)(1 - u).
+// Slab-test widening: a computed slab distance (bound - origin) * invDir is within 4 units of roundoff
+// (u = 2^-24: the subtraction and the product round once each, the reciprocal is within one ulp) of the exact one
+// and keeps its sign, so comparing the computed near distance against the far distance (or the closest hit) grown
+// by 2^-20 = 16u, with the growth itself rounded (one more u), keeps every box the exact test enters:
+// (1 + 4u) / (1 - 4u) < (1 + 16u)(1 - u).
 constexpr float kSlabGrow = 1.0f + 9.5367431640625e-7f;
 
 // Component selection by a one-hot vector e: e.x*v.x + e.y*v.y + e.z*v.z is exactly the selected component (the
@@ -77,12 +78,13 @@ struct TravRay
 	OneHot3 ex, ey, ez;
 };
 
-// Per-axis reciprocal direction of the slab tests; a zero component is replaced by 1e-30.
+// Per-axis reciprocal direction of the slab tests; a zero component is replaced by 1e-30. The hardware reciprocal
+// (within one ulp) is used: the slab widening covers its error, and nothing but the slab tests reads it.
 __device__ __forceinline__ Vec3f invDir(Vec3f d)
 {
 	const float tiny = 1e-30f;
-	return Vec3f{1.0f / (d.x != 0.0f ? d.x : tiny), 1.0f / (d.y != 0.0f ? d.y : tiny),
-	             1.0f / (d.z != 0.0f ? d.z : tiny)};
+	return Vec3f{__builtin_amdgcn_rcpf(d.x != 0.0f ? d.x : tiny), __builtin_amdgcn_rcpf(d.y != 0.0f ? d.y : tiny),
+	             __builtin_amdgcn_rcpf(d.z != 0.0f ? d.z : tiny)};
 }
 
 __device__ __forceinline__ TravRay makeTravRay(Vec3f o, Vec3f d)
