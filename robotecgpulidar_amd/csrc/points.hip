@@ -222,7 +222,7 @@ __device__ __forceinline__ int findItem(const Item* items, int itemCount, Key ke
 }
 
 template<int kItems_>
-__global__ void kTileCountsBatch(const CompactItem* items, int itemCount)
+__global__ void kTileCountsBatch(const CompactItem* __restrict__ items, int itemCount)
 {
 	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
@@ -230,7 +230,7 @@ __global__ void kTileCountsBatch(const CompactItem* items, int itemCount)
 }
 
 template<int kItems_>
-__global__ void kCompactScatterBatch(const CompactItem* items, int itemCount)
+__global__ void kCompactScatterBatch(const CompactItem* __restrict__ items, int itemCount)
 {
 	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
@@ -392,7 +392,7 @@ __global__ void kAosToSoa(int n, int recordSize, const char* in, FieldCopyList f
 }
 
 // Elementwise stages of many graphs in one launch.
-__global__ void kPointOpsBatch(const PointOpItem* items, int itemCount)
+__global__ void kPointOpsBatch(const PointOpItem* __restrict__ items, int itemCount)
 {
 	const int k = findItem(items, itemCount, [](const PointOpItem& p) { return p.firstBlock; });
 	const PointOpItem& it = items[k];

@@ -513,7 +513,7 @@ __device__ __forceinline__ int findRadarItem(const RadarItem* items, int count)
 	return batchItemOf(items, count, static_cast<int>(blockIdx.x), [](const RadarItem& r) { return r.firstBlock; });
 }
 
-__global__ void kRadarEnergyBatch(const RadarItem* items, int count)
+__global__ void kRadarEnergyBatch(const RadarItem* __restrict__ items, int count)
 {
 	const RadarItem& it = items[findRadarItem(items, count)];
 	const int i = (static_cast<int>(blockIdx.x) - it.firstBlock) * kBlock + threadIdx.x;
@@ -522,9 +522,9 @@ __global__ void kRadarEnergyBatch(const RadarItem* items, int count)
 	}
 }
 
-__global__ __launch_bounds__(64) void kRadarClusterBatch(const RadarItem* items) { clusterOne(items[blockIdx.x]); }
+__global__ __launch_bounds__(64) void kRadarClusterBatch(const RadarItem* __restrict__ items) { clusterOne(items[blockIdx.x]); }
 
-__global__ void kRadarGatherBatch(const RadarItem* items, int count)
+__global__ void kRadarGatherBatch(const RadarItem* __restrict__ items, int count)
 {
 	const RadarItem& it = items[findRadarItem(items, count)];
 	const int k = (static_cast<int>(blockIdx.x) - it.firstBlock) * kBlock + threadIdx.x;

@@ -884,13 +884,13 @@ __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceMulti(Scen
 }
 
 template<int kFmt, int kBlock>
-__global__ void __launch_bounds__(kBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const RaytraceBatchItem* items, int itemCount)
+__global__ void __launch_bounds__(kBlock) RGL_RT_OCCUPANCY kRaytraceBatch(const RaytraceBatchItem* __restrict__ items, int itemCount)
 {
 	raytraceBatchOne<kFmt, false, kBlock>(items, itemCount);
 }
 
 template<int kFmt>
-__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceBatchMulti(const RaytraceBatchItem* items, int itemCount)
+__global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceBatchMulti(const RaytraceBatchItem* __restrict__ items, int itemCount)
 {
 	raytraceBatchOne<kFmt, true>(items, itemCount);
 }
