@@ -93,6 +93,36 @@ def test_bvh_traversal_equals_brute_force(rgl_lib, seed):
     np.testing.assert_array_equal(res[ENT][inst >= 0], inst[inst >= 0] + 1)
 
 
+def test_bvh_traversal_equals_brute_force_large_trees(rgl_lib):
+    """Trees above the agglomerative-build limit (16,384 leaves) take the radix-tree topology: a 20,000-triangle
+    ground spanning the scene (sort-key bit for scene-spanning instances) under 16,500 small instances. Traversal
+    must still equal brute force bit for bit."""
+    rng = np.random.default_rng(77)
+    spec = S.SceneSpec()
+    ground = spec.add_mesh(*S.grid_mesh(120.0, 120.0, 100, 100))
+    cube = spec.add_mesh(S.CUBE_VERTICES, S.CUBE_INDICES)
+    spec.add_instance(ground, S.translation(0, 0, -1.0), entity_id=1)
+    for k in range(16500):
+        spec.add_instance(cube, S.trs((rng.uniform(-55, 55), rng.uniform(-55, 55), rng.uniform(-0.5, 4)),
+                                      rng.uniform(0, 360, 3), rng.uniform(0.1, 0.8, 3)), entity_id=k + 2)
+    rays = S.make_lidar3d_rays(360, 90, 4, 2)
+    case = Case(spec, rays)
+    S.load_into_rgl(spec)
+    g = Graph(rgl_lib, case, [DIST, IS_HIT, ENT])
+    g.run()
+    n = len(rays)
+    t = np.empty(n, np.float32)
+    inst = np.empty(n, np.int32)
+    prim = np.empty(n, np.int32)
+    import ctypes as C
+    rgl_lib.check("rgl_node_raytrace_brute_force", g.raytrace.h, t.ctypes.data_as(C.POINTER(C.c_float)),
+                  inst.ctypes.data_as(C.POINTER(C.c_int32)), prim.ctypes.data_as(C.POINTER(C.c_int32)), n)
+    res = g.run()
+    assert (inst >= 0).sum() > n // 4
+    np.testing.assert_array_equal(res[IS_HIT] == 1, inst >= 0)
+    np.testing.assert_array_equal(res[ENT][inst >= 0], inst[inst >= 0] + 1)
+
+
 @pytest.mark.parametrize("count", [1, 100, 20000])
 def test_compaction_by_field_from_array(rgl_lib, count):
     """test/src/graph/nodes/CompactByFieldPointsNodeTest.cpp:86-177: random hits, exact stable output."""
