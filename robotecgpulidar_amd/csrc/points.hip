@@ -102,6 +102,20 @@ __device__ __forceinline__ void scatterField(const FieldCopy& f, int base, const
 	}
 }
 
+typedef unsigned Word3 __attribute__((ext_vector_type(3)));
+typedef unsigned Word2 __attribute__((ext_vector_type(2)));
+
+// 4-byte accesses through global-address-space pointers: global (not flat) loads and stores count only on the vector
+// memory counter, so reading the LDS-staged field list does not wait for the data loads in flight.
+__device__ __forceinline__ uint32_t loadGlobalWord(const char* p)
+{
+	return *(const __attribute__((address_space(1))) uint32_t*)(p);
+}
+__device__ __forceinline__ void storeGlobalWord(char* p, uint32_t v)
+{
+	*(__attribute__((address_space(1))) uint32_t*)(p) = v;
+}
+
 // One 4-byte word of records of f.size bytes (f.src / f.dst point at the word inside the first record).
 template<int K>
 __device__ __forceinline__ void scatterStrided(const FieldCopy& f, int base, const uint64_t* masks, const int* pos, int lane)
@@ -121,32 +135,26 @@ __device__ __forceinline__ void scatterStrided(const FieldCopy& f, int base, con
 	}
 }
 
-// Offset of tile = sum of the counts of all previous tiles. Element i = tile + k*256 + thread (k-major), so each of
+// Offset of tile = exclusive scan of the tile counts (kTileScan). Element i = tile + k*256 + thread (k-major), so each of
 // the kItems rounds reads and writes contiguous runs; ranks come from wave ballots and per-(round, wave) counts in
 // LDS, which preserves input order.
 template<int kItems_ = kItems>
-__device__ void compactScatter(int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fields,
+__device__ void compactScatter(int capacity, const int32_t* countIn, const int32_t* flags, const FieldCopyList& fieldsIn,
                                const int32_t* tileCounts, int tile, int numTiles, int32_t* countOut)
 {
-	__shared__ int32_t red[kBlock];
 	__shared__ int32_t waveBase[kItems_][kWaves];
+	// The field list, staged in LDS: read from global memory inside the copy loops, every descriptor access would
+	// wait for all outstanding data loads.
+	__shared__ FieldCopy lf[kMaxCompactFields];
+	__shared__ int32_t lfCount;
+	if (threadIdx.x < kMaxCompactFields && static_cast<int>(threadIdx.x) < fieldsIn.count) {
+		lf[threadIdx.x] = fieldsIn.f[threadIdx.x];
+	}
+	if (threadIdx.x == 0) {
+		lfCount = fieldsIn.count;
+	}
 	const int n = liveCount(capacity, countIn);
-	int s = 0;
-	for (int t = threadIdx.x; t < tile; t += kBlock) {
-		s += tileCounts[t];
-	}
-	red[threadIdx.x] = s;
-	__syncthreads();
-	for (int off = kBlock / 2; off > 0; off >>= 1) {
-		if (threadIdx.x < off) {
-			red[threadIdx.x] += red[threadIdx.x + off];
-		}
-		__syncthreads();
-	}
-	const int tileOffset = red[0];
-	if (tile == numTiles - 1 && threadIdx.x == 0) {
-		*countOut = tileOffset + tileCounts[tile];
-	}
+	const int tileOffset = tileCounts[tile]; // exclusive offsets, written by the tile scan
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -177,32 +185,140 @@ __device__ void compactScatter(int capacity, const int32_t* countIn, const int32
 	for (int k = 0; k < kItems_; ++k) {
 		pos[k] = waveBase[k][wave] + static_cast<int>(__popcll(masks[k] & below));
 	}
-	// Field by field (the size switch is uniform), all of a field's loads issued before its stores.
-	const int fieldCount = fields.count;
-	for (int f = 0; f < fieldCount; ++f) {
-		const FieldCopy fc = fields.f[f];
-		switch (fc.size) {
-			case 1: scatterField<uint8_t, kItems_>(fc, base, masks, pos, lane); break;
-			case 2: scatterField<uint16_t, kItems_>(fc, base, masks, pos, lane); break;
-			case 4: scatterField<uint32_t, kItems_>(fc, base, masks, pos, lane); break;
-			case 8: scatterField<uint64_t, kItems_>(fc, base, masks, pos, lane); break;
-			default:
-				if (fc.size % 4 == 0) {
-					// Wider records (vectors, ray poses) one 4-byte word at a time.
-					for (int w = 0; w < fc.size / 4; ++w) {
-						FieldCopy part{fc.src + 4 * w, fc.dst + 4 * w, fc.size, 0};
-						scatterStrided<kItems_>(part, base, masks, pos, lane);
-					}
+	// Fields whose size is a multiple of 4 B move as a sequence of streams of up to three words per element (a 12 B
+	// vector is one stream of three-word accesses, a 48 B ray pose four), with the loads of the next stream issued
+	// before the stores of the current one, so two streams are always in flight.
+	const int fieldCount = lfCount;
+	const FieldCopy* fields = lf;
+	auto normalize = [&](int& f, int& w) {
+		while (f < fieldCount && (fields[f].size % 4 != 0 || 4 * w >= fields[f].size)) {
+			++f;
+			w = 0;
+		}
+	};
+	auto chunkOf = [&](int f, int w) { return min(3, fields[f].size / 4 - w); };
+	auto advance = [&](int& f, int& w) {
+		w += chunkOf(f, w);
+		normalize(f, w);
+	};
+	// Loads are unconditional (clamped to the live range; unkept elements share cache lines with kept ones), so the
+	// compiler knows how many are in flight and the stores of one stream wait only for that stream's loads.
+	auto loadChunk = [&](int f, int w, Word3* v) {
+		const FieldCopy& fc = fields[f];
+		const int c = chunkOf(f, w);
+#pragma unroll
+		for (int k = 0; k < kItems_; ++k) {
+			const char* p = fc.src + static_cast<size_t>(min(base + k * kBlock, n - 1)) * fc.size + 4 * w;
+			if (c == 3) {
+				v[k] = *(const __attribute__((address_space(1))) Word3*)(p);
+			} else if (c == 2) {
+				const Word2 x = *(const __attribute__((address_space(1))) Word2*)(p);
+				v[k] = Word3{x.x, x.y, 0u};
+			} else {
+				v[k] = Word3{loadGlobalWord(p), 0u, 0u};
+			}
+		}
+	};
+	auto storeChunk = [&](int f, int w, const Word3* v) {
+		const FieldCopy& fc = fields[f];
+		const int c = chunkOf(f, w);
+#pragma unroll
+		for (int k = 0; k < kItems_; ++k) {
+			if ((masks[k] >> lane) & 1ull) {
+				char* p = fc.dst + static_cast<size_t>(pos[k]) * fc.size + 4 * w;
+				if (c == 3) {
+					*(__attribute__((address_space(1))) Word3*)(p) = v[k];
+				} else if (c == 2) {
+					*(__attribute__((address_space(1))) Word2*)(p) = Word2{v[k].x, v[k].y};
 				} else {
-					for (int k = 0; k < kItems_; ++k) {
-						if ((masks[k] >> lane) & 1ull) {
-							copyElem(fc, base + k * kBlock, pos[k]);
-						}
-					}
+					storeGlobalWord(p, v[k].x);
 				}
+			}
+		}
+	};
+	int f = 0, w = 0;
+	normalize(f, w);
+	if (n <= 0) {
+		f = fieldCount; // nothing live (no loads: the clamped index would be -1)
+	}
+	// Two register buffers used alternately (no copies between them, which would wait for the loads in flight).
+	Word3 va[kItems_], vb[kItems_];
+	if (f < fieldCount) {
+		loadChunk(f, w, va);
+	}
+	while (f < fieldCount) {
+		int f2 = f, w2 = w;
+		advance(f2, w2);
+		if (f2 < fieldCount) {
+			loadChunk(f2, w2, vb);
+		}
+		storeChunk(f, w, va);
+		if (f2 >= fieldCount) {
+			break;
+		}
+		int f3 = f2, w3 = w2;
+		advance(f3, w3);
+		if (f3 < fieldCount) {
+			loadChunk(f3, w3, va);
+		}
+		storeChunk(f2, w2, vb);
+		f = f3;
+		w = w3;
+	}
+	// 1- and 2-byte fields (and any other size) element by element.
+	for (int g = 0; g < fieldCount; ++g) {
+		const FieldCopy fc = fields[g];
+		if (fc.size == 1) {
+			scatterField<uint8_t, kItems_>(fc, base, masks, pos, lane);
+		} else if (fc.size == 2) {
+			scatterField<uint16_t, kItems_>(fc, base, masks, pos, lane);
+		} else if (fc.size % 4 != 0) {
+			for (int k = 0; k < kItems_; ++k) {
+				if ((masks[k] >> lane) & 1ull) {
+					copyElem(fc, base + k * kBlock, pos[k]);
+				}
+			}
 		}
 	}
 }
+
+// Exclusive scan of a graph's tile counts in place (tile counts become tile offsets) and its kept count: one block
+// per graph, 256 tiles per round, shuffle scans within waves.
+__device__ void tileScan(int32_t* counts, int numTiles, int32_t* countOut)
+{
+	__shared__ int32_t waveSums[kWaves];
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	int carry = 0;
+	for (int base = 0; base < numTiles; base += kBlock) {
+		const int i = base + threadIdx.x;
+		const int v = i < numTiles ? counts[i] : 0;
+		int x = v;
+		for (int off = 1; off < 64; off <<= 1) {
+			const int y = __shfl_up(x, off, 64);
+			x += lane >= off ? y : 0;
+		}
+		if (lane == 63) {
+			waveSums[wave] = x;
+		}
+		__syncthreads();
+		int before = 0, total = 0;
+		for (int w = 0; w < kWaves; ++w) {
+			before += w < wave ? waveSums[w] : 0;
+			total += waveSums[w];
+		}
+		if (i < numTiles) {
+			counts[i] = carry + before + x - v;
+		}
+		carry += total;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		*countOut = carry;
+	}
+}
+
+__global__ void kTileScan(int32_t* counts, int numTiles, int32_t* countOut) { tileScan(counts, numTiles, countOut); }
 
 __global__ void kTileCounts(int capacity, const int32_t* countIn, const int32_t* flags, int32_t* tileCounts)
 {
@@ -227,6 +343,12 @@ __global__ void kTileCountsBatch(const CompactItem* __restrict__ items, int item
 	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
 	tileCount<kItems_>(it.capacity, it.countIn, it.flags, static_cast<int>(blockIdx.x) - it.firstTile, it.tileCounts);
+}
+
+__global__ void kTileScanBatch(const CompactItem* __restrict__ items)
+{
+	const CompactItem& it = items[blockIdx.x];
+	tileScan(it.tileCounts, it.numTiles, it.countOut);
 }
 
 template<int kItems_>
@@ -495,6 +617,7 @@ void compact(hipStream_t stream, int capacity, const int32_t* countIn, const int
 	}
 	const int tiles = compactTiles(capacity);
 	hipLaunchKernelGGL(kTileCounts, dim3(tiles), dim3(kBlock), 0, stream, capacity, countIn, flags, tileCounts);
+	hipLaunchKernelGGL(kTileScan, dim3(1), dim3(kBlock), 0, stream, tileCounts, tiles, countOut);
 	hipLaunchKernelGGL(kCompactScatter, dim3(tiles), dim3(kBlock), 0, stream, capacity, countIn, flags, fields, tileCounts,
 	                   tiles, countOut);
 }
@@ -506,9 +629,11 @@ void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, i
 	}
 	if (variant == 1) {
 		hipLaunchKernelGGL(kTileCountsBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kTileScanBatch, dim3(itemCount), dim3(kBlock), 0, stream, items);
 		hipLaunchKernelGGL(kCompactScatterBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	} else {
 		hipLaunchKernelGGL(kTileCountsBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kTileScanBatch, dim3(itemCount), dim3(kBlock), 0, stream, items);
 		hipLaunchKernelGGL(kCompactScatterBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	}
 }
