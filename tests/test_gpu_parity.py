@@ -141,6 +141,31 @@ def test_compaction_by_field_from_array(rgl_lib, count):
     np.testing.assert_array_equal(rgl_lib.result(comp, 5), rec["ground"][keep])
 
 
+@pytest.mark.parametrize("count", [5000, 700001])
+def test_compaction_mixed_field_sizes(rgl_lib, count):
+    """Stable compaction of 12-, 8-, 4-, 2- and 1-byte fields; 700,001 points span more than 256 tiles (several rounds
+    of the tile-offset scan and a partial last tile)."""
+    rng = np.random.default_rng(count)
+    rec = np.zeros(count, dtype=[("xyz", np.float32, 3), ("hit", np.int32), ("ring", np.uint16), ("rt", np.uint8),
+                                 ("ts", np.float64)])
+    rec["xyz"] = rng.normal(size=(count, 3))
+    rec["hit"] = rng.integers(0, 2, count) * rng.integers(1, 5, count)
+    rec["ring"] = rng.integers(0, 65535, count)
+    rec["rt"] = rng.integers(0, 255, count)
+    rec["ts"] = rng.normal(size=count)
+    fields = [XYZ, IS_HIT, 11, 12, 13]  # RING_ID_U16, RETURN_TYPE_U8, TIME_STAMP_F64
+    src = rgl_lib.node_points_from_array(rgl_lib.Node(), rec, fields)
+    comp = rgl_lib.node_points_compact_by_field(rgl_lib.Node(), IS_HIT)
+    rgl_lib.add_child(src, comp)
+    rgl_lib.graph_run(src)
+    keep = rec["hit"] != 0
+    np.testing.assert_array_equal(rgl_lib.result(comp, XYZ), rec["xyz"][keep])
+    np.testing.assert_array_equal(rgl_lib.result(comp, IS_HIT), rec["hit"][keep])
+    np.testing.assert_array_equal(rgl_lib.result(comp, 11), rec["ring"][keep])
+    np.testing.assert_array_equal(rgl_lib.result(comp, 12), rec["rt"][keep])
+    np.testing.assert_array_equal(rgl_lib.result(comp, 13), rec["ts"][keep])
+
+
 def test_compaction_all_non_hit_gives_empty_cloud(rgl_lib):
     rec = np.zeros(100, dtype=[("xyz", np.float32, 3), ("hit", np.int32)])
     src = rgl_lib.node_points_from_array(rgl_lib.Node(), rec, [XYZ, IS_HIT])
