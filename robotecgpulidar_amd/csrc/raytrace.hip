@@ -287,43 +287,50 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 	} while (0)
 
 	const Vec3f worldInv = invDir(d);
-	TravRay obj;  // object-space ray of the entered instance (valid while inst >= 0)
+	// Ray of the node steps at the current level (world in the TLAS, object space in a BLAS) and the object-space
+	// ray record of the entered instance; both change only at the top of the node loop (instance entry, and the
+	// return to the TLAS after the pops left the instance).
+	Vec3f ro = o, ri = worldInv;
+	TravRay obj;
 	obj.o = o;
 	obj.inv = worldInv;
 	obj.sx = obj.sy = obj.sz = 0.0f;
 	obj.ex = obj.ey = obj.ez = oneHot(0);
 	const void* tlas = kStats ? static_cast<const void*>(sc.tlas2) : sc.tlasNodes;
-	const void* blas = tlas;
+	const void* nodes = tlas;
 	const LeafTri* tris = nullptr;
 	int inst = -1;
 	int instBase = -1;
+	bool inBlas = false;
 	bool found = false;
+	float tBound = slabBound(best.t);
 	int node = 0;
 	while (true) {
 		if (node < 0) {
-			// An instance leaf of the TLAS: continue at the root of its BLAS with the object-space ray. Entry is
-			// handled here, outside the pop loop, so the ray record is only written at this point of the loop.
+			// An instance leaf of the TLAS: continue at the root of its BLAS with the object-space ray.
 			inst = ~node;
 			instBase = sp;
 			const InstanceRec& ir = sc.instances[inst];
 			obj = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
-			blas = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
+			ro = obj.o;
+			ri = obj.inv;
+			nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
 			tris = ir.blasTris;
+			inBlas = true;
 			if (kStats) {
 				st->insts++;
 			}
 			node = 0;
+		} else if (inBlas && inst < 0) {
+			ro = o;
+			ri = worldInv;
+			nodes = tlas;
+			inBlas = false;
 		}
 		if (kStats) {
 			st->nodes++;
 		}
 		int next;
-		const float tBound = slabBound(best.t);
-		const bool inTlas = inst < 0;
-		const Vec3f ro{inTlas ? o.x : obj.o.x, inTlas ? o.y : obj.o.y, inTlas ? o.z : obj.o.z};
-		const Vec3f ri{inTlas ? worldInv.x : obj.inv.x, inTlas ? worldInv.y : obj.inv.y,
-		               inTlas ? worldInv.z : obj.inv.z};
-		const void* nodes = inTlas ? tlas : blas;
 		if constexpr (kFmt == kBvh2) {
 			GlobalFloat4* np = globalPtr<GlobalFloat4>(static_cast<const BvhNode*>(nodes) + node);
 			const RglF4 q0 = np[0], q1 = np[1], q2 = np[2];
@@ -461,6 +468,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 					best.leaf = leaf;
 					best.u = vNum / det;
 					best.v = wNum / det;
+					tBound = slabBound(t);
 					found = true;
 				}
 				next = kPop;
