@@ -239,9 +239,9 @@ __device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<f
 // traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted.
 //
 // One stack serves both levels. Entering an instance records the stack depth (instBase); when pops bring the depth
-// back to it, the BLAS is finished and traversal continues in the TLAS. The world-space ray is never copied: node
-// steps select the world or the object-space origin and reciprocal direction by the current level, so the loop
-// carries one ray record that changes only on instance entry.
+// back to it, the BLAS is finished and traversal continues in the TLAS. The ray of the node steps (origin and
+// reciprocal direction of the current level) and the object-space ray record change only at the top of the node
+// loop, on instance entry and on the return to the TLAS, so the pop loop carries no ray state.
 template<bool kStats, int kFmt, int kBlock = kRtBlock>
 __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
 {
