@@ -69,7 +69,7 @@ RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable);
 RGL_API rgl_status_t rgl_graph_batch_force_variants(int32_t raytrace_variant, int32_t compact_variant);
 RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant, int32_t* out_compact_variant);
 
-// HIP graph capture of rgl_graph_run_batch (on by default): a batch whose kernel sequence (item counts per stage,
+// HIP graph capture of rgl_graph_run_batch (off by default; worth enabling when batches are small and launch-bound): a batch whose kernel sequence (item counts per stage,
 // grid shapes, launch variants, descriptor buffers) equals the previous batch's is captured into a HIP graph, and
 // later batches with that same sequence replay it in one launch; descriptor contents (node buffers, scene
 // pointers, parameters) may change freely between replays. Batches that are being timed or tuned, or that contain

@@ -236,7 +236,8 @@ struct GraphBatch
 	void setAutotune(bool enable);
 	// Disables autotuning and uses the given variants.
 	void forceVariants(int raytraceVariant, int compactVariant);
-	// Capture of repeated launch sequences into HIP graphs (on by default); counters of captures and replays.
+	// Capture of repeated launch sequences into HIP graphs (off by default: a batch is a handful of launches, and on the
+	// 64-lidar frame replay measured no faster than launching them one by one); counters of captures and replays.
 	void setGraphs(bool enable);
 	int64_t graphCaptures = 0, graphReplays = 0;
 	void autotuneChoice(int32_t* raytraceVariant, int32_t* compactVariant) const;
@@ -292,7 +293,7 @@ private:
 		void release();
 	};
 	static constexpr size_t kCapturedGraphs = 4, kRecentSigs = 8;
-	bool graphs = true;
+	bool graphs = false;
 	std::vector<CapturedGraph> captured;
 	std::vector<std::vector<int64_t>> recentSigs;
 	uint64_t graphClock = 0;

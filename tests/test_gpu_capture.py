@@ -16,7 +16,7 @@ def lib(rgl_lib):
     rgl_lib.check("rgl_graph_batch_force_variants", 0, 0)  # no tuning launches, so every batch may be captured
     rgl_lib.batch_capture(True)
     yield rgl_lib
-    rgl_lib.batch_capture(True)
+    rgl_lib.batch_capture(False)
     rgl_lib.check("rgl_graph_batch_configure_autotune", True)
 
 
