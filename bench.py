@@ -208,6 +208,9 @@ def run_batch64(args):
     rgl.batch_capture(False)
     # Kernel durations from HIP events, in separate frames so the timed frames carry no extra events.
     kframes = 3
+    f0 = rgl.batch_fused_stages()
+    batched_frame()
+    fused_per_frame = rgl.batch_fused_stages() - f0
     for g in graphs:
         rgl.check("rgl_node_raytrace_set_kernel_timing", g[1].h, True)
     timed(plain_frame, kframes)
@@ -274,6 +277,10 @@ def run_batch64(args):
         b_io += io
         b_trav += trav
         sums += np.array(s3, np.int64)
+    # The batched launch also computes the points transform (writes XYZ, 12 B) and the distance noise (writes XYZ and
+    # DISTANCE, 16 B) of every sensor; the XYZ and DISTANCE they start from are the launch's own writes.
+    assert fused_per_frame == 2 * len(graphs), fused_per_frame
+    b_io_batched = b_io + rays_per_frame * (12 + 16)
     value = args.steps * rays_per_frame / b_total / 1e6
     plain_value = args.steps * rays_per_frame / p_total / 1e6
     p50, p99 = latency_stats(b_lat)
@@ -318,10 +325,12 @@ def run_batch64(args):
                              "mode": "same batches replayed from captured HIP graphs (rgl_graph_batch_configure_capture)"},
         "with_readback": readback,
         "raytrace_kernel_ms": round(kernel_ms, 4),
+        "point_stages_folded_per_step": fused_per_frame,
         "raytrace_launches_per_step": launches_per_frame,
         "traversal_per_ray": {"nodes": round(sums[0] / rays_per_frame, 2), "triangles": round(sums[1] / rays_per_frame, 2),
                               "instances": round(sums[2] / rays_per_frame, 2)},
-        "roofline": roofline(b_io + b_trav, b_io, kernel_ms, kernel_profile("raytrace_batch64_traffic.json")),
+        "roofline": roofline(b_io_batched + b_trav, b_io_batched, kernel_ms,
+                             kernel_profile("raytrace_batch64_traffic.json")),
         "cpu_baseline": cpu,
     }
 

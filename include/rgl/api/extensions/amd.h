@@ -77,6 +77,12 @@ RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant,
 RGL_API rgl_status_t rgl_graph_batch_configure_capture(bool enable);
 RGL_API rgl_status_t rgl_graph_batch_get_capture_stats(int64_t* out_captures, int64_t* out_replays);
 
+// Number of elementwise point stages (points transform, distance noise) computed inside a batched raytrace launch
+// instead of a launch of their own, since the library was loaded. rgl_graph_run_batch folds into the launch of a
+// single-sample raytrace node the chain of such nodes that starts at its only output and continues through nodes
+// with one output (at most two stages); each folded node still writes its own results.
+RGL_API rgl_status_t rgl_graph_batch_get_fused_stages(int64_t* out_stages);
+
 // Enables (and resets) HIP-event timing of the batched raytrace launches.
 RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable);
 

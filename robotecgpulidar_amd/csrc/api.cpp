@@ -1151,6 +1151,14 @@ RGL_API rgl_status_t rgl_graph_batch_get_capture_stats(int64_t* out_captures, in
 	});
 }
 
+RGL_API rgl_status_t rgl_graph_batch_get_fused_stages(int64_t* out_stages)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out_stages != nullptr);
+		*out_stages = GraphBatch::instance().fusedStages;
+	});
+}
+
 RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable)
 {
 	return safeCall([&]() { GraphBatch::instance().setAutotune(enable); });

@@ -101,6 +101,43 @@ std::shared_ptr<Event> GraphBatch::completionEvent()
 
 GraphBatch::GraphBatch() = default;
 
+// Folds the elementwise stages that directly follow a single-sample raytrace node into its batched launch: a chain
+// of points-transform and distance-noise nodes starting at the raytrace node's only output, continued through nodes
+// that have a single output, at most two stages. Every folded node still writes its own output buffers (the results
+// of each node stay readable), so what is saved is the re-read of XYZ and DISTANCE and one launch per stage; the
+// arithmetic is the shared elementwise code, so results are bit-identical to separate launches.
+void GraphBatch::fusePointStages(RaytraceNode* rt, RaytraceBatchItem& it)
+{
+	it.fusedCount = 0;
+	const void* xyz = it.out.xyz;
+	const void* distance = it.out.distance;
+	const int points = it.p.rayCount * it.p.returnCount;
+	Node* prev = rt;
+	while (it.fusedCount < 2 && prev->getOutputs().size() == 1) {
+		Node* n = prev->getOutputs()[0].get();
+		const bool transform = dynamic_cast<TransformPointsNode*>(n) != nullptr;
+		const bool noise = dynamic_cast<GaussianNoiseDistanceNode*>(n) != nullptr;
+		if ((!transform && !noise) || n->getInputs().size() != 1 || xyz == nullptr || (noise && distance == nullptr)) {
+			break;
+		}
+		PointOpItem& op = it.fused[it.fusedCount];
+		op = PointOpItem{};
+		FieldCopyList unused;
+		n->pointOp(op, unused);
+		if (op.in0 != xyz || (noise && op.in1 != distance) || op.count != nullptr || op.capacity != points) {
+			throw std::logic_error("batch: a folded point stage does not read the raytrace outputs");
+		}
+		++it.fusedCount;
+		++fusedStages;
+		n->batchFused = true;
+		xyz = op.out0;
+		if (noise) {
+			distance = op.out1;
+		}
+		prev = n;
+	}
+}
+
 namespace {
 constexpr double kTuneMinUnits = 100000.0;
 constexpr int kTuneSamples = 3;
@@ -317,6 +354,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	for (auto& c : pl.ctxs) {
 		for (auto& n : c->validatedOrder()) {
 			n->batchLevel = static_cast<int32_t>(n->getInputs().size()); // inputs not yet scheduled
+			n->batchFused = false;
 			++nodeCount;
 		}
 	}
@@ -347,6 +385,10 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			return L;
 		};
 		auto makeReady = [&](Node* n) {
+			if (n->batchFused) {
+				pl.passive.push_back(n); // its work runs inside its raytrace node's launch
+				return;
+			}
 			switch (n->batchStage()) {
 				case BatchStage::Passive:
 				case BatchStage::Other: pl.passive.push_back(n); break;
@@ -406,6 +448,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 							L.rtm.push_back(it);
 							L.rtmNodes.push_back(rt);
 						} else {
+							fusePointStages(rt, it);
 							it.firstBlock = L.rtBlocks;
 							L.rtBlocks += gpu::raytraceBlocks(it.p.rayCount, false, rtVariant);
 							L.rt.push_back(it);

@@ -279,15 +279,6 @@ constexpr int32_t kRadarErrTooManyClusters = 2;
 constexpr double kRcsFixedScale = 16777216.0;
 
 // ------------------------------------------------------------------------------------------------ batches
-// One sensor of a batched raytrace launch; blocks [firstBlock, next item's firstBlock) trace its beams.
-struct RaytraceBatchItem
-{
-	SceneView sc;
-	RaytraceParams p;
-	RaytraceOutputs out;
-	int32_t firstBlock;
-};
-
 // Elementwise point / ray stage of one graph inside a batched launch.
 enum PointOpKind : int32_t
 {
@@ -318,6 +309,20 @@ struct PointOpItem
 	const FieldCopyList* format; // kOpFormat / kOpToHost field list (device memory)
 	int32_t recordSize;
 	int32_t firstBlock;
+};
+
+// One sensor of a batched raytrace launch; blocks [firstBlock, next item's firstBlock) trace its beams.
+// fused[0..fusedCount) are elementwise stages of the same graph folded into the launch (points transform, distance
+// noise; single-sample beams only): after a beam's return points are written, stage 0 reads their XYZ and DISTANCE
+// back and every later stage takes the previous one's, each writing its own outputs as its own launch would.
+struct RaytraceBatchItem
+{
+	SceneView sc;
+	RaytraceParams p;
+	RaytraceOutputs out;
+	int32_t firstBlock;
+	int32_t fusedCount;
+	PointOpItem fused[2];
 };
 
 // Compaction of one graph inside a batched launch; tiles [firstTile, firstTile + numTiles).

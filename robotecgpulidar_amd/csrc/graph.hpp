@@ -89,6 +89,7 @@ protected:
 	std::shared_ptr<GraphRunCtx> ctx;
 	std::shared_ptr<Event> done; // shared by every node of a batch
 	int32_t batchLevel = 0;      // longest path from an entry node (batched runs)
+	bool batchFused = false;     // computed inside its raytrace node's batched launch this batch (no launch of its own)
 	friend struct GraphRunCtx;
 	friend struct GraphBatch;
 };
@@ -211,6 +212,8 @@ private:
 	hipStream_t batchDoneStream = nullptr;
 };
 
+struct RaytraceNode;
+
 // Runs several independent graphs as one batch: stages of the same kind at the same depth of every graph execute in
 // one launch (raytrace, elementwise point/ray stages, compaction), on one stream.
 struct GraphBatch
@@ -240,6 +243,7 @@ struct GraphBatch
 	// 64-lidar frame replay measured no faster than launching them one by one); counters of captures and replays.
 	void setGraphs(bool enable);
 	int64_t graphCaptures = 0, graphReplays = 0;
+	int64_t fusedStages = 0; // point stages folded into batched raytrace launches
 	void autotuneChoice(int32_t* raytraceVariant, int32_t* compactVariant) const;
 
 private:
@@ -260,6 +264,7 @@ private:
 		double units;
 	};
 	int pick(const Tuner& t) const;
+	void fusePointStages(RaytraceNode* rt, RaytraceBatchItem& it);
 	void harvest(bool wait);
 	void timeStart(hipStream_t s, PendingTiming& p);
 	std::shared_ptr<PinnedBuffer> hostSlots(size_t bytes);

@@ -17,6 +17,7 @@
 #include "device_types.hpp"
 #include "kernels.hpp"
 #include "noise_rng.hpp"
+#include "point_ops.hpp"
 
 namespace rgl::gpu {
 
@@ -363,7 +364,7 @@ __global__ void kCompactScatterBatch(const CompactItem* __restrict__ items, int 
 // ------------------------------------------------------------------------------------------------ element ops
 __device__ __forceinline__ void opTransformPoints(int i, const Vec3f* in, Vec3f* out, const Mat3x4f& tf)
 {
-	out[i] = tf * in[i];
+	out[i] = pointTransform(tf, in[i]);
 }
 
 __device__ __forceinline__ void opTransformRays(int i, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf)
@@ -399,11 +400,12 @@ __device__ __forceinline__ void opNoiseDistance(int i, const Vec3f* inXyz, const
                                                 const Mat3x4f& lookAt, const Mat3x4f& lookAtInv, uint64_t seed,
                                                 uint32_t run, uint32_t tag)
 {
-	const float sigma = inDistance[i] * stDevRise + stDevBase;
-	const float err = mean + rngNormal(seed, run, tag, static_cast<uint32_t>(i)) * sigma;
-	const Vec3f pr = lookAt * inXyz[i];
-	outXyz[i] = lookAtInv * (pr + normalized(pr) * err);
-	outDistance[i] = inDistance[i] + err;
+	Vec3f xyz;
+	float distance;
+	pointNoiseDistance(inXyz[i], inDistance[i], mean, stDevBase, stDevRise, lookAt, lookAtInv, seed, run, tag,
+	                   static_cast<uint32_t>(i), xyz, distance);
+	outXyz[i] = xyz;
+	outDistance[i] = distance;
 }
 
 __device__ __forceinline__ void opSoaToAos(int i, int recordSize, const FieldCopyList& fields, char* out)

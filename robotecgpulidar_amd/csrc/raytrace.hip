@@ -21,6 +21,7 @@
 
 #include "device_types.hpp"
 #include "kernels.hpp"
+#include "point_ops.hpp"
 
 namespace rgl::gpu {
 
@@ -867,6 +868,30 @@ __device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceP
 	}
 }
 
+// The point stages folded into a batched launch, for the return points of one beam. The thread reads back the XYZ
+// and DISTANCE it has just written (its own stores) and carries them through the stages in registers.
+__device__ __forceinline__ void applyFusedStages(const RaytraceBatchItem& it, int beam)
+{
+	const int R = it.p.returnCount;
+	for (int r = 0; r < R; ++r) {
+		const int i = beam * R + r;
+		Vec3f xyz = it.out.xyz[i];
+		float distance = it.out.distance != nullptr ? it.out.distance[i] : 0.0f;
+		for (int k = 0; k < it.fusedCount; ++k) {
+			const PointOpItem& op = it.fused[k];
+			if (op.op == kOpTransformPoints) {
+				xyz = pointTransform(op.m0, xyz);
+				static_cast<Vec3f*>(op.out0)[i] = xyz;
+			} else {
+				pointNoiseDistance(xyz, distance, op.a, op.b, op.c, op.m0, op.m1, op.seed, op.run, op.tag,
+				                   static_cast<uint32_t>(i), xyz, distance);
+				static_cast<Vec3f*>(op.out0)[i] = xyz;
+				static_cast<float*>(op.out1)[i] = distance;
+			}
+		}
+	}
+}
+
 // Many sensors (possibly on different scenes) in one launch: block b belongs to the last item whose firstBlock <= b.
 // The item index is wave-uniform, so its descriptor is read with scalar loads.
 template<int kFmt, bool kMulti, int kBlock = kRtBlock>
@@ -875,7 +900,14 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 	const int item = batchItemOf(items, itemCount, static_cast<int>(blockIdx.x),
 	                             [](const RaytraceBatchItem& r) { return r.firstBlock; });
 	const RaytraceBatchItem& it = items[item];
-	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, static_cast<int>(blockIdx.x) - it.firstBlock);
+	const int localBlock = static_cast<int>(blockIdx.x) - it.firstBlock;
+	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, localBlock);
+	if constexpr (!kMulti) {
+		const int beam = localBlock * kBlock + threadIdx.x;
+		if (it.fusedCount > 0 && beam < it.p.rayCount) {
+			applyFusedStages(it, beam);
+		}
+	}
 }
 
 // Single-sample beams (no divergence), one per thread; divergent beams, one per wave.

@@ -194,6 +194,7 @@ _PROTOTYPES = [
     ("rgl_tape_play", [C.c_char_p]),
     ("rgl_graph_batch_configure_capture", [C.c_bool]),
     ("rgl_graph_batch_get_capture_stats", [P(C.c_int64), P(C.c_int64)]),
+    ("rgl_graph_batch_get_fused_stages", [P(C.c_int64)]),
     ("rgl_configure_bvh_format", [C.c_int32]),
     ("rgl_get_bvh_format", [i32p]),
     ("rgl_node_points_radar_bin", [P(c_node)] + [C.c_float] * 6),
@@ -464,6 +465,13 @@ def batch_kernel_timing(enable: bool | None = None):
     n = C.c_int32()
     check("rgl_graph_batch_get_kernel_timing", C.byref(ms), C.byref(n))
     return ms.value, n.value
+
+
+def batch_fused_stages() -> int:
+    """Point stages folded into batched raytrace launches so far."""
+    v = C.c_int64()
+    check("rgl_graph_batch_get_fused_stages", C.byref(v))
+    return v.value
 
 
 def batch_capture(enable: bool | None = None):
