@@ -175,8 +175,12 @@ def run_batch64(args):
             rgl.check("rgl_graph_run", e.h)
         return sum(kept_count(o) for o in outs)
 
+    submit_s = []
+
     def batched_frame():
+        t0 = time.perf_counter()
         rgl.graph_run_batch(entries)
+        submit_s.append(time.perf_counter() - t0)
         return sum(kept_count(o) for o in outs)
 
     def timed(fn, steps):
@@ -197,7 +201,9 @@ def run_batch64(args):
     batched_frame()
     batched_frame()
     caps0 = rgl.batch_capture()
+    submit_s.clear()
     b_total, b_lat, b_kept = timed(batched_frame, args.steps)
+    submit_p50 = latency_stats(submit_s)[0]
     caps1 = rgl.batch_capture()
     # The same batches replayed from captured HIP graphs (capture is off by default), for comparison.
     rgl.batch_capture(True)
@@ -317,6 +323,7 @@ def run_batch64(args):
                   "p50_ms": pp50, "p99_ms": pp99, "raytrace_kernel_ms_per_step": round(p_kernel, 4),
                   "kept_points_per_step": p_kept, "mode": "one rgl_graph_run per sensor"},
         "batched_vs_plain": round(value / plain_value, 3),
+        "host_submit_p50_ms": submit_p50,
         "graph_replays_per_step": round((caps1[1] - caps0[1]) / args.steps, 3),
         "batched_captured": {"value": round(args.steps * rays_per_frame / e_total / 1e6, 3),
                              "ms_per_step": round(e_total / args.steps * 1e3, 4),
