@@ -784,9 +784,6 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		if (g > 0) {
 			RGL_HIP(hipMemcpyAsync(hostBuf->get(), slots, slotBytes, hipMemcpyDeviceToHost, s));
 		}
-		for (Scene* sc : pl.scenes) {
-			sc->noteUse(s);
-		}
 		if (g > 0) {
 			int k = 0;
 			for (int l = 0; l <= maxLevel; ++l) {
@@ -812,6 +809,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	} catch (...) {
 		auto ev = completionEvent();
 		ev->record(s);
+		for (Scene* sc : pl.scenes) {
+			sc->noteUse(s, ev);
+		}
 		for (auto& c : pl.ctxs) {
 			c->endBatch(s, ev);
 		}
@@ -820,6 +820,9 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	}
 	auto ev = completionEvent();
 	ev->record(s);
+	for (Scene* sc : pl.scenes) { // the scenes' structures are read until the batch completes
+		sc->noteUse(s, ev);
+	}
 	for (auto& c : pl.ctxs) {
 		for (auto& n : c->validatedOrder()) {
 			n->markEnqueued(ev);

@@ -456,10 +456,15 @@ float Scene::deltaTimeSeconds() const
 void Scene::noteUse(hipStream_t s)
 {
 	auto& ev = uses[s];
-	if (!ev) {
-		ev = std::make_unique<Event>();
+	if (!ev || ev.use_count() > 1) { // never re-record an event that is shared with its recorder
+		ev = std::make_shared<Event>();
 	}
 	ev->record(s);
+}
+
+void Scene::noteUse(hipStream_t s, std::shared_ptr<Event> done)
+{
+	uses[s] = std::move(done);
 }
 
 int32_t* Scene::overflowFlag()

@@ -179,6 +179,9 @@ public:
 	// Records that work enqueued on `s` so far reads the scene's device data; the next rebuild overwrites that data
 	// only after this work.
 	void noteUse(hipStream_t s);
+	// Same, with an event the caller has recorded on `s` after that work (a batch's completion event), so no event of
+	// the scene's own is recorded.
+	void noteUse(hipStream_t s, std::shared_ptr<Event> done);
 	// Traversal-stack overflow flag written by the raytrace kernel (never expected to be set).
 	int32_t* overflowFlag();
 	size_t entityCount() const { return entities.size(); }
@@ -209,7 +212,7 @@ private:
 	DeviceBuffer dInst, dShade, dBoxes, tlasNodes, tlasLeafPrim, tlasParent, tlasCounters, tlasHeight, tlasScratch;
 	DeviceBuffer dOverflow;
 	DeviceBuffer tlasWide, tlasWideScratch, tlasWideCount; // 4-wide TLAS (when the format is 4-wide)
-	std::map<hipStream_t, std::unique_ptr<Event>> uses;
+	std::map<hipStream_t, std::shared_ptr<Event>> uses;
 	uint64_t readyGeneration = 0;
 	hipStream_t waitedStream = nullptr;
 	uint64_t waitedGeneration = 0;
