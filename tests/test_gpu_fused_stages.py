@@ -17,9 +17,10 @@ XYZ, IS_HIT, DIST, ENT = 1, 4, 8, 7
 
 class Chain:
     """rays -> pose -> range -> raytrace -> points transform -> distance noise -> compact -> yield; with
-    `branch`, the transform has a second output (a yield), so only the transform can be folded."""
+    `branch`, the transform has a second output (a yield), so only the transform can be folded; with `two_returns`,
+    FIRST_LAST returns of undiverged beams (two return points per ray, both from the single sample)."""
 
-    def __init__(self, rgl, seed, branch):
+    def __init__(self, rgl, seed, branch, two_returns=False):
         self.rgl = rgl
         n = rgl.Node
         rays = S.make_lidar3d_rays(360, 40, 1.0, 1.0)
@@ -27,6 +28,8 @@ class Chain:
         pose = rgl.node_rays_transform(n(), S.f32(S.trs((1.0, -2.0, 1.5), (0, 0, 20))))
         rng = rgl.node_rays_set_range(n(), np.array([[0.3, 120.0]], np.float32))
         self.rt = rgl.node_raytrace(n())
+        if two_returns:
+            rgl.check("rgl_node_raytrace_configure_return_mode", self.rt.h, rgl.RGL_RETURN_FIRST_LAST)
         self.tf = rgl.node_points_transform(n(), S.f32(S.trs((0.25, 0.5, -1.0), (0, 5, 30))))
         self.noise = n()
         rgl.check("rgl_node_gaussian_noise_distance", self.noise.ref, 0.01, 0.02, 0.001)
@@ -48,13 +51,13 @@ class Chain:
         return r
 
 
-@pytest.mark.parametrize("branch", [False, True])
-def test_folded_stages_equal_separate_launches(rgl_lib, branch):
+@pytest.mark.parametrize("branch,two_returns", [(False, False), (True, False), (False, True)])
+def test_folded_stages_equal_separate_launches(rgl_lib, branch, two_returns):
     rgl = rgl_lib
     spec = S.config1_scene()
     _, ents, _ = S.load_into_rgl(spec)
-    solo = [Chain(rgl, k, branch) for k in range(3)]
-    batched = [Chain(rgl, k, branch) for k in range(3)]
+    solo = [Chain(rgl, k, branch, two_returns) for k in range(3)]
+    batched = [Chain(rgl, k, branch, two_returns) for k in range(3)]
     for frame in range(3):
         if frame:
             rgl.entity_set_transform(ents[2], S.f32(S.trs((1.0 * frame, 0.5, 0.0), (0, 0, 12 * frame), (1, 1, 1))))
