@@ -7,6 +7,12 @@
 
 #include <rgl/api/core.h>
 
+// Execution order of the graph containing `node`: its nodes in the order rgl_graph_run enqueues them, the reversed
+// post-order of a depth-first search from the entry nodes taken in ascending priority (the order the reference's
+// GraphRunCtx computes, src/graph/GraphRunCtx.cpp:107-142). On entry *inout_count is the capacity of out_nodes, on
+// return the graph's node count; nodes beyond the capacity are not written. Nothing is run or validated.
+RGL_API rgl_status_t rgl_graph_get_execution_order(rgl_node_t node, rgl_node_t* out_nodes, int32_t* inout_count);
+
 // Sets the seed of a Gaussian noise node's stream and restarts its run counter. The noise drawn for element i in
 // run r is a pure function of (seed, r, node kind, i), so the same seed replays the same noise on CPU and GPU.
 // Radar postprocess nodes (rgl_node_points_radar_postprocess) draw their per-cluster noise from such a stream too

@@ -31,6 +31,9 @@
 
 namespace {
 
+// Deliberate mistake planted by orc_plant_mistake (0 = none); only tests set it.
+int gMistake = 0;
+
 struct V3
 {
 	double x, y, z;
@@ -435,7 +438,7 @@ Shade shade(const World& w, const OrcRequest& q, const M34& rayToWorld, double m
 	s.distance = static_cast<float>(distance);
 	const V3 wA = in.toWorld.point(t.a), wB = in.toWorld.point(t.b), wC = in.toWorld.point(t.c);
 	const V3 n = unit(cross(wB - wA, wC - wA));
-	const double cosI = std::fabs(dot(n, d));
+	const double cosI = std::fabs(dot(gMistake == 6 ? cross(wB - wA, wC - wA) : n, d));
 	s.normal = n;
 	s.incident = static_cast<float>(std::acos(std::min(cosI, 1.0)));
 	double intensity = q.defaultIntensity;
@@ -461,6 +464,9 @@ Shade shade(const World& w, const OrcRequest& q, const M34& rayToWorld, double m
 		}
 		int x = std::min(std::max(static_cast<int>(std::floor(fu * W)), 0), W - 1);
 		int y = std::min(std::max(static_cast<int>(std::floor(fv * H)), 0), H - 1);
+		if (gMistake == 4) {
+			std::swap(x, y);
+		}
 		intensity = sc.texels[sc.texOffset[in.texture] + static_cast<int64_t>(y) * W + x];
 	}
 	s.base = intensity;
@@ -508,7 +514,8 @@ void put3(float* dst, int i, V3 v)
 // Euler angles of the Unity-style ZXY left-handed convention (src/math/Mat3x4f.hpp:146-175).
 float azimuthOf(const M34& r)
 {
-	return (r.m[2][1] < 1 && r.m[2][1] > -1) ? static_cast<float>(std::atan2(-r.m[2][0], r.m[2][2])) : 0.0f;
+	const double m20 = gMistake == 5 ? r.m[0][2] : r.m[2][0];
+	return (r.m[2][1] < 1 && r.m[2][1] > -1) ? static_cast<float>(std::atan2(-m20, r.m[2][2])) : 0.0f;
 }
 float elevationOf(const M34& r)
 {
@@ -523,8 +530,24 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	const int R = static_cast<int>(static_cast<uint32_t>(q.returnMode) >> 24);
 	const M34 toWorld = M34::fromFloat(q.rayOriginToWorld);
 	const M34 toRays = toWorld.inverse();
-	const float* rayF = q.rays + 12 * b;
-	const M34 rayWorld = M34::fromFloat(rayF);
+	const float* rayF = q.raysF64 != nullptr ? nullptr : q.rays + 12 * b;
+	M34 rayWorld;
+	if (q.raysF64 != nullptr) {
+		for (int i = 0; i < 12; ++i) {
+			rayWorld.m[i / 4][i % 4] = q.raysF64[12 * b + i];
+		}
+	} else {
+		rayWorld = M34::fromFloat(rayF);
+	}
+	// Origin and direction of the beam's world ray: the float rounding of the reference for float rays, the exact
+	// third column for FP64 rays.
+	auto beamRay = [&](V3& o, V3& d) {
+		if (rayF != nullptr) {
+			rayOf(rayF, o, d);
+		} else {
+			rayOf(rayWorld, o, d);
+		}
+	};
 	const M34 rayLocal = toRays * rayWorld;
 	const double eps = q.ambiguityEps;
 	bool ambiguous = false;
@@ -542,7 +565,7 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	const double minRange = range[0], maxRange = range[1];
 
 	V3 o0, d0;
-	rayOf(rayF, o0, d0);
+	beamRay(o0, d0);
 	auto writeNonHit = [&](int rp, uint8_t type, float dist) {
 		if (out.xyz) {
 			V3 disp = unit(d0) * dist;
@@ -573,7 +596,7 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	}
 	M34 ray = rayWorld;
 	if (q.distortion) {
-		const double off = q.timeOffsetsMs[b];
+		const double off = gMistake == 3 ? -q.timeOffsetsMs[b] : q.timeOffsetsMs[b];
 		const V3 t{off * q.linVel[0] * 1e-3, off * q.linVel[1] * 1e-3, off * q.linVel[2] * 1e-3};
 		const double toDeg = 180.0 / M_PI;
 		const V3 rd{off * q.angVel[0] * toDeg * 1e-3, off * q.angVel[1] * toDeg * 1e-3, off * q.angVel[2] * toDeg * 1e-3};
@@ -610,7 +633,7 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		}
 		V3 so, sd;
 		if (s == 0 && !q.distortion) {
-			rayOf(rayF, so, sd);
+			beamRay(so, sd);
 		} else {
 			rayOf(sr, so, sd);
 		}
@@ -644,6 +667,10 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		}
 		if (first.sample < 0) {
 			first = second = last = strongest = secondStrongest = c;
+			if (gMistake == 1) {
+				second.distance = INFINITY;
+				secondStrongest.intensity = -INFINITY;
+			}
 			continue;
 		}
 		const float dd = c.distance;
@@ -653,7 +680,7 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		} else if (first.distance < dd && dd < second.distance) {
 			second = c;
 		}
-		if (dd > last.distance) last = c;
+		if (gMistake == 2 ? dd < last.distance : dd > last.distance) last = c;
 		const float I = c.intensity;
 		if (I > strongest.intensity) {
 			secondStrongest = strongest;
@@ -778,6 +805,8 @@ extern "C" int orc_world_raytrace(void* world, const OrcRequest* q, OrcOutputs* 
 }
 
 extern "C" void orc_world_destroy(void* world) { delete static_cast<World*>(world); }
+
+extern "C" void orc_plant_mistake(int which) { gMistake = which; }
 
 static int traceWorld(const World& w, const OrcRequest* q, OrcOutputs* out)
 {

@@ -63,6 +63,9 @@ typedef struct
 	int bruteForce;
 	int threads;
 	double ambiguityEps; /* relative tolerance of the grazing / tie diagnostic */
+	/* Optional: the same world-frame rays in FP64 (12 doubles per ray), e.g. the exact product of a sensor pose and
+	   sensor-frame rays. When set they replace `rays`, and a ray's direction is its matrix's third column. */
+	const double* raysF64;
 } OrcRequest;
 
 typedef struct
@@ -103,6 +106,13 @@ int orc_raytrace(const OrcScene* scene, const OrcRequest* request, OrcOutputs* o
 void* orc_world_create(const OrcScene* scene, int bruteForce);
 int orc_world_raytrace(void* world, const OrcRequest* request, OrcOutputs* out);
 void orc_world_destroy(void* world);
+
+/* Plants one deliberate mistake in the oracle (0 = none), so tests can show that their checks catch it:
+   1 = the beam reduction seeds its SECOND / SECOND_STRONGEST trackers with +inf / -inf instead of the first hit,
+   2 = the LAST tracker keeps the minimum distance, 3 = velocity distortion moves the ray backwards in time,
+   4 = texel lookup swaps the texture's x and y, 5 = azimuth reads the transposed matrix element,
+   6 = the incidence factor uses the unnormalized normal. Process-wide; tests reset it to 0. */
+void orc_plant_mistake(int which);
 
 /* Philox4x32-10 block (Salmon et al., SC'11). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -41,7 +41,7 @@ class OrcRequest(C.Structure):
         ("distortion", C.c_int), ("linVel", C.c_float * 3), ("angVel", C.c_float * 3), ("nearDistance", C.c_float),
         ("farDistance", C.c_float), ("defaultIntensity", C.c_float), ("sceneDeltaTime", C.c_float),
         ("hDivergence", C.c_float), ("vDivergence", C.c_float), ("returnMode", C.c_int32), ("bruteForce", C.c_int),
-        ("threads", C.c_int), ("ambiguityEps", C.c_double),
+        ("threads", C.c_int), ("ambiguityEps", C.c_double), ("raysF64", C.POINTER(C.c_double)),
     ]
 
 
@@ -90,6 +90,7 @@ def load() -> C.CDLL:
         lib.orc_logf.argtypes = [C.c_float]
         lib.orc_logf.restype = C.c_float
         lib.orc_sincos2pi.argtypes = [C.c_float, f32p, f32p]
+        lib.orc_plant_mistake.argtypes = [C.c_int]
         _lib = lib
     return _lib
 
@@ -174,14 +175,21 @@ def raytrace(spec, rays_world, ray_origin_to_world=None, ranges=((0.0, 3.4028234
              near=float("inf"), far=float("inf"), default_intensity=0.0, scene_dt=0.0, divergence=(0.0, 0.0),
              return_mode=(1 << 24) | 4, brute_force=False, threads=None, eps=1e-5, prev_transforms=None,
              fields=None, world: "World | None" = None, displacements=None) -> dict:
-    """Runs the raytrace node semantics on the CPU. rays_world: (N, 3, 4) rays after the rays chain. Returns a dict
-    of per-return-point arrays keyed by rgl field id, plus 'ambiguous', 'primaryT', 'primaryInst', 'primaryPrim'."""
+    """Runs the raytrace node semantics on the CPU. rays_world: (N, 3, 4) rays after the rays chain; a float64
+    array is traced as given (FP64 world rays, direction = third column), anything else as float32 rays with the
+    reference's float rounding of the direction. Returns a dict of per-return-point arrays keyed by rgl field id,
+    plus 'ambiguous', 'primaryT', 'primaryInst', 'primaryPrim'."""
     lib = load()
     keep = _Keep()
     sc = make_scene(spec, keep, prev_transforms, displacements) if world is None else None
-    rays = keep(np.ascontiguousarray(np.asarray(rays_world, np.float32).reshape(-1, 3, 4)))
-    n = len(rays)
     q = OrcRequest()
+    if isinstance(rays_world, np.ndarray) and rays_world.dtype == np.float64:
+        rays64 = keep(np.ascontiguousarray(rays_world.reshape(-1, 3, 4)))
+        q.raysF64 = _p(rays64, C.c_double)
+        rays = keep(rays64.astype(np.float32))
+    else:
+        rays = keep(np.ascontiguousarray(np.asarray(rays_world, np.float32).reshape(-1, 3, 4)))
+    n = len(rays)
     q.nRays = n
     q.rays = _p(rays, C.c_float)
     tw = np.asarray(ray_origin_to_world if ray_origin_to_world is not None else np.eye(3, 4), np.float32).ravel()
@@ -251,6 +259,18 @@ class World:
         self.close()
 
 
+# Deliberate mistake planted in the oracle (see orc_plant_mistake in oracle.h for 1-6); 7 and 8 live in the numpy
+# stages below: 7 = distance noise moves points along their world position instead of their look-at position,
+# 8 = angular noise rotates by the negated angle. Only tests set it, and reset it to 0.
+MISTAKE = 0
+
+
+def plant_mistake(which: int) -> None:
+    global MISTAKE
+    MISTAKE = which
+    load().orc_plant_mistake(which if which <= 6 else 0)
+
+
 def philox(ctr, key) -> np.ndarray:
     lib = load()
     c = (C.c_uint32 * 4)(*ctr)
@@ -310,6 +330,8 @@ def noise_distance(xyz, distance, normals_, mean, base, rise, look_at):
     """src/gpu/gaussianNoiseKernels.cu:49-66 in FP64: sigma = base + d*rise; e = mean + n*sigma."""
     d = np.asarray(distance, np.float64)
     e = mean + np.asarray(normals_, np.float64) * (d * rise + base)
+    if MISTAKE == 7:
+        look_at = np.eye(3, 4)
     pr = _mat_apply(look_at, np.asarray(xyz, np.float64))
     with np.errstate(invalid="ignore", divide="ignore"):
         moved = pr + pr / np.linalg.norm(pr, axis=1, keepdims=True) * e[:, None]
@@ -318,7 +340,7 @@ def noise_distance(xyz, distance, normals_, mean, base, rise, look_at):
 
 def noise_angular_hitpoint(xyz, normals_, mean, st_dev, axis, look_at):
     """src/gpu/gaussianNoiseKernels.cu:31-47 in FP64."""
-    ang = mean + np.asarray(normals_, np.float64) * st_dev
+    ang = (mean + np.asarray(normals_, np.float64) * st_dev) * (-1.0 if MISTAKE == 8 else 1.0)
     q = np.einsum("nij,nj->ni", _rot_axis(axis, ang), _mat_apply(look_at, np.asarray(xyz, np.float64)))
     return _mat_apply(_inverse(look_at), q), np.linalg.norm(q, axis=1)
 
@@ -328,7 +350,7 @@ def noise_angular_rays(rays, normals_, mean, st_dev, axis, look_at):
     rays = np.asarray(rays, np.float64).reshape(-1, 3, 4)
     L = np.asarray(look_at, np.float64)
     Li = _inverse(L)
-    ang = mean + np.asarray(normals_, np.float64) * st_dev
+    ang = (mean + np.asarray(normals_, np.float64) * st_dev) * (-1.0 if MISTAKE == 8 else 1.0)
     R = _rot_axis(axis, ang)
 
     def mm(a, b):  # 3x4 affine products, batched

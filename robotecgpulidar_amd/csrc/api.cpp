@@ -1286,6 +1286,23 @@ RGL_API rgl_status_t rgl_graph_node_get_priority(rgl_node_t node, int32_t* out_p
 }
 
 // ============================================================================================ extensions
+RGL_API rgl_status_t rgl_graph_get_execution_order(rgl_node_t node, rgl_node_t* out_nodes, int32_t* inout_count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node != nullptr);
+		CHECK_ARG(inout_count != nullptr);
+		CHECK_ARG(*inout_count >= 0);
+		CHECK_ARG(out_nodes != nullptr || *inout_count == 0);
+		auto n = lookupNode(node);
+		const auto order = rgl::GraphRunCtx::findExecutionOrder(n->connectedComponent());
+		const int32_t capacity = *inout_count;
+		for (int32_t i = 0; i < capacity && i < static_cast<int32_t>(order.size()); ++i) {
+			out_nodes[i] = handleOf(order[i]);
+		}
+		*inout_count = static_cast<int32_t>(order.size());
+	});
+}
+
 RGL_API rgl_status_t rgl_node_gaussian_noise_set_seed(rgl_node_t node, uint64_t seed)
 {
 	return safeCall([&]() {

@@ -172,6 +172,7 @@ _PROTOTYPES = [
     ("rgl_graph_node_set_priority", [c_node, C.c_int32]),
     ("rgl_graph_node_get_priority", [c_node, i32p]),
     # extensions (include/rgl/api/extensions/amd.h)
+    ("rgl_graph_get_execution_order", [c_node, P(c_node), i32p]),
     ("rgl_node_gaussian_noise_set_seed", [c_node, C.c_uint64]),
     ("rgl_node_gaussian_noise_get_stream", [c_node, P(C.c_uint64), P(C.c_uint32), P(C.c_uint32)]),
     ("rgl_scene_set_tlas_refit", [C.c_void_p, C.c_bool]),
@@ -444,6 +445,21 @@ def add_child(parent: Node, child: Node) -> None:
 def chain(*nodes: Node) -> None:
     for a, b in zip(nodes, nodes[1:]):
         add_child(a, b)
+
+
+def execution_order(node: Node) -> list[int]:
+    """Handles (as integers) of the nodes of `node`'s graph in execution order."""
+    count = C.c_int32(0)
+    check("rgl_graph_get_execution_order", node.h, None, C.byref(count))
+    arr = (c_node * max(count.value, 1))()
+    check("rgl_graph_get_execution_order", node.h, arr, C.byref(count))
+    return [arr[i] for i in range(count.value)]
+
+
+def get_priority(node: Node) -> int:
+    p = C.c_int32()
+    check("rgl_graph_node_get_priority", node.h, C.byref(p))
+    return p.value
 
 
 def graph_run(node: Node) -> None:

@@ -361,6 +361,13 @@ def radar_rays(azimuths: int = 256, elevations: int = 64, az_half_deg: float = 6
     return f32(m)
 
 
+# Radar cells of the serving workload: 0.2 m range x 1 degree azimuth over [0, 200) m x [-60, 60) degrees; the
+# solid angle of one radar ray is the azimuth step x elevation step of radar_rays().
+RADAR_CELL = dict(range_step=0.2, range_max=200.0, azimuth_step=float(np.deg2rad(1.0)),
+                  azimuth_min=float(np.deg2rad(-60.0)), azimuth_max=float(np.deg2rad(60.0)))
+RADAR_SOLID_ANGLE = float(np.deg2rad(120.0 / 256) * np.deg2rad(20.0 / 63))
+
+
 @dataclass
 class Client:
     scene: int

@@ -28,6 +28,8 @@ def rgl_lib():
     from robotecgpulidar_amd import rgl
     lib = rgl.load()
     rgl.check("rgl_configure_logging", 6, None, False)
+    rgl.set_bvh_format(rgl.BVH4_F32)  # process-wide settings a previous test may have changed
+    rgl.check("rgl_scene_set_tlas_refit", None, False)
     yield rgl
     active = ctypes.c_bool(False)
     if rgl.call("rgl_tape_record_is_active", ctypes.byref(active)) == rgl.RGL_SUCCESS and active.value:

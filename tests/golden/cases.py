@@ -12,7 +12,7 @@ from robotecgpulidar_amd import rgl
 from robotecgpulidar_amd import scenes as S
 from tests.helpers import Case
 
-XYZ, INTEN, IS_HIT, ENT, DIST, RET, NORMAL = 1, 2, 4, 7, 8, 12, 22
+XYZ, INTEN, IS_HIT, ENT, DIST, RET, NORMAL, INC = 1, 2, 4, 7, 8, 12, 22, 23
 GEOMETRY_FIELDS = [XYZ, DIST, ENT, NORMAL, IS_HIT]
 MULTI_FIELDS = [DIST, ENT, IS_HIT, INTEN, RET]
 
@@ -55,6 +55,25 @@ def multireturn_case(mode: str):
     return case, MULTI_FIELDS
 
 
+def c3_case(mode: str):
+    """C3 at full scene size, reduced beam set: the ~10M-triangle textured scene of 2,001 instances, every 256th beam
+    of the 64 x 4096 lidar (1,024 beams) with 0.13 mrad divergence (58 samples per beam), return mode `mode`,
+    default intensity 30, range 200 m."""
+    spec, _ = S.config3_scene()
+    lidar = S.config3_lidar()
+    sel = np.arange(0, len(lidar.rays), 256)
+    case = Case(spec, lidar.rays[sel], pose=lidar.pose, ranges=np.array([lidar.range], np.float32),
+                return_mode=getattr(rgl, "RGL_RETURN_" + mode), divergence=(0.13e-3, 0.13e-3), default_intensity=30.0)
+    return case, MULTI_FIELDS
+
+
+def _c3(mode: str):
+    def make():
+        return c3_case(mode)
+    make.__doc__ = c3_case.__doc__.replace("return mode `mode`", f"return mode {mode}")
+    return make
+
+
 def _multireturn(mode: str):
     def make():
         return multireturn_case(mode)
@@ -67,7 +86,20 @@ CASES = {
     "urban": urban_case,
     "multireturn_first_last": _multireturn("FIRST_LAST"),
     "multireturn_strongest": _multireturn("STRONGEST_SECOND_STRONGEST"),
+    "c3_first_last": _c3("FIRST_LAST"),
+    "c3_strongest": _c3("STRONGEST"),
 }
+
+# C4 sample: for every sensor of scenes.config4_workload(), C4_BEAMS seeded beam indices (C4_SEED + sensor index);
+# geometry fields of the raytrace node for those beams (explicit XYZ and normal for the first C4_EXPLICIT of them).
+C4_BEAMS = 1024
+C4_EXPLICIT = 256
+C4_SEED = 0xC4C4
+
+
+def c4_beams(sensor: int, rays: int) -> np.ndarray:
+    rng = np.random.default_rng(C4_SEED + sensor)
+    return np.sort(rng.choice(rays, min(C4_BEAMS, rays), replace=False)).astype(np.int64)
 
 # (seed, run, tag) streams whose first 64 normal variates are committed.
 NOISE_STREAMS = [(0, 0, 1), (0xC0F2D, 3, 3), (0xDEADBEEFCAFEF00D, 123456, 2)]

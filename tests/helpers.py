@@ -30,14 +30,21 @@ class Case:
     extra: dict = field(default_factory=dict)
 
     def world_rays(self) -> np.ndarray:
+        """World-frame rays for the oracle. With a pose, the exact FP64 product T * in of the FP32 pose and rays
+        (TransformRaysNode, src/gpu/nodeKernels.cu:45-49), so that the library's own composition error counts
+        against it; without one, the FP32 rays as given."""
         if self.pose is None:
             return self.rays
-        p = np.asarray(self.pose, np.float32)
-        r = np.asarray(self.rays, np.float32)
-        # TransformRaysNode: out = T * in, evaluated in FP32 like the kernel (src/gpu/nodeKernels.cu:45-49).
-        out = np.einsum("ij,njk->nik", p[:, :3], r).astype(np.float32)
-        out[:, :, 3] += p[:, 3]
-        return out
+        return compose64(self.pose, self.rays)
+
+
+def compose64(pose, rays) -> np.ndarray:
+    """T * rays for (3, 4) T and (N, 3, 4) rays, in FP64 from their FP32 values."""
+    p = np.asarray(pose, np.float32).astype(np.float64)
+    r = np.asarray(rays, np.float32).astype(np.float64)
+    out = np.matmul(p[None, :, :3], r)
+    out[:, :, 3] += p[:, 3]
+    return out
 
 
 def run_oracle(oracle, case: Case, fields=None, brute_force=False, eps=1e-5):
@@ -123,3 +130,77 @@ DISTANCE_PARAMS = [23.1753849, 119.9884538, 333.3394629]  # test/src/graph/Dista
 
 def f32sub(a, b) -> np.float32:
     return np.float32(np.float32(a) - np.float32(b))
+
+
+# ------------------------------------------------------------------------------------------ golden comparison
+GEOM, DIST_TIE, INT_TIE, TEXEL = 1, 2, 4, 8  # oracle ambiguity bits
+G_XYZ, G_INTEN, G_IS_HIT, G_ENT, G_DIST, G_NORMAL = 1, 2, 4, 7, 8, 22
+
+
+def blocking_bits(return_type: int, quantity: str) -> int:
+    """Ambiguity bits under which a return point's `quantity` may legitimately differ from FP64. A return selected
+    by distance (FIRST = 4, LAST = 2) keeps its distance under near ties but not its sample, hence not its
+    intensity; STRONGEST (1) keeps its intensity but not its sample; SECOND (3) and SECOND_STRONGEST (5) follow
+    order-dependent strict comparisons and change on any near tie of their key. Texel edges change intensities."""
+    by_distance = return_type in (2, 3, 4)
+    if quantity == "distance":
+        if return_type in (2, 4):
+            return GEOM
+        if return_type == 3:
+            return GEOM | DIST_TIE
+        return GEOM | TEXEL | INT_TIE
+    if by_distance:
+        return GEOM | DIST_TIE | TEXEL
+    return GEOM | TEXEL | (INT_TIE if return_type == 5 else 0)
+
+
+def check_golden(gpu: dict, hit, entity, ambiguous, idx, values: dict, return_mode: int, label: str,
+                 min_compared: int, exact_idx=None) -> dict:
+    """Checks the library's return points `gpu` (field -> array over every return point) against oracle outputs:
+    `hit`, `entity` over every return point, per-beam `ambiguous` bits, and `values` (field -> oracle values at the
+    return points `idx`). XYZ and normals are compared at `exact_idx` (positions into idx; default all of idx).
+    Bar: hit masks equal except on geometrically ambiguous beams, and for single returns at most max(1, 1e-5 n)
+    differences; entity ids equal on unambiguous beams; distance within max(1 mm, 1e-5 relative), XYZ within 2 mm,
+    normals within 1e-4, intensity within 1e-4 relative + 2e-3 where no ambiguity bit blocks the quantity.
+    Returns counts and the largest distance error."""
+    n = len(hit)
+    beams = len(ambiguous)
+    R = n // beams
+    amb = np.repeat(np.asarray(ambiguous), R)
+    g_hit = gpu[G_IS_HIT]
+    assert len(g_hit) == n, (label, len(g_hit), n)
+    hit_mism = g_hit != hit
+    unexplained = hit_mism & ((amb & GEOM) == 0)
+    assert not unexplained.any(), (label, np.nonzero(unexplained)[0][:10])
+    if R == 1:
+        assert hit_mism.sum() <= max(1, int(1e-5 * n)), (label, int(hit_mism.sum()))
+    ent_mism = (g_hit == 1) & (hit == 1) & (gpu[G_ENT] != entity)
+    assert not (ent_mism & (amb == 0)).any(), (label, np.nonzero(ent_mism & (amb == 0))[0][:10])
+    types = (return_mode >> (8 * (idx % R))) & 0xFF
+    both = (hit[idx] == 1) & (g_hit[idx] == 1)
+    block_d = np.array([blocking_bits(int(t), "distance") for t in types], np.int64)
+    block_i = np.array([blocking_bits(int(t), "intensity") for t in types], np.int64)
+    sel = both & ((amb[idx] & block_d) == 0)
+    sel_i = both & ((amb[idx] & block_i) == 0)
+    assert sel.sum() >= min_compared, (label, int(sel.sum()))
+    d_o = values[G_DIST][sel].astype(np.float64)
+    d_g = gpu[G_DIST][idx][sel].astype(np.float64)
+    err = np.abs(d_g - d_o)
+    assert np.all(err <= np.maximum(1e-3, 1e-5 * d_o)), (label, err.max())
+    exact = both & (amb[idx] == 0)
+    ex = np.arange(len(idx)) if exact_idx is None else np.asarray(exact_idx)
+    ex = ex[exact[ex]]
+    if G_XYZ in values:
+        assert np.abs(gpu[G_XYZ][idx][ex] - values[G_XYZ][ex]).max(initial=0.0) <= 2e-3, label
+    if G_NORMAL in values:
+        assert np.abs(gpu[G_NORMAL][idx][ex] - values[G_NORMAL][ex]).max(initial=0.0) <= 1e-4, label
+    if G_INTEN in values:
+        i_o = values[G_INTEN][sel_i].astype(np.float64)
+        i_g = gpu[G_INTEN][idx][sel_i].astype(np.float64)
+        bad = np.abs(i_g - i_o) > 1e-4 * i_o + 2e-3
+        assert not bad.any(), (label, idx[sel_i][bad], i_g[bad], i_o[bad])
+    non_hit = (hit[idx] == 0) & (g_hit[idx] == 0)
+    np.testing.assert_array_equal(gpu[G_DIST][idx][non_hit], values[G_DIST][non_hit])
+    return {"points": n, "hits": int(hit.sum()), "mask_differences": int(hit_mism.sum()),
+            "ambiguous_beams": int((np.asarray(ambiguous) != 0).sum()), "compared": int(sel.sum()),
+            "max_dist_err_m": float(err.max(initial=0.0))}
