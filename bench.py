@@ -33,59 +33,15 @@ sys.path.insert(0, str(ROOT))
 
 from robotecgpulidar_amd import rgl  # noqa: E402
 from robotecgpulidar_amd import scenes as S  # noqa: E402
+from robotecgpulidar_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s + p50/p99 frame latency, served-batched vs plain, 1 MI355X"
-FIELDS = [rgl.XYZ_VEC3_F32, rgl.DISTANCE_F32, rgl.ENTITY_ID_I32, rgl.NORMAL_VEC3_F32, rgl.IS_HIT_I32]
-ANGULAR_SIGMA = float(np.float32(0.05 * np.pi / 180.0))
+FIELDS = W.FIELDS
+GOLDEN = ROOT / "tests" / "golden"
 
 
-def build_urban_graph(lidar: S.Lidar):
-    n = rgl.Node
-    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
-    pose = rgl.node_rays_transform(n(), lidar.pose)
-    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
-    offs = rgl.node_rays_set_time_offsets(n(), lidar.time_offsets)
-    ang = n()
-    rgl.check("rgl_node_gaussian_noise_angular_ray", ang.ref, 0.0, ANGULAR_SIGMA, rgl.RGL_AXIS_Z)
-    rt = rgl.node_raytrace(n())
-    lin = np.array([10.0, 0.0, 0.0], np.float32)
-    angv = np.zeros(3, np.float32)
-    rgl.check("rgl_node_raytrace_configure_velocity", rt.h, lin.ctypes.data, angv.ctypes.data)
-    rgl.check("rgl_node_raytrace_configure_distortion", rt.h, True)
-    dist = n()
-    rgl.check("rgl_node_gaussian_noise_distance", dist.ref, 0.0, 0.02, 0.001)
-    comp = rgl.node_points_compact_by_field(n(), rgl.IS_HIT_I32)
-    out = rgl.node_points_yield(n(), FIELDS)
-    rgl.chain(rays, pose, rng, offs, ang, rt, dist, comp, out)
-    rgl.check("rgl_node_gaussian_noise_set_seed", ang.h, C.c_uint64(0xC0F2A))
-    rgl.check("rgl_node_gaussian_noise_set_seed", dist.h, C.c_uint64(0xC0F2D))
-    return rays, rt, out
-
-
-def build_sensor_graph(lidar: S.Lidar, scene, seed: int):
-    """rays -> pose -> range -> raytrace(scene) -> points transform (world -> sensor) -> distance noise -> compact
-    -> yield."""
-    n = rgl.Node
-    rays = rgl.node_rays_from_mat3x4f(n(), lidar.rays)
-    pose = rgl.node_rays_transform(n(), lidar.pose)
-    rng = rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32))
-    rt = rgl.node_raytrace(n(), scene)
-    to_sensor = S.f32(np.linalg.inv(np.vstack([lidar.pose.astype(np.float64), [0, 0, 0, 1]]))[:3])
-    tf = rgl.node_points_transform(n(), to_sensor)
-    dist = n()
-    rgl.check("rgl_node_gaussian_noise_distance", dist.ref, 0.0, 0.02, 0.001)
-    comp = rgl.node_points_compact_by_field(n(), rgl.IS_HIT_I32)
-    out = rgl.node_points_yield(n(), FIELDS)
-    rgl.chain(rays, pose, rng, rt, tf, dist, comp, out)
-    rgl.check("rgl_node_gaussian_noise_set_seed", dist.h, C.c_uint64(seed))
-    return rays, rt, out
-
-
-def kept_count(out) -> int:
-    count = C.c_int32()
-    rgl.check("rgl_graph_get_result_size", out.h, rgl.XYZ_VEC3_F32, C.byref(count), None)
-    return count.value
+kept_count = W.kept_count
 
 
 def frame(entry, out) -> int:

@@ -41,6 +41,12 @@ RGL_API rgl_status_t rgl_get_bvh_format(int32_t* out_format);
 RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t* nodes_visited, int32_t* tris_tested,
                                                        int32_t* instances_entered, int32_t ray_count);
 
+// Bytes of acceleration-structure and scene records the raytrace launches of the given raytrace nodes read, each
+// 64 B line counted once over all of them (the unique, compulsory part of their BVH traffic): nodes in the node format
+// in use, leaf triangles, instance records, and the shading records, index triples, vertices and texture coordinates
+// of the triangles hit. Traces the nodes' current rays again (a measurement; results are not touched).
+RGL_API rgl_status_t rgl_graph_traversal_footprint(const rgl_node_t* raytrace_nodes, int32_t count, int64_t* out_bytes);
+
 // Closest hit of every primary ray by testing all triangles of all instances (no BVH). Misses give t = -1,
 // instance = -1, triangle = -1. Intended for verifying the BVH traversal.
 RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t, int32_t* out_instance,

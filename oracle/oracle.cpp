@@ -149,14 +149,13 @@ M34 trsDeg(V3 t, V3 rdeg)
 	return r;
 }
 
-// Ray origin M*(0,0,0) and direction M*(0,0,1) - origin, with the float rounding the reference applies
-// (src/gpu/optixPrograms.cu:222-228): this is how a user's rgl_mat3x4f defines its ray.
+// Ray origin M*(0,0,0) and direction M*(0,0,1) - M*(0,0,0) (src/gpu/optixPrograms.cu:222-228), evaluated exactly:
+// the direction is the matrix's third column (the reference's float subtraction rounds it to the ulp of the
+// translation, which the library does not reproduce).
 void rayOf(const float* m, V3& o, V3& d)
 {
 	o = {m[3], m[7], m[11]};
-	volatile float s0 = m[2] + m[3], s1 = m[6] + m[7], s2 = m[10] + m[11];
-	d = {static_cast<double>(static_cast<float>(s0 - m[3])), static_cast<double>(static_cast<float>(s1 - m[7])),
-	     static_cast<double>(static_cast<float>(s2 - m[11]))};
+	d = {m[2], m[6], m[10]};
 }
 void rayOf(const M34& m, V3& o, V3& d)
 {
@@ -894,18 +893,23 @@ extern "C" void orc_sincos2pi(float u, float* s, float* c)
 	}
 }
 
-extern "C" void orc_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int count)
+extern "C" float orc_normal_at(uint64_t seed, uint32_t run, uint32_t tag, uint32_t index)
 {
 	const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+	const uint32_t ctr[4] = {index, run, tag, 0u};
+	uint32_t o[4];
+	orc_philox4x32_10(ctr, key, o);
+	const float u1 = static_cast<float>((o[0] >> 8) + 1u) * 5.9604644775390625e-8f;
+	const float u2 = static_cast<float>(o[1] >> 8) * 5.9604644775390625e-8f;
+	const float rad = std::sqrt(-2.0f * orc_logf(u1));
+	float s, c;
+	orc_sincos2pi(u2, &s, &c);
+	return rad * c;
+}
+
+extern "C" void orc_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int count)
+{
 	for (int i = 0; i < count; ++i) {
-		const uint32_t ctr[4] = {static_cast<uint32_t>(i), run, tag, 0u};
-		uint32_t o[4];
-		orc_philox4x32_10(ctr, key, o);
-		const float u1 = static_cast<float>((o[0] >> 8) + 1u) * 5.9604644775390625e-8f;
-		const float u2 = static_cast<float>(o[1] >> 8) * 5.9604644775390625e-8f;
-		const float rad = std::sqrt(-2.0f * orc_logf(u1));
-		float s, c;
-		orc_sincos2pi(u2, &s, &c);
-		out[i] = rad * c;
+		out[i] = orc_normal_at(seed, run, tag, static_cast<uint32_t>(i));
 	}
 }

@@ -119,6 +119,46 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Standard normal variates of the library's seeded stream: normal(seed, run, tag, i), i in [0, count). */
 void orc_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int count);
+float orc_normal_at(uint64_t seed, uint32_t run, uint32_t tag, uint32_t index);
+
+/* ------------------------------------------------------------------------------------------------ CPU port
+   FP32 CPU port of the library's lidar graph rays -> pose -> range -> raytrace -> points transform -> distance
+   noise -> compact (cpu_port.cpp): the timed CPU baseline. Its own canonical LBVH (Morton codes, radix tree), built
+   once per scene; frames run on `threads` std::threads over 64-ray tiles. */
+typedef struct
+{
+	int nRays;
+	const float* rays;    /* 12 floats per sensor-frame ray */
+	float pose[12];       /* rays transform, composed with each ray in FP64 and rounded once */
+	float range[2];
+	float toSensor[12];   /* points transform */
+	float lookAt[12];     /* look-at transform of the distance-noise node and its inverse */
+	float lookAtInv[12];
+	float noiseMean, noiseBase, noiseRise;
+	uint64_t seed;
+	uint32_t run, tag;
+} CpuFrame;
+
+typedef struct
+{
+	/* uncompacted outputs of the noise node (nRays entries each) */
+	float* fullXyz;
+	float* fullDistance;
+	int32_t* fullEntityId;
+	float* fullNormal;
+	int32_t* fullIsHit;
+	/* compacted hits (room for nRays); `kept` is written */
+	float* xyz;
+	float* distance;
+	int32_t* entityId;
+	float* normal;
+	int32_t kept;
+} CpuOut;
+
+void* cpu_scene_create(const OrcScene* scene);
+void cpu_scene_destroy(void* scene);
+/* One frame of `count` sensors (sensor k on scenes[k]) on `threads` threads. Returns 0. */
+int cpu_run(void* const* scenes, const CpuFrame* frames, CpuOut* outs, int count, int threads);
 
 /* Natural log and sin/cos(2 pi u) with the single-precision methods the noise stream is defined with. */
 float orc_logf(float x);

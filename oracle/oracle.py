@@ -2,7 +2,8 @@
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, as the checker (or the
 timed CPU baseline). It wraps oracle/liboracle.so (oracle.cpp, FP64 restatement of the reference's per-ray
-semantics) and restates the point-cloud stages in numpy:
+semantics; cpu_port.cpp, the FP32 multithreaded CPU port of the lidar graph timed as the CPU baseline) and restates
+the point-cloud stages in numpy:
   compaction     src/graph/CompactByFieldPointsNode.cpp:33-105 (stable: keep elements whose field != 0)
   point transform src/gpu/nodeKernels.cu:51-56
   noise          src/gpu/gaussianNoiseKernels.cu:20-66 with the library's seeded stream (orc_normals)
@@ -65,6 +66,19 @@ class OrcOutputs(C.Structure):
     ]
 
 
+class CpuFrame(C.Structure):
+    _fields_ = [("nRays", C.c_int), ("rays", f32p), ("pose", C.c_float * 12), ("range", C.c_float * 2),
+                ("toSensor", C.c_float * 12), ("lookAt", C.c_float * 12), ("lookAtInv", C.c_float * 12),
+                ("noiseMean", C.c_float), ("noiseBase", C.c_float), ("noiseRise", C.c_float), ("seed", C.c_uint64),
+                ("run", C.c_uint32), ("tag", C.c_uint32)]
+
+
+class CpuOut(C.Structure):
+    _fields_ = [("fullXyz", f32p), ("fullDistance", f32p), ("fullEntityId", i32p), ("fullNormal", f32p),
+                ("fullIsHit", i32p), ("xyz", f32p), ("distance", f32p), ("entityId", i32p), ("normal", f32p),
+                ("kept", C.c_int32)]
+
+
 _lib = None
 
 
@@ -91,6 +105,11 @@ def load() -> C.CDLL:
         lib.orc_logf.restype = C.c_float
         lib.orc_sincos2pi.argtypes = [C.c_float, f32p, f32p]
         lib.orc_plant_mistake.argtypes = [C.c_int]
+        lib.cpu_scene_create.argtypes = [C.POINTER(OrcScene)]
+        lib.cpu_scene_create.restype = C.c_void_p
+        lib.cpu_scene_destroy.argtypes = [C.c_void_p]
+        lib.cpu_run.argtypes = [C.POINTER(C.c_void_p), C.POINTER(CpuFrame), C.POINTER(CpuOut), C.c_int, C.c_int]
+        lib.cpu_run.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -269,6 +288,71 @@ def plant_mistake(which: int) -> None:
     global MISTAKE
     MISTAKE = which
     load().orc_plant_mistake(which if which <= 6 else 0)
+
+
+class CpuPort:
+    """The FP32 CPU port of the lidar graph rays -> pose -> range -> raytrace -> points transform -> distance noise
+    -> compact over `specs` (one CPU scene each, LBVHs built once)."""
+
+    def __init__(self, specs):
+        lib = load()
+        self.keep = _Keep()
+        self.scenes = [lib.cpu_scene_create(C.byref(make_scene(sp, self.keep))) for sp in specs]
+
+    def run(self, sensors, threads: int):
+        """sensors: list of dicts(scene, rays, pose, range, to_sensor, look_at, look_at_inv, noise=(mean, base,
+        rise), seed, run, tag). Returns per sensor the kept (xyz, distance, entity, normal) and the uncompacted
+        arrays (xyz, distance, entity, normal, is_hit)."""
+        n = len(sensors)
+        frames = (CpuFrame * n)()
+        outs = (CpuOut * n)()
+        keep = _Keep()
+        res = []
+        for k, sn in enumerate(sensors):
+            rays = keep(np.ascontiguousarray(np.asarray(sn["rays"], np.float32).reshape(-1, 3, 4)))
+            m = len(rays)
+            f = frames[k]
+            f.nRays = m
+            f.rays = _p(rays, C.c_float)
+            for name in ("pose", "to_sensor", "look_at", "look_at_inv"):
+                arr = np.asarray(sn[name], np.float32).ravel()
+                dst = getattr(f, {"pose": "pose", "to_sensor": "toSensor", "look_at": "lookAt",
+                                  "look_at_inv": "lookAtInv"}[name])
+                for i in range(12):
+                    dst[i] = float(arr[i])
+            f.range[0], f.range[1] = sn["range"]
+            f.noiseMean, f.noiseBase, f.noiseRise = sn["noise"]
+            f.seed, f.run, f.tag = sn["seed"], sn["run"], sn["tag"]
+            r = {"full_xyz": np.empty((m, 3), np.float32), "full_distance": np.empty(m, np.float32),
+                 "full_entity": np.empty(m, np.int32), "full_normal": np.empty((m, 3), np.float32),
+                 "full_is_hit": np.empty(m, np.int32), "xyz": np.empty((m, 3), np.float32),
+                 "distance": np.empty(m, np.float32), "entity": np.empty(m, np.int32),
+                 "normal": np.empty((m, 3), np.float32)}
+            o = outs[k]
+            for field, key, ct in (("fullXyz", "full_xyz", C.c_float), ("fullDistance", "full_distance", C.c_float),
+                                   ("fullEntityId", "full_entity", C.c_int32), ("fullNormal", "full_normal", C.c_float),
+                                   ("fullIsHit", "full_is_hit", C.c_int32), ("xyz", "xyz", C.c_float),
+                                   ("distance", "distance", C.c_float), ("entityId", "entity", C.c_int32),
+                                   ("normal", "normal", C.c_float)):
+                setattr(o, field, _p(r[key], ct))
+            res.append(r)
+        handles = (C.c_void_p * n)(*[self.scenes[sn["scene"]] for sn in sensors])
+        load().cpu_run(handles, frames, outs, n, threads)
+        for k, r in enumerate(res):
+            kept = outs[k].kept
+            r["kept"] = kept
+            for key in ("xyz", "distance", "entity", "normal"):
+                r[key] = r[key][:kept]
+        return res
+
+    def close(self):
+        for h in self.scenes:
+            if h:
+                load().cpu_scene_destroy(h)
+        self.scenes = []
+
+    def __del__(self):
+        self.close()
 
 
 def philox(ctr, key) -> np.ndarray:

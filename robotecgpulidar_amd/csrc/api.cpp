@@ -1494,6 +1494,7 @@ RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t*
 		CHECK_ARG(nodes_visited != nullptr && tris_tested != nullptr && instances_entered != nullptr);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
 		n->waitForResults();
+		CHECK_ARG(ray_count == n->capacity() / std::max(1, n->returnCount())); // the kernel writes one entry per ray
 		const int rays = ray_count;
 		DeviceBuffer a, b, c;
 		n->traversalStats(a.ensureT<int32_t>(rays), b.ensureT<int32_t>(rays), c.ensureT<int32_t>(rays));
@@ -1501,6 +1502,38 @@ RGL_API rgl_status_t rgl_node_raytrace_traversal_stats(rgl_node_t node, int32_t*
 		RGL_HIP(hipMemcpy(nodes_visited, a.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
 		RGL_HIP(hipMemcpy(tris_tested, b.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
 		RGL_HIP(hipMemcpy(instances_entered, c.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
+	});
+}
+
+RGL_API rgl_status_t rgl_graph_traversal_footprint(const rgl_node_t* raytrace_nodes, int32_t count, int64_t* out_bytes)
+{
+	return safeCall([&]() {
+		CHECK_ARG(raytrace_nodes != nullptr);
+		CHECK_ARG(count > 0);
+		CHECK_ARG(out_bytes != nullptr);
+		std::vector<std::shared_ptr<RaytraceNode>> nodes;
+		for (int32_t i = 0; i < count; ++i) {
+			nodes.push_back(nodeAs<RaytraceNode>(raytrace_nodes[i], "RaytraceNode"));
+			nodes.back()->waitForResults();
+		}
+		// 2^26 keys (512 MiB): room for 2^25 distinct lines (2 GiB of records) at half load.
+		constexpr uint32_t kKeys = 1u << 26;
+		DeviceBuffer keys, counters;
+		RGL_HIP(hipMemset(keys.ensure(sizeof(unsigned long long) * kKeys), 0, sizeof(unsigned long long) * kKeys));
+		RGL_HIP(hipMemset(counters.ensure(16), 0, 16));
+		auto* full = reinterpret_cast<uint32_t*>(counters.as<char>() + 8);
+		for (auto& n : nodes) {
+			n->footprint(keys.as<unsigned long long>(), kKeys - 1, full);
+			RGL_HIP(hipStreamSynchronize(n->stream()));
+		}
+		gpu::countNonzeroKeys(nullptr, keys.as<unsigned long long>(), kKeys, counters.as<unsigned long long>());
+		RGL_HIP(hipGetLastError());
+		unsigned long long host[2] = {0, 0};
+		RGL_HIP(hipMemcpy(host, counters.get(), 16, hipMemcpyDeviceToHost));
+		if ((host[1] & 0xffffffffull) != 0 || host[0] > kKeys / 2) {
+			throw std::runtime_error("traversal footprint exceeds the measurement's line set");
+		}
+		*out_bytes = static_cast<int64_t>(host[0]) * 64;
 	});
 }
 
@@ -1512,6 +1545,7 @@ RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t
 		CHECK_ARG(out_t != nullptr && out_instance != nullptr && out_prim != nullptr);
 		auto n = nodeAs<RaytraceNode>(node, "RaytraceNode");
 		n->waitForResults();
+		CHECK_ARG(ray_count == n->capacity() / std::max(1, n->returnCount())); // the kernel writes one entry per ray
 		const int rays = ray_count;
 		DeviceBuffer a, b, c;
 		n->bruteForce(a.ensureT<float>(rays), b.ensureT<int32_t>(rays), c.ensureT<int32_t>(rays));

@@ -75,14 +75,15 @@ struct alignas(16) LeafTri
 };
 static_assert(sizeof(LeafTri) == 48, "leaf triangle must be 48 B");
 
-// Instance record read by traversal when it enters an instance (64 B + 64 B, two cache-friendly halves).
+// Instance record read by traversal when it enters an instance (160 B).
 struct alignas(16) InstanceRec
 {
-	Mat3x4f worldToObject;      // FP32, composed in FP64 on the host and rounded once
+	Mat3x4f worldToObject;      // FP32 rounding of worldToObject64 (its linear part maps ray directions)
 	const void* blasNodes;      // BLAS of the instance's mesh, in the scene's node format
 	const LeafTri* blasTris;
+	Mat3x4d worldToObject64;    // FP64 inverse of the entity transform (maps the FP64 ray origin)
 };
-static_assert(sizeof(InstanceRec) == 64, "instance record must be 64 B");
+static_assert(sizeof(InstanceRec) == 160, "instance record must be 160 B");
 
 // Per-instance data used only when shading a hit (closest-hit stage).
 struct alignas(16) InstanceShade
@@ -155,10 +156,11 @@ struct RaytraceParams
 {
 	const Mat3x4f* rays;          // rays of the rays-node chain (world frame unless rayPreTransformed)
 	int32_t rayCount;
-	int32_t rayPreTransformed;    // 1: the world-frame ray is rayPreTransform * rays[i]
+	int32_t rayPreTransformed;    // 1: the world-frame ray is rayPreTransform * rays[i], composed in FP64
 	Mat3x4f rayPreTransform;
 	Mat3x4f rayOriginToWorld;     // cumulative ray transform of the rays chain
-	Mat3x4f worldToRayOrigin;     // its inverse, computed once on the host
+	Mat3x4f worldToRayOrigin;     // its inverse, computed once on the host (FP64, rounded)
+	Mat3x4d worldToRayOrigin64;   // the same inverse in FP64 (velocity distortion)
 	const Vec2f* ranges;          // 1 or rayCount entries
 	int32_t rangesCount;
 	const int32_t* ringIds;

@@ -611,7 +611,8 @@ RaytraceParams RaytraceNode::buildParams(const SceneView&) const
 	p.rayPreTransform = dr.pre;
 	p.rayCount = raysNode->rayCount();
 	p.rayOriginToWorld = raysNode->cumulativeRayTransform();
-	p.worldToRayOrigin = p.rayOriginToWorld.inverse();
+	p.worldToRayOrigin64 = inverse64(widen(p.rayOriginToWorld));
+	p.worldToRayOrigin = round32(p.worldToRayOrigin64);
 	if (raysNode->ranges() != nullptr) {
 		p.ranges = raysNode->ranges();
 		p.rangesCount = raysNode->rangesCount();
@@ -759,6 +760,13 @@ void RaytraceNode::traversalStats(int32_t* nodes, int32_t* tris, int32_t* insts)
 {
 	const SceneView view = scene->prepare(stream());
 	gpu::traversalStats(stream(), view, buildParams(view), nodes, tris, insts);
+	RGL_HIP(hipGetLastError());
+}
+
+void RaytraceNode::footprint(unsigned long long* keys, uint32_t mask, uint32_t* full)
+{
+	const SceneView view = scene->prepare(stream());
+	gpu::traversalFootprint(stream(), view, buildParams(view), keys, mask, full);
 	RGL_HIP(hipGetLastError());
 }
 

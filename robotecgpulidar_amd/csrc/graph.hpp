@@ -468,6 +468,7 @@ struct RaytraceNode : IPointsNode
 	}
 	// Traversal statistics of the last configuration (used by the benchmark for byte accounting).
 	void traversalStats(int32_t* nodes, int32_t* tris, int32_t* insts);
+	void footprint(unsigned long long* keys, uint32_t mask, uint32_t* full);
 	void bruteForce(float* t, int32_t* inst, int32_t* prim);
 	RaytraceParams buildParams(const SceneView& view) const;
 	bool overflowed();

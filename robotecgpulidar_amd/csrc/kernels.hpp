@@ -51,6 +51,13 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 // with the same traversal order as raytrace(); used for algorithmic-byte accounting.
 void traversalStats(hipStream_t stream, const SceneView& scene, const RaytraceParams& params, int32_t* nodesVisited,
                     int32_t* trisTested, int32_t* instancesEntered);
+// Footprint of raytrace(): inserts every 64 B line its rays read (BVH nodes in the scene's format, leaf triangles,
+// instance records, and the shading records, index triples, vertices and texture coordinates of the hits) into the
+// open-addressing set keys[0..mask] (key = line + 1, 0 = empty); *full is set if the set ran out of room.
+void traversalFootprint(hipStream_t stream, const SceneView& scene, const RaytraceParams& params,
+                        unsigned long long* keys, uint32_t mask, uint32_t* full);
+// *out += number of nonzero keys[0..n) (out must be zeroed by the caller).
+void countNonzeroKeys(hipStream_t stream, const unsigned long long* keys, size_t n, unsigned long long* out);
 // Brute-force closest hit over every triangle of every instance (testing aid; same intersection routine).
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
                        float* outT, int32_t* outInstance, int32_t* outPrim);

@@ -210,6 +210,109 @@ RGL_HD Mat3x4f TRS(Vec3f t, Vec3f rDeg, Vec3f s)
 	       Mat3x4f::scale(s.x, s.y, s.z);
 }
 
+// ------------------------------------------------------------------------------------------------ FP64 composition
+// Sensor poses and entity transforms at large world coordinates lose their small offsets when composed in FP32
+// (a ray at 40 m from the origin has its direction rounded to ~4e-6 if it is formed as M(0,0,1) - M(0,0,0)). Poses
+// are therefore composed in FP64 and rounded to FP32 once; the FP64 origin is kept for the world -> object transform
+// and for the hit distance. Device code fixes every rounding (fused multiply-adds in a fixed order), so a ray gets the
+// same bits in every kernel.
+struct Vec3d
+{
+	double x, y, z;
+};
+
+struct Mat3x4d
+{
+	double rc[3][4];
+};
+
+RGL_HD double fma64(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __fma_rn(a, b, c);
+#else
+	return std::fma(a, b, c);
+#endif
+}
+
+// a0*b0 + a1*b1 + a2*b2 + c with the same rounding order everywhere.
+RGL_HD double dot3d(double a0, double b0, double a1, double b1, double a2, double b2, double c)
+{
+	return fma64(a2, b2, fma64(a1, b1, fma64(a0, b0, c)));
+}
+
+RGL_HD Mat3x4d widen(const Mat3x4f& m)
+{
+	Mat3x4d r;
+	for (int y = 0; y < 3; ++y) {
+		for (int x = 0; x < 4; ++x) {
+			r.rc[y][x] = m.rc[y][x];
+		}
+	}
+	return r;
+}
+
+RGL_HD Mat3x4f round32(const Mat3x4d& m)
+{
+	Mat3x4f r;
+	for (int y = 0; y < 3; ++y) {
+		for (int x = 0; x < 4; ++x) {
+			r.rc[y][x] = static_cast<float>(m.rc[y][x]);
+		}
+	}
+	return r;
+}
+
+RGL_HD Mat3x4d operator*(const Mat3x4d& a, const Mat3x4d& b)
+{
+	Mat3x4d r;
+	for (int y = 0; y < 3; ++y) {
+		for (int x = 0; x < 4; ++x) {
+			r.rc[y][x] = dot3d(a.rc[y][0], b.rc[0][x], a.rc[y][1], b.rc[1][x], a.rc[y][2], b.rc[2][x],
+			                   x == 3 ? a.rc[y][3] : 0.0);
+		}
+	}
+	return r;
+}
+
+RGL_HD Vec3d apply(const Mat3x4d& m, Vec3d p)
+{
+	return {dot3d(m.rc[0][0], p.x, m.rc[0][1], p.y, m.rc[0][2], p.z, m.rc[0][3]),
+	        dot3d(m.rc[1][0], p.x, m.rc[1][1], p.y, m.rc[1][2], p.z, m.rc[1][3]),
+	        dot3d(m.rc[2][0], p.x, m.rc[2][1], p.y, m.rc[2][2], p.z, m.rc[2][3])};
+}
+
+RGL_HD Vec3d apply(const Mat3x4f& m, Vec3d p) { return apply(widen(m), p); }
+
+RGL_HD Vec3f round32(Vec3d v) { return {static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z)}; }
+
+// Affine inverse in FP64 (adjugate over the determinant); zero matrix if singular.
+RGL_HD Mat3x4d inverse64(const Mat3x4d& a)
+{
+	const double c00 = a.rc[1][1] * a.rc[2][2] - a.rc[1][2] * a.rc[2][1];
+	const double c01 = a.rc[1][2] * a.rc[2][0] - a.rc[1][0] * a.rc[2][2];
+	const double c02 = a.rc[1][0] * a.rc[2][1] - a.rc[1][1] * a.rc[2][0];
+	const double det = a.rc[0][0] * c00 + a.rc[0][1] * c01 + a.rc[0][2] * c02;
+	Mat3x4d r{};
+	if (det == 0.0) {
+		return r;
+	}
+	const double id = 1.0 / det;
+	r.rc[0][0] = c00 * id;
+	r.rc[0][1] = (a.rc[0][2] * a.rc[2][1] - a.rc[0][1] * a.rc[2][2]) * id;
+	r.rc[0][2] = (a.rc[0][1] * a.rc[1][2] - a.rc[0][2] * a.rc[1][1]) * id;
+	r.rc[1][0] = c01 * id;
+	r.rc[1][1] = (a.rc[0][0] * a.rc[2][2] - a.rc[0][2] * a.rc[2][0]) * id;
+	r.rc[1][2] = (a.rc[0][2] * a.rc[1][0] - a.rc[0][0] * a.rc[1][2]) * id;
+	r.rc[2][0] = c02 * id;
+	r.rc[2][1] = (a.rc[0][1] * a.rc[2][0] - a.rc[0][0] * a.rc[2][1]) * id;
+	r.rc[2][2] = (a.rc[0][0] * a.rc[1][1] - a.rc[0][1] * a.rc[1][0]) * id;
+	for (int y = 0; y < 3; ++y) {
+		r.rc[y][3] = -(r.rc[y][0] * a.rc[0][3] + r.rc[y][1] * a.rc[1][3] + r.rc[y][2] * a.rc[2][3]);
+	}
+	return r;
+}
+
 static_assert(sizeof(Mat3x4f) == 48, "Mat3x4f must match rgl_mat3x4f");
 static_assert(sizeof(Vec3f) == 12, "Vec3f must match rgl_vec3f");
 

@@ -112,6 +112,36 @@ __device__ __forceinline__ TravRay makeTravRay(Vec3f o, Vec3f d)
 	return r;
 }
 
+// Object-space ray of an instance: the FP64 world origin through the FP64 world->object transform, rounded once, and
+// the FP32 direction through the FP32 linear part.
+__device__ __forceinline__ TravRay objectRay(const InstanceRec& ir, Vec3d o64, Vec3f d)
+{
+	return makeTravRay(round32(apply(ir.worldToObject64, o64)), mulDir(ir.worldToObject, d));
+}
+
+// FP64 world origin of the ray a thread traces, kept in LDS ([component][thread]) between ray setup, instance entries
+// during traversal and shading, so it takes no registers in the traversal loop.
+template<int kBlock>
+__device__ __forceinline__ double* originSlot()
+{
+	__shared__ double slot[3 * kBlock];
+	return slot + threadIdx.x;
+}
+template<int kBlock>
+__device__ __forceinline__ void storeOrigin(Vec3d o)
+{
+	double* s = originSlot<kBlock>();
+	s[0] = o.x;
+	s[kBlock] = o.y;
+	s[2 * kBlock] = o.z;
+}
+template<int kBlock>
+__device__ __forceinline__ Vec3d loadOrigin()
+{
+	const double* s = originSlot<kBlock>();
+	return Vec3d{s[0], s[kBlock], s[2 * kBlock]};
+}
+
 struct HitRec
 {
 	float t;
@@ -172,6 +202,42 @@ struct TravStats
 {
 	int nodes, tris, insts;
 };
+
+// Set of the 64 B lines a launch reads (footprint measurement): open addressing over mask + 1 keys, key = line + 1,
+// 0 = empty. A line that finds no free slot within kLineProbes probes sets *full.
+struct LineSet
+{
+	unsigned long long* keys;
+	uint32_t mask;
+	uint32_t* full;
+};
+constexpr int kLineProbes = 256;
+
+__device__ __forceinline__ uint32_t lineHash(unsigned long long k)
+{
+	k ^= k >> 33;
+	k *= 0xff51afd7ed558ccdull;
+	k ^= k >> 33;
+	return static_cast<uint32_t>(k);
+}
+
+__device__ void touchLines(const LineSet* s, const void* p, int bytes)
+{
+	const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+	for (unsigned long long line = a >> 6; line <= (a + bytes - 1) >> 6; ++line) {
+		const unsigned long long key = line + 1;
+		uint32_t h = lineHash(key) & s->mask;
+		bool placed = false;
+		for (int probe = 0; probe < kLineProbes && !placed; ++probe) {
+			const unsigned long long prev = atomicCAS(&s->keys[h], 0ull, key);
+			placed = prev == 0ull || prev == key;
+			h = (h + 1) & s->mask;
+		}
+		if (!placed) {
+			atomicOr(s->full, 1u);
+		}
+	}
+}
 
 // Upper t bound of the slab tests while the closest hit so far is at tBest: widened like the far plane and kept
 // finite, so that a box whose near distance is +inf (the empty-slot box of Bvh4FNode) is never entered.
@@ -237,14 +303,17 @@ __device__ __forceinline__ float ubyte(uint32_t w, int c) { return static_cast<f
 
 // Closest hit of one ray against the two-level BVH; returns false if nothing is hit in (0, tmax].
 // kFmt selects the node format of the TLAS and BLASes (BvhFormat). With kStats the canonical binary trees are
-// traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted.
+// traversed (sc.tlas2 and each instance's binary BLAS) and visits are counted. o is the FP32 rounding of the FP64
+// origin the caller stored with storeOrigin<kBlock>; instance entries transform the FP64 one.
 //
 // One stack serves both levels. Entering an instance records the stack depth (instBase); when pops bring the depth
 // back to it, the BLAS is finished and traversal continues in the TLAS. The ray of the node steps (origin and
 // reciprocal direction of the current level) and the object-space ray record change only at the top of the node
 // loop, on instance entry and on the return to the TLAS, so the pop loop carries no ray state.
-template<bool kStats, int kFmt, int kBlock = kRtBlock>
-__device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st)
+// With kFoot every record read (nodes in format kFmt, leaf triangles, instance records) is added to *ls.
+template<bool kStats, int kFmt, int kBlock = kRtBlock, bool kFoot = false>
+__device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, HitRec& best, TravStats* st,
+                             const LineSet* ls = nullptr)
 {
 	static_assert(!kStats || kFmt == kBvh2, "traversal statistics are defined on the binary trees");
 	// Short stack in LDS, [depth][thread] so a wave's pushes hit distinct banks; deeper entries spill to scratch.
@@ -312,7 +381,10 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			inst = ~node;
 			instBase = sp;
 			const InstanceRec& ir = sc.instances[inst];
-			obj = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
+			if constexpr (kFoot) {
+				touchLines(ls, &ir, sizeof(InstanceRec));
+			}
+			obj = objectRay(ir, loadOrigin<kBlock>(), d);
 			ro = obj.o;
 			ri = obj.inv;
 			nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
@@ -330,6 +402,10 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		}
 		if (kStats) {
 			st->nodes++;
+		}
+		if constexpr (kFoot) {
+			const int nodeBytes = kFmt == kBvh4F ? sizeof(Bvh4FNode) : (kFmt == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+			touchLines(ls, static_cast<const char*>(nodes) + static_cast<size_t>(node) * nodeBytes, nodeBytes);
 		}
 		int next;
 		if constexpr (kFmt == kBvh2) {
@@ -459,6 +535,9 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				if (kStats) {
 					st->tris++;
 				}
+				if constexpr (kFoot) {
+					touchLines(ls, tris + leaf, sizeof(LeafTri));
+				}
 				const LeafTri lt = loadLeafTri(tris, leaf);
 				float t, vNum, wNum, det;
 				if (intersectTri(obj, lt, t, vNum, wNum, det) && t > 0.0f && t <= tmax &&
@@ -502,20 +581,26 @@ __device__ Mat3x4f beamSampleTransform(float hHalf, float vHalf, int layer, int 
 	return Mat3x4f::rotationRad(vAngle, 0.0f, 0.0f) * Mat3x4f::rotationRad(0.0f, hAngle, 0.0f);
 }
 
-// World-frame pose of ray i of the rays chain; a transform the chain deferred to this kernel is applied here (the
-// device matrix product rounds the same way in every kernel, so the pose equals the transform node's output).
-__device__ __forceinline__ Mat3x4f rayWorldPose(const RaytraceParams& p, int i)
+// World-frame pose of ray i of the rays chain in FP64; a transform the chain deferred to this kernel is composed here
+// in FP64 (exact products of the FP32 operands, fixed rounding order), so every kernel gets the same bits.
+__device__ __forceinline__ Mat3x4d rayWorldPose64(const RaytraceParams& p, int i)
 {
-	return p.rayPreTransformed ? p.rayPreTransform * p.rays[i] : p.rays[i];
+	return p.rayPreTransformed ? widen(p.rayPreTransform) * widen(p.rays[i]) : widen(p.rays[i]);
 }
+__device__ __forceinline__ Mat3x4f rayWorldPose(const RaytraceParams& p, int i) { return round32(rayWorldPose64(p, i)); }
 
-// Ray origin = M*(0,0,0); direction = M*(0,0,1) - origin, evaluated as the reference does (not column 2 verbatim).
+// Ray origin = M*(0,0,0) and direction = M*(0,0,1) - M*(0,0,0), i.e. the matrix's fourth and third columns. (The
+// reference subtracts in FP32, src/gpu/optixPrograms.cu:222-228, which rounds the direction to the ulp of the
+// translation: ~4e-6 for a sensor 40 m from the origin, millimetres at grazing incidence. The exact column is used.)
 __device__ __forceinline__ void rayOriginDir(const Mat3x4f& m, Vec3f& o, Vec3f& d)
 {
 	o = Vec3f{m.rc[0][3], m.rc[1][3], m.rc[2][3]};
-	d = Vec3f{__fsub_rn(__fadd_rn(m.rc[0][2], m.rc[0][3]), m.rc[0][3]),
-	          __fsub_rn(__fadd_rn(m.rc[1][2], m.rc[1][3]), m.rc[1][3]),
-	          __fsub_rn(__fadd_rn(m.rc[2][2], m.rc[2][3]), m.rc[2][3])};
+	d = Vec3f{m.rc[0][2], m.rc[1][2], m.rc[2][2]};
+}
+__device__ __forceinline__ Vec3d originOf(const Mat3x4d& m) { return Vec3d{m.rc[0][3], m.rc[1][3], m.rc[2][3]}; }
+__device__ __forceinline__ Vec3f directionOf(const Mat3x4d& m)
+{
+	return Vec3f{static_cast<float>(m.rc[0][2]), static_cast<float>(m.rc[1][2]), static_cast<float>(m.rc[2][2])};
 }
 
 struct Shade
@@ -533,19 +618,22 @@ struct Shade
 
 // Closest-hit program for one sample (src/gpu/optixPrograms.cu:96-216). `full` also computes the fields that the
 // beam reduction does not need (normal, velocities).
-__device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float minRange, Vec3f so, Vec3f sd,
+// so64: the sample's FP64 world origin.
+__device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float minRange, Vec3d so64, Vec3f sd,
                              const HitRec& h, bool full)
 {
 	Shade s;
 	const InstanceShade& is = sc.shades[h.inst];
 	const Vec3i tri = is.indices[h.prim];
 	const Vec3f A = is.vertices[tri.x], B = is.vertices[tri.y], C = is.vertices[tri.z];
-	// Barycentric point A + u(B-A) + v(C-A): exact when A, B, C share a coordinate (axis-aligned faces).
+	// Barycentric point A + u(B-A) + v(C-A): exact when A, B, C share a coordinate (axis-aligned faces). It goes to
+	// world space and to the distance in FP64 (src/gpu/optixPrograms.cu:122-129 takes the length in FP64).
 	const Vec3f hitObject = A + h.u * (B - A) + h.v * (C - A);
-	const Vec3f hitWorld = is.objectToWorld * hitObject;
-	const double dx = static_cast<double>(hitWorld.x) - so.x;
-	const double dy = static_cast<double>(hitWorld.y) - so.y;
-	const double dz = static_cast<double>(hitWorld.z) - so.z;
+	const Vec3d hitWorld64 = apply(is.objectToWorld, Vec3d{hitObject.x, hitObject.y, hitObject.z});
+	const Vec3f hitWorld = round32(hitWorld64);
+	const double dx = hitWorld64.x - so64.x;
+	const double dy = hitWorld64.y - so64.y;
+	const double dz = hitWorld64.z - so64.z;
 	const double distance = sqrt(dx * dx + dy * dy + dz * dz);
 	if (distance < minRange) {
 		s.isHit = 0;
@@ -555,8 +643,10 @@ __device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float
 	}
 	s.isHit = 1;
 	s.distance = static_cast<float>(distance);
-	const Vec3f wA = is.objectToWorld * A, wB = is.objectToWorld * B, wC = is.objectToWorld * C;
-	const Vec3f n = normalized(cross(wB - wA, wC - wA));
+	// World normal from the world-space edges (src/gpu/optixPrograms.cu:139-145), formed as the entity's linear part
+	// applied to the object-space edges: the same vectors without the translation, so large world coordinates do not
+	// cancel in the differences.
+	const Vec3f n = normalized(cross(mulDir(is.objectToWorld, B - A), mulDir(is.objectToWorld, C - A)));
 	const float cosI = fabsf(dot(n, sd));
 	s.normal = n;
 	s.incident = acosf(cosI);
@@ -598,7 +688,7 @@ __device__ Shade shadeSample(const SceneView& sc, const RaytraceParams& p, float
 		const Mat3x4f rotInv = p.rayOriginToWorld.rotation().inverse();
 		const Vec3f absInSensor = rotInv * absVel;
 		const Vec3f relLin = absInSensor - p.sensorLinearVelocity;
-		const Vec3f hitRays = p.worldToRayOrigin * hitWorld;
+		const Vec3f hitRays = round32(apply(p.worldToRayOrigin64, hitWorld64));
 		const Vec3f relAng = Vec3f{0.0f, 0.0f, 0.0f} - cross(p.sensorAngularVelocity, hitRays);
 		s.absVel = absVel;
 		s.relVel = relLin + relAng;
@@ -678,18 +768,19 @@ __device__ void writeBeamShared(const RaytraceOutputs& out, const RaytraceParams
 	}
 }
 
-// The beam's ray after velocity distortion (src/gpu/optixPrograms.cu:61-70).
-__device__ __forceinline__ Mat3x4f distortedRay(const RaytraceParams& p, int beam, const Mat3x4f& rayWorld)
+// The beam's ray after velocity distortion (src/gpu/optixPrograms.cu:61-70): the sensor-frame ray moved by the
+// sensor's motion over the ray's time offset, composed in FP64 (the motion itself, TRS, is formed in FP32).
+__device__ __forceinline__ Mat3x4d distortedRay64(const RaytraceParams& p, int beam, const Mat3x4d& rayWorld)
 {
 	if (!p.doDistortion) {
 		return rayWorld;
 	}
-	const Mat3x4f rayLocal = p.worldToRayOrigin * rayWorld;
+	const Mat3x4d rayLocal = p.worldToRayOrigin64 * rayWorld;
 	const float toDeg = static_cast<float>(180.0 / M_PI);
 	const float off = p.timeOffsetsMs[beam];
 	const Vec3f t = (off * p.sensorLinearVelocity) * 0.001f;
 	const Vec3f rdeg = (off * (p.sensorAngularVelocity * toDeg)) * 0.001f;
-	return p.rayOriginToWorld * (TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f}) * rayLocal);
+	return widen(p.rayOriginToWorld) * (widen(TRS(t, rdeg, Vec3f{1.0f, 1.0f, 1.0f})) * rayLocal);
 }
 
 // One beam traced with a single sample (no divergence): the beam reduction degenerates to "every return is the
@@ -698,32 +789,31 @@ template<int kFmt, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBeam(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
                                              const int beam)
 {
-	const Mat3x4f rayWorld = rayWorldPose(p, beam);
+	const Mat3x4d rayWorld64 = rayWorldPose64(p, beam);
 	const int R = p.returnCount;
-	writeBeamShared(out, p, beam, rayWorld);
+	writeBeamShared(out, p, beam, round32(rayWorld64));
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
 	if (p.mask != nullptr && p.mask[beam] == 0) {
-		Vec3f wo, wd;
-		rayOriginDir(rayWorld, wo, wd);
+		const Vec3f wo = round32(originOf(rayWorld64)), wd = directionOf(rayWorld64);
 		for (int r = 0; r < R; ++r) {
 			writeReturnNonHit(out, wo, wd, beam * R + r, returnTypeOf(p.returnMode, r), p.farNonHitDistance);
 		}
 		return;
 	}
-	const Mat3x4f ray = distortedRay(p, beam, rayWorld);
-	Vec3f so, sd;
-	rayOriginDir(ray, so, sd);
+	const Mat3x4d ray64 = distortedRay64(p, beam, rayWorld64);
+	const Vec3f so = round32(originOf(ray64)), sd = directionOf(ray64);
+	storeOrigin<kBlock>(originOf(ray64));
 	HitRec h;
 	float d0 = p.farNonHitDistance;
 	const bool hit = traceClosest<false, kFmt, kBlock>(sc, so, sd, range.y, h, nullptr);
 	// Points are placed along the undistorted world ray; without distortion that is the traced ray, otherwise it is
-	// reloaded here rather than kept in registers through the traversal.
+	// recomputed here rather than kept in registers through the traversal.
 	Vec3f wo = so, wd = sd;
 	if (p.doDistortion) {
 		rayOriginDir(rayWorldPose(p, beam), wo, wd);
 	}
 	if (hit) {
-		const Shade sh = shadeSample(sc, p, range.x, so, sd, h, p.velocityRequested != 0);
+		const Shade sh = shadeSample(sc, p, range.x, loadOrigin<kBlock>(), sd, h, p.velocityRequested != 0);
 		if (sh.isHit) {
 			for (int r = 0; r < R; ++r) {
 				const uint8_t type = returnTypeOf(p.returnMode, r);
@@ -750,7 +840,8 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
                                                  const int beam)
 {
 	const int lane = threadIdx.x & 63;
-	const Mat3x4f rayWorld = rayWorldPose(p, beam);
+	const Mat3x4d rayWorld64 = rayWorldPose64(p, beam);
+	const Mat3x4f rayWorld = round32(rayWorld64);
 	const int R = p.returnCount;
 	if (lane == 0) {
 		writeBeamShared(out, p, beam, rayWorld);
@@ -766,7 +857,9 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		}
 		return;
 	}
-	const Mat3x4f ray = distortedRay(p, beam, rayWorld);
+	const Mat3x4d ray64 = distortedRay64(p, beam, rayWorld64);
+	const Mat3x4f ray = round32(ray64);
+	storeOrigin<kRtBlock>(originOf(ray64)); // every sample of the beam starts at the beam's origin
 	Vec3f so{0.0f, 0.0f, 0.0f}, sd{0.0f, 0.0f, 1.0f};
 	HitRec h;
 	h.t = 0.0f;
@@ -783,7 +876,7 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		}
 		rayOriginDir(sr, so, sd);
 		if (traceClosest<false, kFmt>(sc, so, sd, range.y, h, nullptr)) {
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, false);
+			const Shade sh = shadeSample(sc, p, range.x, loadOrigin<kRtBlock>(), sd, h, false);
 			isHit = sh.isHit;
 			dist = sh.distance;
 			inten = sh.intensity;
@@ -843,7 +936,7 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		}
 		const int sel = type == 3 ? sS : (type == 2 ? lS : (type == 1 ? stS : (type == 5 ? ssS : fS)));
 		if (lane == sel) {
-			const Shade sh = shadeSample(sc, p, range.x, so, sd, h, needVel);
+			const Shade sh = shadeSample(sc, p, range.x, loadOrigin<kRtBlock>(), sd, h, needVel);
 			writeReturnHit(out, wo, wd, rp, type, sh);
 		}
 	}
@@ -943,8 +1036,9 @@ __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, Raytra
 		return;
 	}
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
-	Vec3f so, sd;
-	rayOriginDir(rayWorldPose(p, beam), so, sd);
+	const Mat3x4d world = rayWorldPose64(p, beam);
+	const Vec3f so = round32(originOf(world)), sd = directionOf(world);
+	storeOrigin<kRtBlock>(originOf(world));
 	HitRec h;
 	TravStats st{0, 0, 0};
 	const bool masked = p.mask != nullptr && p.mask[beam] == 0;
@@ -956,6 +1050,61 @@ __global__ void __launch_bounds__(kRtBlock) kTraversalStats(SceneView sc, Raytra
 	instancesEntered[beam] = st.insts;
 }
 
+// Footprint of a raytrace launch: every record its rays read is added to *ls, including the shading records of the
+// hits (instance shading record, index triple, vertices, texture coordinates and texel). Divergent beams trace their
+// 58 samples in one thread.
+template<int kFmt>
+__global__ void __launch_bounds__(kRtBlock) kTraversalFootprint(SceneView sc, RaytraceParams p, LineSet ls)
+{
+	const int beam = blockIdx.x * kRtBlock + threadIdx.x;
+	if (beam >= p.rayCount || (p.mask != nullptr && p.mask[beam] == 0)) {
+		return;
+	}
+	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
+	const Mat3x4d ray64 = distortedRay64(p, beam, rayWorldPose64(p, beam));
+	const Mat3x4f ray = round32(ray64);
+	storeOrigin<kRtBlock>(originOf(ray64));
+	const int samples = p.hHalfDivergence > 0.0f && p.vHalfDivergence > 0.0f ? kBeamSamples : 1;
+	for (int s = 0; s < samples; ++s) {
+		Mat3x4f sr = ray;
+		if (s > 0) {
+			sr = ray * beamSampleTransform(p.hHalfDivergence, p.vHalfDivergence, (s - 1) / kBeamVertices,
+			                               (s - 1) % kBeamVertices);
+		}
+		Vec3f so, sd;
+		rayOriginDir(sr, so, sd);
+		HitRec h;
+		if (!traceClosest<false, kFmt, kRtBlock, true>(sc, so, sd, range.y, h, nullptr, &ls)) {
+			continue;
+		}
+		const InstanceShade& is = sc.shades[h.inst];
+		touchLines(&ls, &is, sizeof(InstanceShade));
+		touchLines(&ls, is.indices + h.prim, sizeof(Vec3i));
+		const Vec3i tri = is.indices[h.prim];
+		const int v[3] = {tri.x, tri.y, tri.z};
+		for (int k = 0; k < 3; ++k) {
+			touchLines(&ls, is.vertices + v[k], sizeof(Vec3f));
+			if (is.uvs != nullptr && is.texels != nullptr) {
+				touchLines(&ls, is.uvs + v[k], sizeof(Vec2f));
+			}
+		}
+	}
+}
+
+__global__ void kCountKeys(const unsigned long long* keys, size_t n, unsigned long long* out)
+{
+	unsigned long long c = 0;
+	for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+		c += keys[i] != 0ull ? 1 : 0;
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		c += __shfl_down(c, off, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		atomicAdd(out, c);
+	}
+}
+
 __global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytraceParams p, float* outT, int32_t* outInst,
                             int32_t* outPrim)
 {
@@ -964,8 +1113,9 @@ __global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytracePara
 		return;
 	}
 	const Vec2f range = p.rangesCount == 1 ? p.ranges[0] : p.ranges[beam];
-	Vec3f o, d;
-	rayOriginDir(rayWorldPose(p, beam), o, d);
+	const Mat3x4d world = rayWorldPose64(p, beam);
+	const Vec3d o64 = originOf(world);
+	const Vec3f d = directionOf(world);
 	HitRec best;
 	best.t = range.y;
 	best.inst = INT_MAX;
@@ -973,7 +1123,7 @@ __global__ void kBruteForce(SceneView sc, const int32_t* triCounts, RaytracePara
 	bool found = false;
 	for (int i = 0; i < sc.instanceCount; ++i) {
 		const InstanceRec& ir = sc.instances[i];
-		const TravRay r = makeTravRay(ir.worldToObject * o, mulDir(ir.worldToObject, d));
+		const TravRay r = objectRay(ir, o64, d);
 		for (int k = 0; k < triCounts[i]; ++k) {
 			const LeafTri tri = ir.blasTris[k];
 			float t, vNum, wNum, det;
@@ -994,7 +1144,7 @@ __global__ void kTransformRays(int n, const Mat3x4f* in, Mat3x4f* out, Mat3x4f t
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) {
-		out[i] = tf * in[i];
+		out[i] = round32(widen(tf) * widen(in[i])); // the same FP64 composition as a deferred transform
 	}
 }
 
@@ -1070,6 +1220,26 @@ void traversalStats(hipStream_t stream, const SceneView& scene, const RaytracePa
 	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
 	hipLaunchKernelGGL(kTraversalStats, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, nodesVisited, trisTested,
 	                   instancesEntered);
+}
+
+void traversalFootprint(hipStream_t stream, const SceneView& scene, const RaytraceParams& params,
+                        unsigned long long* keys, uint32_t mask, uint32_t* full)
+{
+	if (params.rayCount <= 0 || scene.tlasNodes == nullptr) {
+		return;
+	}
+	const LineSet ls{keys, mask, full};
+	const int blocks = (params.rayCount + kRtBlock - 1) / kRtBlock;
+	switch (scene.format) {
+		case kBvh4F: hipLaunchKernelGGL(kTraversalFootprint<kBvh4F>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
+		case kBvh4Q: hipLaunchKernelGGL(kTraversalFootprint<kBvh4Q>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
+		default: hipLaunchKernelGGL(kTraversalFootprint<kBvh2>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
+	}
+}
+
+void countNonzeroKeys(hipStream_t stream, const unsigned long long* keys, size_t n, unsigned long long* out)
+{
+	hipLaunchKernelGGL(kCountKeys, dim3(1024), dim3(256), 0, stream, keys, n, out);
 }
 
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,

@@ -7,50 +7,6 @@
 
 namespace rgl {
 
-namespace {
-
-// FP64 inverse of an affine 3x4 matrix, rounded to FP32 once. Used for world->object instance transforms so
-// that large world coordinates do not accumulate FP32 error in the inverse.
-Mat3x4f inverseFp64(const Mat3x4f& m)
-{
-	double a[3][4];
-	for (int r = 0; r < 3; ++r) {
-		for (int c = 0; c < 4; ++c) {
-			a[r][c] = m.rc[r][c];
-		}
-	}
-	const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
-	const double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
-	const double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
-	const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
-	if (det == 0.0) {
-		return Mat3x4f::zero();
-	}
-	const double id = 1.0 / det;
-	double r[3][3];
-	r[0][0] = c00 * id;
-	r[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
-	r[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
-	r[1][0] = c01 * id;
-	r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
-	r[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
-	r[2][0] = c02 * id;
-	r[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
-	r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
-	Mat3x4f out;
-	for (int i = 0; i < 3; ++i) {
-		double t = 0.0;
-		for (int k = 0; k < 3; ++k) {
-			out.rc[i][k] = static_cast<float>(r[i][k]);
-			t -= r[i][k] * a[k][3];
-		}
-		out.rc[i][3] = static_cast<float>(t);
-	}
-	return out;
-}
-
-} // namespace
-
 // ------------------------------------------------------------------------------------------------ Mesh
 Mesh::Mesh(const Vec3f* v, int vc, const Vec3i* idx, int ic) : vertexCount(vc), triCount(ic)
 {
@@ -547,7 +503,8 @@ void Scene::rebuild(hipStream_t s)
 	for (int i = 0; i < n; ++i) {
 		const Entity& e = *entities[i];
 		Blas& blas = entities[i]->tracedBlas(s);
-		inst[i].worldToObject = inverseFp64(e.transform);
+		inst[i].worldToObject64 = inverse64(widen(e.transform));
+		inst[i].worldToObject = round32(inst[i].worldToObject64);
 		inst[i].blasNodes = blas.nodesFor(s, fmt);
 		inst[i].blasTris = blas.tris.as<LeafTri>();
 		InstanceShade& sh = shade[i];

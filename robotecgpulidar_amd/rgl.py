@@ -178,6 +178,7 @@ _PROTOTYPES = [
     ("rgl_scene_set_tlas_refit", [C.c_void_p, C.c_bool]),
     ("rgl_node_raytrace_traversal_stats", [c_node, i32p, i32p, i32p, C.c_int32]),
     ("rgl_node_raytrace_brute_force", [c_node, f32p, i32p, i32p, C.c_int32]),
+    ("rgl_graph_traversal_footprint", [P(c_node), C.c_int32, P(C.c_int64)]),
     ("rgl_noise_normals", [C.c_uint64, C.c_uint32, C.c_uint32, f32p, C.c_int32]),
     ("rgl_node_raytrace_set_kernel_timing", [c_node, C.c_bool]),
     ("rgl_node_raytrace_get_kernel_timing", [c_node, P(C.c_double), i32p]),
@@ -454,6 +455,14 @@ def execution_order(node: Node) -> list[int]:
     arr = (c_node * max(count.value, 1))()
     check("rgl_graph_get_execution_order", node.h, arr, C.byref(count))
     return [arr[i] for i in range(count.value)]
+
+
+def traversal_footprint(raytrace_nodes) -> int:
+    """Unique bytes of BVH and scene records the given raytrace nodes' launches read (rgl_graph_traversal_footprint)."""
+    arr = (c_node * len(raytrace_nodes))(*[n.h.value for n in raytrace_nodes])
+    out = C.c_int64()
+    check("rgl_graph_traversal_footprint", arr, len(raytrace_nodes), C.byref(out))
+    return out.value
 
 
 def get_priority(node: Node) -> int:

@@ -161,7 +161,8 @@ def check_golden(gpu: dict, hit, entity, ambiguous, idx, values: dict, return_mo
     return points `idx`). XYZ and normals are compared at `exact_idx` (positions into idx; default all of idx).
     Bar: hit masks equal except on geometrically ambiguous beams, and for single returns at most max(1, 1e-5 n)
     differences; entity ids equal on unambiguous beams; distance within max(1 mm, 1e-5 relative), XYZ within 2 mm,
-    normals within 1e-4, intensity within 1e-4 relative + 2e-3 where no ambiguity bit blocks the quantity.
+    normals within 1e-4, intensity within 1e-4 relative + 2e-3 where no ambiguity bit blocks the quantity. At least
+    `min_compared` distances and intensities together must be compared.
     Returns counts and the largest distance error."""
     n = len(hit)
     beams = len(ambiguous)
@@ -182,7 +183,8 @@ def check_golden(gpu: dict, hit, entity, ambiguous, idx, values: dict, return_mo
     block_i = np.array([blocking_bits(int(t), "intensity") for t in types], np.int64)
     sel = both & ((amb[idx] & block_d) == 0)
     sel_i = both & ((amb[idx] & block_i) == 0)
-    assert sel.sum() >= min_compared, (label, int(sel.sum()))
+    compared = int(sel.sum()) + (int(sel_i.sum()) if G_INTEN in values else 0)
+    assert compared >= min_compared, (label, int(sel.sum()), int(sel_i.sum()))
     d_o = values[G_DIST][sel].astype(np.float64)
     d_g = gpu[G_DIST][idx][sel].astype(np.float64)
     err = np.abs(d_g - d_o)
@@ -202,5 +204,6 @@ def check_golden(gpu: dict, hit, entity, ambiguous, idx, values: dict, return_mo
     non_hit = (hit[idx] == 0) & (g_hit[idx] == 0)
     np.testing.assert_array_equal(gpu[G_DIST][idx][non_hit], values[G_DIST][non_hit])
     return {"points": n, "hits": int(hit.sum()), "mask_differences": int(hit_mism.sum()),
-            "ambiguous_beams": int((np.asarray(ambiguous) != 0).sum()), "compared": int(sel.sum()),
+            "ambiguous_beams": int((np.asarray(ambiguous) != 0).sum()), "compared_distances": int(sel.sum()),
+            "compared_intensities": int(sel_i.sum()) if G_INTEN in values else 0,
             "max_dist_err_m": float(err.max(initial=0.0))}

@@ -50,6 +50,27 @@ def test_config1_frame_matches_oracle(rgl_lib, oracle):
     assert err.max() <= 2e-3
 
 
+@pytest.mark.parametrize("offset", [(15000.0, -8000.0, 0.0), (-120000.0, 60000.0, 35.0)])
+def test_large_world_coordinates_match_oracle(rgl_lib, oracle, offset):
+    """The C1 scene and lidar moved far from the world origin (ulp of the coordinates 1-8 mm): the sensor pose is
+    composed with the rays in FP64, instance entry maps the FP64 origin, and distances come from FP64 hit points, so
+    distances, hit masks, entity ids and normals still meet the parity bar against the FP64 oracle fed the exact
+    (FP64) world rays; XYZ, an FP32 field in world coordinates, is within 3 ulp of the coordinates."""
+    base, lidar = S.config1_scene(), S.config1_lidar()
+    shift = S.translation(*offset)
+    spec = S.SceneSpec(list(base.meshes), [S.Instance(i.mesh, S.f32(S.mul(shift, i.transform)), i.entity_id)
+                                           for i in base.instances], [])
+    pose = S.f32(S.mul(shift, lidar.pose))
+    case = Case(spec, lidar.rays, pose=pose, ranges=np.array([lidar.range], np.float32))
+    fields = [XYZ, DIST, ENT, NORMAL, IS_HIT]
+    gpu = run_gpu(rgl_lib, case, fields)
+    orc = run_oracle(oracle, case, fields=fields)
+    same = compare(gpu, orc, len(lidar.rays), f"C1 at {offset}")
+    ulp = float(np.spacing(np.float32(max(abs(c) for c in offset))))
+    assert np.abs(gpu[XYZ][same].astype(np.float64) - orc[XYZ][same]).max() <= 3 * ulp + 2e-3
+    assert same.sum() > 30000
+
+
 def test_config1_compaction_is_stable_and_exact(rgl_lib):
     spec, lidar = S.config1_scene(), S.config1_lidar()
     case = Case(spec, lidar.rays, pose=lidar.pose, ranges=np.array([lidar.range], np.float32))

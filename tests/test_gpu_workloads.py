@@ -14,7 +14,8 @@ triangle under every node format.
 
 C5 (scenes.config5_workload: 32 clients, 16 scenes of 62k-9.3M triangles): a 64-request Poisson trace served with
 continuous batching gives every request exactly the results of the same graph run alone (lidars and radars), and
-the radar cells of three radar clients match the oracle's binning of its own hits (tests/golden/c5_radar.npz).
+the radar cells of three radar clients match the oracle's binning of its own hits (tests/golden/c5_radar.npz): the
+same cells except those a hit within FP32 reach of a cell edge may enter or leave, RCS within 1e-3 dB.
 """
 import ctypes as C
 import json
@@ -173,26 +174,26 @@ def test_c3_refit_equals_rebuild_over_moving_frames(rgl_lib):
     _, ents_b, _ = S.load_into_rgl(spec, scene=scene_rebuild, meshes=meshes)
     rgl.check("rgl_scene_set_tlas_refit", scene_refit, True)
     rgl.check("rgl_scene_set_tlas_refit", scene_rebuild, False)
-    graphs = {sc: [W.multireturn_graph(lidar, rgl.RGL_RETURN_FIRST_LAST, 0.13e-3, scene=sc),
-                   W.multireturn_graph(lidar, rgl.RGL_RETURN_STRONGEST, 0.13e-3, scene=sc)]
-              for sc in (scene_refit, scene_rebuild)}
+    copies = [(scene_refit, ents_a), (scene_rebuild, ents_b)]
+    graphs = [[W.multireturn_graph(lidar, rgl.RGL_RETURN_FIRST_LAST, 0.13e-3, scene=sc),
+               W.multireturn_graph(lidar, rgl.RGL_RETURN_STRONGEST, 0.13e-3, scene=sc)] for sc, _ in copies]
     base = [spec.instances[i].transform.astype(np.float64) for i in movable]
-    pos = {sc: np.array([b[:2, 3] for b in base]) for sc in graphs}
+    pos = [np.array([b[:2, 3] for b in base]) for _ in copies]
 
     def frame():
-        rgl.graph_run_batch([x.rays for gs in graphs.values() for x in gs])
-        return {sc: [{f: rgl.result(x.out, f) for f in W.MR_FIELDS} for x in gs] for sc, gs in graphs.items()}
+        rgl.graph_run_batch([x.rays for gs in graphs for x in gs])
+        return [[{f: rgl.result(x.out, f) for f in W.MR_FIELDS} for x in gs] for gs in graphs]
 
     frame()  # both TLASes built from scratch
     for k in range(3):
-        for sc, ents in ((scene_refit, ents_a), (scene_rebuild, ents_b)):
-            pick = move_c3(rgl, ents, base, movable, pos[sc], np.random.default_rng(0xC0F3 + k))
+        for c, (_, ents) in enumerate(copies):
+            pick = move_c3(rgl, ents, base, movable, pos[c], np.random.default_rng(0xC0F3 + k))
         res = frame()
-        for gi, (a, b) in enumerate(zip(res[scene_refit], res[scene_rebuild])):
+        for gi, (a, b) in enumerate(zip(res[0], res[1])):
             for f in W.MR_FIELDS:
                 assert a[f].tobytes() == b[f].tobytes(), (k, gi, f)
         say(f"C3 frame {k}: {len(pick)} instances moved; refitted TLAS == rebuilt TLAS "
-            f"({len(res[scene_refit][0][IS_HIT])} + {len(res[scene_refit][1][IS_HIT])} kept points)")
+            f"({len(res[0][0][IS_HIT])} + {len(res[0][1][IS_HIT])} kept points)")
 
 
 @pytest.mark.parametrize("fmt", [0, 1, 2])
@@ -290,7 +291,9 @@ def test_c5_served_equals_solo_and_radar_matches_oracle(rgl_lib):
         assert len(clean) > 0.95 * len(o_cells)
         for c, v in clean:
             assert c in g_map, (k, c)
-            assert abs(g_map[c] - v) <= 1e-4, (k, c, g_map[c], v)
+            # A cell sums d^2 * omega * cos(incidence) over its hits: distances within max(1 mm, 1e-5 d) and normals
+            # within 1e-4 of FP64 keep the sum within ~2e-4 relative, 1e-3 dB.
+            assert abs(g_map[c] - v) <= 1e-3, (k, c, g_map[c], v)
         assert set(g_map) - set(o_cells.tolist()) <= touched
         assert np.all(np.diff(g_cells) > 0)
         say(f"C5 radar client {k}: {len(g_map)} GPU cells, {len(o_cells)} oracle cells, {len(clean)} compared")
