@@ -346,7 +346,7 @@ bool better(const Candidate& a, const Candidate& b)
 }
 
 // Closest hit over (0, tmax]; optionally the ambiguity diagnostic at relative tolerance eps.
-Hit closest(const World& w, V3 o, V3 d, double tmax, bool brute, double eps, bool* ambiguous)
+Hit closest(const World& w, V3 o, V3 d, double tmax, bool brute, double eps, bool* ambiguous, bool* instanceTie)
 {
 	Hit best;
 	best.c.t = tmax;
@@ -391,7 +391,10 @@ Hit closest(const World& w, V3 o, V3 d, double tmax, bool brute, double eps, boo
 		pass(lim2, [&](const Candidate& c) {
 			if (c.margin < -eps || !(c.t > 0.0) || c.t > lim2) return;
 			if (c.margin < eps) amb = true; // grazing: within eps of an edge
-			if (best.found && c.inst != best.c.inst && c.margin >= 0.0 && c.t >= best.c.t * (1.0 - eps)) amb = true;
+			if (best.found && c.inst != best.c.inst && c.margin >= 0.0 && c.t >= best.c.t * (1.0 - eps)) {
+				amb = true;
+				if (instanceTie != nullptr) *instanceTie = true;
+			}
 			if (!best.found && c.margin >= 0.0) amb = true; // hit just beyond tmax
 		});
 		if (best.found && best.c.t >= tmax * (1.0 - eps)) amb = true;
@@ -616,7 +619,7 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	// operands are within FP32's reach of each other may go the other way on the GPU and change the selection.
 	// Distances carry ~eps relative error; intensities texel*cos(incidence) carry ~eps relative plus about
 	// 4e-6 of the texel value from the FP32 normal (dominant at grazing incidence).
-	bool distTie = false, intTie = false, texelEdge = false;
+	bool distTie = false, intTie = false, texelEdge = false, instanceTie = false;
 	auto nearD = [&](double a, double b) { return std::fabs(a - b) <= eps * std::max(std::fabs(a), std::fabs(b)); };
 	auto nearI = [&](double a, double b, double baseA, double baseB) {
 		return std::fabs(a - b) <= eps * std::max(std::fabs(a), std::fabs(b)) + 4e-6 * std::max(baseA, baseB);
@@ -636,7 +639,8 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		} else {
 			rayOf(sr, so, sd);
 		}
-		const Hit h = closest(w, so, sd, maxRange, q.bruteForce != 0, eps, out.ambiguous ? &ambiguous : nullptr);
+		const Hit h = closest(w, so, sd, maxRange, q.bruteForce != 0, eps, out.ambiguous ? &ambiguous : nullptr,
+		                      out.ambiguous ? &instanceTie : nullptr);
 		Track c;
 		c.sample = s;
 		bool isHit = false;
@@ -690,7 +694,8 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	}
 	if (out.ambiguous) {
 		out.ambiguous[b] =
-		    static_cast<int8_t>((ambiguous ? 1 : 0) | (distTie ? 2 : 0) | (intTie ? 4 : 0) | (texelEdge ? 8 : 0));
+		    static_cast<int8_t>((ambiguous ? 1 : 0) | (distTie ? 2 : 0) | (intTie ? 4 : 0) | (texelEdge ? 8 : 0) |
+		                        (instanceTie ? 16 : 0));
 	}
 	for (int r = 0; r < R; ++r) {
 		const int rp = b * R + r;

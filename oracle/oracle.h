@@ -93,7 +93,9 @@ typedef struct
 	int8_t* ambiguous;     /* bit 0: a sample grazes an edge, ties another instance or sits at a range boundary;
 	                          bit 1: a distance comparison of the beam reduction is a near tie;
 	                          bit 2: an intensity comparison of the beam reduction is a near tie;
-	                          bit 3: a sample's texture coordinate lies at a texel edge */
+	                          bit 3: a sample's texture coordinate lies at a texel edge;
+	                          bit 4: the closest hit ties a hit of another instance (set with bit 0), e.g. coplanar
+	                          faces of two entities */
 	double* primaryT;      /* FP64 ray parameter of the primary sample's closest hit, -1 if none */
 	int32_t* primaryInst;  /* instance of that hit, -1 if none */
 	int32_t* primaryPrim;  /* triangle of that hit, -1 if none */

@@ -299,10 +299,9 @@ class CpuPort:
         self.keep = _Keep()
         self.scenes = [lib.cpu_scene_create(C.byref(make_scene(sp, self.keep))) for sp in specs]
 
-    def run(self, sensors, threads: int):
+    def prepare(self, sensors) -> dict:
         """sensors: list of dicts(scene, rays, pose, range, to_sensor, look_at, look_at_inv, noise=(mean, base,
-        rise), seed, run, tag). Returns per sensor the kept (xyz, distance, entity, normal) and the uncompacted
-        arrays (xyz, distance, entity, normal, is_hit)."""
+        rise), seed, run, tag). Allocates the frame's descriptors and output arrays once (execute() reuses them)."""
         n = len(sensors)
         frames = (CpuFrame * n)()
         outs = (CpuOut * n)()
@@ -314,10 +313,9 @@ class CpuPort:
             f = frames[k]
             f.nRays = m
             f.rays = _p(rays, C.c_float)
-            for name in ("pose", "to_sensor", "look_at", "look_at_inv"):
+            for name, dst in (("pose", f.pose), ("to_sensor", f.toSensor), ("look_at", f.lookAt),
+                              ("look_at_inv", f.lookAtInv)):
                 arr = np.asarray(sn[name], np.float32).ravel()
-                dst = getattr(f, {"pose": "pose", "to_sensor": "toSensor", "look_at": "lookAt",
-                                  "look_at_inv": "lookAtInv"}[name])
                 for i in range(12):
                     dst[i] = float(arr[i])
             f.range[0], f.range[1] = sn["range"]
@@ -328,6 +326,8 @@ class CpuPort:
                  "full_is_hit": np.empty(m, np.int32), "xyz": np.empty((m, 3), np.float32),
                  "distance": np.empty(m, np.float32), "entity": np.empty(m, np.int32),
                  "normal": np.empty((m, 3), np.float32)}
+            for a in r.values():  # touch every page before any timing
+                a.fill(0)
             o = outs[k]
             for field, key, ct in (("fullXyz", "full_xyz", C.c_float), ("fullDistance", "full_distance", C.c_float),
                                    ("fullEntityId", "full_entity", C.c_int32), ("fullNormal", "full_normal", C.c_float),
@@ -337,13 +337,27 @@ class CpuPort:
                 setattr(o, field, _p(r[key], ct))
             res.append(r)
         handles = (C.c_void_p * n)(*[self.scenes[sn["scene"]] for sn in sensors])
-        load().cpu_run(handles, frames, outs, n, threads)
-        for k, r in enumerate(res):
-            kept = outs[k].kept
-            r["kept"] = kept
+        return {"frames": frames, "outs": outs, "handles": handles, "keep": keep, "arrays": res, "count": n}
+
+    def execute(self, frame: dict, threads: int) -> None:
+        load().cpu_run(frame["handles"], frame["frames"], frame["outs"], frame["count"], threads)
+
+    def results(self, frame: dict) -> list:
+        """Per sensor: the kept (xyz, distance, entity, normal), `kept`, and the uncompacted arrays."""
+        out = []
+        for k, r in enumerate(frame["arrays"]):
+            kept = frame["outs"][k].kept
+            d = dict(r)
+            d["kept"] = kept
             for key in ("xyz", "distance", "entity", "normal"):
-                r[key] = r[key][:kept]
-        return res
+                d[key] = r[key][:kept]
+            out.append(d)
+        return out
+
+    def run(self, sensors, threads: int) -> list:
+        frame = self.prepare(sensors)
+        self.execute(frame, threads)
+        return self.results(frame)
 
     def close(self):
         for h in self.scenes:

@@ -229,7 +229,12 @@ __device__ void touchLines(const LineSet* s, const void* p, int bytes)
 		uint32_t h = lineHash(key) & s->mask;
 		bool placed = false;
 		for (int probe = 0; probe < kLineProbes && !placed; ++probe) {
-			const unsigned long long prev = atomicCAS(&s->keys[h], 0ull, key);
+			// Slots only change from empty to a key, so a plain read that sees the key is final (hot lines such as
+			// the roots are then not contended by atomics).
+			unsigned long long prev = __atomic_load_n(&s->keys[h], __ATOMIC_RELAXED);
+			if (prev == 0ull) {
+				prev = atomicCAS(&s->keys[h], 0ull, key);
+			}
 			placed = prev == 0ull || prev == key;
 			h = (h + 1) & s->mask;
 		}

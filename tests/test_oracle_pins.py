@@ -84,7 +84,7 @@ def test_ambiguity_flags_edges_and_ties(oracle):
     # Two coincident cubes with different ids tie at equal distance; the lower instance index wins.
     spec2 = cube_scene(S.translation(0, 0, 5), S.translation(0, 0, 5), ids=[7, 8])
     r2 = run_oracle(oracle, Case(spec2, S.f32(S.translation(0.5, 0.25, 0)[None])), fields=[ENT])
-    assert r2["ambiguous"][0] == 1 and r2[ENT][0] == 7
+    assert r2["ambiguous"][0] == 1 | 16 and r2[ENT][0] == 7  # geometric ambiguity, of the instance-tie kind
 
 
 def test_vertex_displacement_velocity_closed_form(oracle):
