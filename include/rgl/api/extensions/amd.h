@@ -160,3 +160,10 @@ RGL_API rgl_status_t rgl_server_reset_stats(rgl_server_t server);
 
 // Standard normal variates normal(seed, run, tag, i) for i in [0, count), computed on the GPU.
 RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);
+
+// rgl_noise_normals without any device allocation. No counterpart in the reference: it is for callers that cannot
+// allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+// variates are staged in a fixed per-device array of 2^20 floats (4 MiB on every device the library runs on, held
+// until the process ends), so count must not exceed 1048576 (RGL_INVALID_ARGUMENT otherwise). Synchronous; calls that
+// overlap on one GPU share the staging array and race.
+RGL_API rgl_status_t rgl_noise_normals_no_alloc(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count);

@@ -1587,6 +1587,15 @@ RGL_API rgl_status_t rgl_noise_normals(uint64_t seed, uint32_t run, uint32_t tag
 	});
 }
 
+RGL_API rgl_status_t rgl_noise_normals_no_alloc(uint64_t seed, uint32_t run, uint32_t tag, float* out, int32_t count)
+{
+	return safeCall([&]() {
+		CHECK_ARG(out != nullptr);
+		CHECK_ARG(count > 0);
+		gpu::noiseNormalsNoAlloc(count, out, seed, run, tag);
+	});
+}
+
 // ============================================================================================ tape
 // src/api/apiCore.cpp:1456-1530
 RGL_API rgl_status_t rgl_tape_record_begin(const char* path)

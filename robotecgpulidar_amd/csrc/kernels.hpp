@@ -91,6 +91,10 @@ void copyFields(hipStream_t stream, int capacity, const int32_t* count, const Fi
 // dst[i] = *src[i] for i < n (`src` lives in device memory).
 void gatherInts(hipStream_t stream, int n, const int32_t* const* src, int32_t* dst);
 void noiseNormals(hipStream_t stream, int n, float* out, uint64_t seed, uint32_t run, uint32_t tag);
+// The same normals into host memory through a fixed per-device staging array (no device allocation); n <=
+// kNoAllocNormalsMax. Synchronous; calls that overlap on one device share the array.
+constexpr int kNoAllocNormalsMax = 1 << 20;
+void noiseNormalsNoAlloc(int n, float* hostOut, uint64_t seed, uint32_t run, uint32_t tag);
 // SoA -> AoS (format) and AoS -> SoA (from-array) over `capacity` records of `recordSize` bytes.
 void formatSoaToAos(hipStream_t stream, int capacity, const int32_t* count, int recordSize, const FieldCopyList& fields,
                     char* out);

@@ -12,12 +12,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 #include "device_types.hpp"
 #include "kernels.hpp"
 #include "noise_rng.hpp"
 #include "point_ops.hpp"
+#include "runtime.hpp"
 
 namespace rgl::gpu {
 
@@ -714,6 +717,53 @@ void radarBin(hipStream_t stream, int capacity, const int32_t* count, const Vec3
 	}
 	hipLaunchKernelGGL(kRadarCells, dim3(blocks(cells)), dim3(kBlock), 0, stream, bp, sums, counts, cellXyz, cellDistance,
 	                   cellAzimuth, cellRcs, cellHit, keep);
+}
+
+// ------------------------------------------------------------------------------------------------ no-alloc normals
+// Staging array of noiseNormalsNoAlloc: a file-scope device array, one copy per device for the life of the process.
+// Host code never passes its name to a kernel (on the host it is a stand-in, not a device address): the device
+// address is looked up once per device with hipGetSymbolAddress and kept in scratchPtr.
+__device__ __align__(256) float scratch[kNoAllocNormalsMax];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratchMutex;
+float* scratchPtr[kMaxDevices] = {};
+
+float* scratchOfCurrentDevice()
+{
+	int dev = 0;
+	RGL_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices) {
+		throw std::runtime_error("device index beyond the no-alloc staging table");
+	}
+	std::lock_guard<std::mutex> lock(scratchMutex);
+	if (scratchPtr[dev] == nullptr) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || p == nullptr) {
+			throw std::runtime_error(std::string("hipGetSymbolAddress of the no-alloc staging array failed: ") +
+			                         hipGetErrorString(e));
+		}
+		scratchPtr[dev] = static_cast<float*>(p);
+	}
+	return scratchPtr[dev];
+}
+} // namespace
+
+void noiseNormalsNoAlloc(int n, float* hostOut, uint64_t seed, uint32_t run, uint32_t tag)
+{
+	if (n < 0 || n > kNoAllocNormalsMax) {
+		throw std::invalid_argument("RGL API Error: Invalid argument, condition unsatisfied: count <= " +
+		                            std::to_string(kNoAllocNormalsMax) + " (the no-alloc staging array)");
+	}
+	if (n == 0) {
+		return;
+	}
+	float* staging = scratchOfCurrentDevice();
+	hipLaunchKernelGGL(kNoiseNormals, dim3(blocks(n)), dim3(kBlock), 0, nullptr, n, staging, seed, run, tag);
+	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipMemcpy(hostOut, staging, sizeof(float) * n, hipMemcpyDeviceToHost)); // null stream: after the kernel
 }
 
 } // namespace rgl::gpu

@@ -180,6 +180,7 @@ _PROTOTYPES = [
     ("rgl_node_raytrace_brute_force", [c_node, f32p, i32p, i32p, C.c_int32]),
     ("rgl_graph_traversal_footprint", [P(c_node), C.c_int32, P(C.c_int64)]),
     ("rgl_noise_normals", [C.c_uint64, C.c_uint32, C.c_uint32, f32p, C.c_int32]),
+    ("rgl_noise_normals_no_alloc", [C.c_uint64, C.c_uint32, C.c_uint32, f32p, C.c_int32]),
     ("rgl_node_raytrace_set_kernel_timing", [c_node, C.c_bool]),
     ("rgl_node_raytrace_get_kernel_timing", [c_node, P(C.c_double), i32p]),
     ("rgl_scene_create", [P(C.c_void_p)]),
@@ -599,4 +600,11 @@ def cleanup() -> None:
 def noise_normals(seed: int, run: int, tag: int, count: int) -> np.ndarray:
     out = np.empty(count, dtype=np.float32)
     check("rgl_noise_normals", C.c_uint64(seed), run, tag, out.ctypes.data_as(f32p), count)
+    return out
+
+
+def noise_normals_no_alloc(seed: int, run: int, tag: int, count: int) -> np.ndarray:
+    """rgl_noise_normals_no_alloc: the same variates staged without device allocation (count <= 2^20)."""
+    out = np.empty(count, dtype=np.float32)
+    check("rgl_noise_normals_no_alloc", C.c_uint64(seed), run, tag, out.ctypes.data_as(f32p), count)
     return out
