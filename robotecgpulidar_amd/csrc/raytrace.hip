@@ -379,32 +379,9 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 	bool inBlas = false;
 	bool found = false;
 	float tBound = slabBound(best.t);
-	int node = 0;
-	while (true) {
-		if (node < 0) {
-			// An instance leaf of the TLAS: continue at the root of its BLAS with the object-space ray.
-			inst = ~node;
-			instBase = sp;
-			const InstanceRec& ir = sc.instances[inst];
-			if constexpr (kFoot) {
-				touchLines(ls, &ir, sizeof(InstanceRec));
-			}
-			obj = objectRay(ir, loadOrigin<kBlock>(), d);
-			ro = obj.o;
-			ri = obj.inv;
-			nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
-			tris = ir.blasTris;
-			inBlas = true;
-			if (kStats) {
-				st->insts++;
-			}
-			node = 0;
-		} else if (inBlas && inst < 0) {
-			ro = o;
-			ri = worldInv;
-			nodes = tlas;
-			inBlas = false;
-		}
+	// One node step at the current level: slab-test the children of `node`, push the far hit ones (far to near) and
+	// return the nearest hit child, or kPop.
+	auto stepNode = [&](int node) -> int {
 		if (kStats) {
 			st->nodes++;
 		}
@@ -514,6 +491,72 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			}
 			next = k0 != kMiss ? r0 : kPop;
 		}
+		return next;
+	};
+	// Triangle leaf test of the entered instance; keeps the closest hit (ties: lower instance, then triangle).
+	auto testLeaf = [&](int leaf) {
+		if (kStats) {
+			st->tris++;
+		}
+		if constexpr (kFoot) {
+			touchLines(ls, tris + leaf, sizeof(LeafTri));
+		}
+		const LeafTri lt = loadLeafTri(tris, leaf);
+		float t, vNum, wNum, det;
+		if (intersectTri(obj, lt, t, vNum, wNum, det) && t > 0.0f && t <= tmax && better(t, inst, lt.prim, best)) {
+			best.t = t;
+			best.inst = inst;
+			best.prim = lt.prim;
+			best.leaf = leaf;
+			best.u = vNum / det;
+			best.v = wNum / det;
+			tBound = slabBound(t);
+			found = true;
+		}
+	};
+	// Entering the instance of TLAS leaf `leafRef`: its BLAS with the object-space ray (FP64 origin mapped once).
+	auto enterInstance = [&](int leafRef) {
+		inst = ~leafRef;
+		instBase = sp;
+		const InstanceRec& ir = sc.instances[inst];
+		if constexpr (kFoot) {
+			touchLines(ls, &ir, sizeof(InstanceRec));
+		}
+		obj = objectRay(ir, loadOrigin<kBlock>(), d);
+		ro = obj.o;
+		ri = obj.inv;
+		nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
+		tris = ir.blasTris;
+		inBlas = true;
+		if (kStats) {
+			st->insts++;
+		}
+	};
+	auto returnToTlas = [&]() {
+		ro = o;
+		ri = worldInv;
+		nodes = tlas;
+		inBlas = false;
+	};
+	auto popEntry = [&]() -> int {
+		if (__all(sp <= kLdsStack)) {
+			--sp;
+			return ldsStack[sp * kBlock + tid];
+		}
+		return RGL_POP();
+	};
+	// Closest child first; a triangle leaf is tested as soon as it is reached. (Postponing leaves until most lanes of
+	// the wave hold one, with or without speculative node steps meanwhile, measured 20-30% slower on the 64-lidar
+	// batch: the extra loop control costs more than the fuller triangle tests save.)
+	int node = 0;
+	while (true) {
+		if (node < 0) {
+			enterInstance(node);
+			node = 0;
+		} else if (inBlas && inst < 0) {
+			returnToTlas();
+		}
+		int next = stepNode(node);
 		// Pops and triangle leaves until the next inner node or instance leaf (one exit, so the loop-carried
 		// values need no copies at divergent exits).
 		bool done = false;
@@ -526,36 +569,12 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				}
 				done = sp == 0 || overflow;
 				if (!done) {
-					if (__all(sp <= kLdsStack)) {
-						--sp;
-						next = ldsStack[sp * kBlock + tid];
-					} else {
-						next = RGL_POP();
-					}
+					next = popEntry();
 				}
 			}
 			tri = !done && next < 0 && inst >= 0;
 			if (tri) {
-				const int leaf = ~next;
-				if (kStats) {
-					st->tris++;
-				}
-				if constexpr (kFoot) {
-					touchLines(ls, tris + leaf, sizeof(LeafTri));
-				}
-				const LeafTri lt = loadLeafTri(tris, leaf);
-				float t, vNum, wNum, det;
-				if (intersectTri(obj, lt, t, vNum, wNum, det) && t > 0.0f && t <= tmax &&
-				    better(t, inst, lt.prim, best)) {
-					best.t = t;
-					best.inst = inst;
-					best.prim = lt.prim;
-					best.leaf = leaf;
-					best.u = vNum / det;
-					best.v = wNum / det;
-					tBound = slabBound(t);
-					found = true;
-				}
+				testLeaf(~next);
 				next = kPop;
 			}
 		} while (tri);
