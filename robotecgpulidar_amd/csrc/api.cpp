@@ -1198,7 +1198,7 @@ RGL_API rgl_status_t rgl_graph_get_result_size(rgl_node_t node, rgl_field_t fiel
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(node != nullptr);
 		auto pn = nodeAs<IPointsNode>(node, "IPointsNode");
-		pn->waitForResults();
+		pn->waitForCounts();
 		checkOverflow(pn);
 		const int32_t count = pn->hostCount();
 		const int32_t size = static_cast<int32_t>(pn->fieldPointSize(field));

@@ -61,6 +61,14 @@ struct GraphBatch::Plan
 		}
 	};
 	std::vector<Level> levels; // steps, in launch order
+	// Prefetch copies of yield nodes into pinned host memory: nothing in the batch reads them, so they run after every
+	// other launch, in chunks of consecutive items, each chunk followed by an event that its nodes' results wait on.
+	std::vector<PointOpItem> toHost;
+	std::vector<int> toHostLayout;
+	std::vector<Node*> toHostNodes;
+	std::vector<int> chunkFirst; // first item of each chunk, then the item count
+	std::vector<int> chunkBlocks;
+	size_t toHostOff = 0;
 	std::vector<Node*> readyOp, readyRt, readyCp, readyRp, passive;
 	std::vector<FieldCopyList> lists;
 	std::vector<const int32_t*> gather;
@@ -140,6 +148,12 @@ void GraphBatch::fusePointStages(RaytraceNode* rt, RaytraceBatchItem& it)
 
 namespace {
 constexpr double kTuneMinUnits = 100000.0;
+// Capacity bytes per prefetch launch: about a third of a millisecond of the host link, so the host starts copying the
+// first sensors' points while the last ones are still on the way.
+#ifndef RGL_PREFETCH_CHUNK_MIB
+#define RGL_PREFETCH_CHUNK_MIB 16
+#endif
+constexpr size_t kPrefetchChunkBytes = static_cast<size_t>(RGL_PREFETCH_CHUNK_MIB) << 20;
 constexpr int kTuneSamples = 3;
 } // namespace
 
@@ -366,9 +380,16 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	for (auto& c : pl.ctxs) {
 		c->beginBatch(s, *scratchEvent);
 	}
+	std::shared_ptr<Event> countsReady;
+	std::vector<std::shared_ptr<Event>> chunkReady;
 	try {
 		pl.lists.clear();
 		pl.gather.clear();
+		pl.toHost.clear();
+		pl.toHostLayout.clear();
+		pl.toHostNodes.clear();
+		pl.chunkFirst.clear();
+		pl.chunkBlocks.clear();
 		pl.scenes.clear();
 		pl.readyOp.clear();
 		pl.readyRt.clear();
@@ -464,10 +485,17 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						PointOpItem it{};
 						FieldCopyList layout;
 						n->pointOp(it, layout);
+						if (it.op == kOpToHost) {
+							pl.toHostLayout.push_back(static_cast<int>(pl.lists.size()));
+							pl.lists.push_back(layout);
+							pl.toHost.push_back(it);
+							pl.toHostNodes.push_back(n);
+							break;
+						}
 						it.firstBlock = L.opBlocks;
 						L.opBlocks += gpu::pointOpBlocks(it.capacity);
 						L.op.push_back(it);
-						if (it.op == kOpFormat || it.op == kOpToHost) {
+						if (it.op == kOpFormat) {
 							L.opLayout.push_back(static_cast<int>(pl.lists.size()));
 							pl.lists.push_back(layout);
 						} else {
@@ -513,6 +541,25 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		}
 		const int maxLevel = steps - 1;
 
+		// Prefetch chunks of about kPrefetchChunkBytes of capacity each, in item order (the order of the graphs):
+		// a chunk's host copies can start while the link still carries the next chunks.
+		size_t chunkBytes = 0;
+		for (size_t i = 0; i < pl.toHost.size(); ++i) {
+			PointOpItem& it = pl.toHost[i];
+			if (pl.chunkFirst.empty() || chunkBytes >= kPrefetchChunkBytes) {
+				pl.chunkFirst.push_back(static_cast<int>(i));
+				pl.chunkBlocks.push_back(0);
+				chunkBytes = 0;
+			}
+			const FieldCopyList& layout = pl.lists[pl.toHostLayout[i]];
+			for (int f = 0; f < layout.count; ++f) {
+				chunkBytes += static_cast<size_t>(std::max(it.capacity, 0)) * static_cast<size_t>(layout.f[f].size);
+			}
+			it.firstBlock = pl.chunkBlocks.back();
+			pl.chunkBlocks.back() += gpu::pointOpBlocks(it.capacity);
+		}
+		pl.chunkFirst.push_back(static_cast<int>(pl.toHost.size()));
+
 		// Host slots: kept counts of compaction nodes, cluster counts and error bits of radar postprocess nodes, then
 		// overflow flags of raytrace nodes.
 		for (int l = 0; l <= maxLevel; ++l) {
@@ -552,6 +599,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			L.cpOff = reserve(sizeof(CompactItem) * L.cp.size());
 			L.rpOff = reserve(sizeof(RadarItem) * L.rp.size());
 		}
+		pl.toHostOff = reserve(sizeof(PointOpItem) * pl.toHost.size());
 		const size_t total = std::max<size_t>(off, 1);
 		if (total > items.capacity()) {
 			synchronize(); // earlier batches may still read the descriptors being reallocated
@@ -593,6 +641,12 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			if (!L.rp.empty()) {
 				std::memcpy(host + L.rpOff, L.rp.data(), sizeof(RadarItem) * L.rp.size());
 			}
+		}
+		for (size_t i = 0; i < pl.toHost.size(); ++i) {
+			pl.toHost[i].format = devLists + pl.toHostLayout[i];
+		}
+		if (!pl.toHost.empty()) {
+			std::memcpy(host + pl.toHostOff, pl.toHost.data(), sizeof(PointOpItem) * pl.toHost.size());
 		}
 		// Kept counts and overflow flags of every graph come back through one gather and one copy.
 		const int g = static_cast<int>(pl.gather.size());
@@ -806,6 +860,18 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 				}
 			}
 		}
+		if (!pl.toHost.empty()) {
+			countsReady = completionEvent();
+			countsReady->record(s);
+			const auto* toHostItems = reinterpret_cast<const PointOpItem*>(dev + pl.toHostOff);
+			for (size_t c = 0; c + 1 < pl.chunkFirst.size(); ++c) {
+				const int first = pl.chunkFirst[c];
+				gpu::pointOpsBatch(s, toHostItems + first, pl.chunkFirst[c + 1] - first, pl.chunkBlocks[c]);
+				RGL_HIP(hipGetLastError());
+				chunkReady.push_back(completionEvent());
+				chunkReady.back()->record(s);
+			}
+		}
 	} catch (...) {
 		auto ev = completionEvent();
 		ev->record(s);
@@ -825,9 +891,14 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 	}
 	for (auto& c : pl.ctxs) {
 		for (auto& n : c->validatedOrder()) {
-			n->markEnqueued(ev);
+			n->markEnqueued(ev, countsReady, countsReady); // all but the prefetched copies are complete before those run
 		}
 		c->endBatch(s, ev);
+	}
+	for (size_t c = 0; c < chunkReady.size(); ++c) {
+		for (int i = pl.chunkFirst[c]; i < pl.chunkFirst[c + 1]; ++i) {
+			pl.toHostNodes[i]->markEnqueued(ev, chunkReady[c], countsReady);
+		}
 	}
 	lastCompletion = ev;
 }

@@ -193,6 +193,8 @@ void Node::enqueue()
 		done = std::make_shared<Event>();
 	}
 	done->record(stream());
+	ready = nullptr;
+	counted = nullptr;
 	ranOnce = true;
 }
 
@@ -201,9 +203,12 @@ void Node::pointOp(PointOpItem&, FieldCopyList&)
 	throw std::logic_error(getName() + " has no elementwise stage");
 }
 
-void Node::markEnqueued(const std::shared_ptr<Event>& ev)
+void Node::markEnqueued(const std::shared_ptr<Event>& ev, const std::shared_ptr<Event>& readyAt,
+                        const std::shared_ptr<Event>& countedAt)
 {
 	done = ev;
+	ready = readyAt;
+	counted = countedAt;
 	ranOnce = true;
 }
 
@@ -219,9 +224,23 @@ void Node::waitForResults()
 	if (!isValid() || !ranOnce) {
 		throw InvalidPipeline("Cannot get results from " + getName() + "; it hasn't been run yet, or the run has failed");
 	}
-	if (done) {
+	if (ready) {
+		ready->sync();
+	} else if (done) {
 		done->sync();
 	}
+}
+
+void Node::waitForCounts()
+{
+	if (counted) {
+		if (!isValid() || !ranOnce) {
+			throw InvalidPipeline("Cannot get results from " + getName() + "; it hasn't been run yet, or the run has failed");
+		}
+		counted->sync();
+		return;
+	}
+	waitForResults();
 }
 
 // ------------------------------------------------------------------------------------------------ GraphRunCtx

@@ -57,11 +57,18 @@ struct Node : std::enable_shared_from_this<Node>
 
 	void validate();
 	void enqueue();
-	// Marks the node's work as enqueued by a batch launch whose completion `ev` records.
-	void markEnqueued(const std::shared_ptr<Event>& ev);
-	// Waits until this node's device work of the last run is complete; throws if the node never ran.
+	// Marks the node's work as enqueued by a batch: `ev` records the completion of the whole batch (every launch that
+	// may still read or write the node's buffers), `ready` (recorded no later than `ev`, or null for `ev`) the point
+	// from which the node's results are complete, `counted` (no later than `ready`, or null for `ready`) the point
+	// from which the host copies of the batch's point counts and overflow flags are.
+	void markEnqueued(const std::shared_ptr<Event>& ev, const std::shared_ptr<Event>& ready = nullptr,
+	                  const std::shared_ptr<Event>& counted = nullptr);
+	// Waits until this node's results of the last run are complete; throws if the node never ran.
 	void waitForResults();
-	// Waits for the last run's device work if there was one.
+	// Waits until the host copies of the last run's point counts and overflow flags are complete (the results
+	// themselves may still be in flight); throws if the node never ran.
+	void waitForCounts();
+	// Waits for all of the last run's device work if there was one (before the node's buffers are replaced).
 	void waitForResultsIfRun();
 	void markDirty() { dirty = true; }
 	bool isValid() const { return !dirty; }
@@ -87,7 +94,9 @@ protected:
 	bool dirty = true;
 	bool ranOnce = false;
 	std::shared_ptr<GraphRunCtx> ctx;
-	std::shared_ptr<Event> done; // shared by every node of a batch
+	std::shared_ptr<Event> done;  // shared by every node of a batch
+	std::shared_ptr<Event> ready;   // results complete (batched runs; null = `done`)
+	std::shared_ptr<Event> counted; // counts and flags on the host (batched runs; null = `ready`)
 	int32_t batchLevel = 0;      // longest path from an entry node (batched runs)
 	bool batchFused = false;     // computed inside its raytrace node's batched launch this batch (no launch of its own)
 	friend struct GraphRunCtx;

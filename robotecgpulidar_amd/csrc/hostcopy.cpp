@@ -17,6 +17,10 @@ namespace rgl {
 
 namespace {
 
+#ifndef RGL_COPY_WORKERS
+#define RGL_COPY_WORKERS 4 // threads beside the caller's (at most a quarter of the host's)
+#endif
+
 constexpr size_t kParallelMin = 256 << 10; // below this one memcpy is faster than waking workers
 constexpr size_t kChunk = 64 << 10;        // unit of work a thread takes at a time
 constexpr size_t kStageChunk = 4 << 20;    // pinned staging chunk of device reads
@@ -73,7 +77,7 @@ private:
 	CopyPool()
 	{
 		const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-		const unsigned n = std::min(4u, hw / 4);
+		const unsigned n = std::min<unsigned>(RGL_COPY_WORKERS, hw / 4);
 		// Workers start from the generation before any job: a job published before a worker first runs is still
 		// seen as new by it (the caller waits for every worker).
 		owner = getpid();

@@ -5,7 +5,9 @@ graph (rays -> pose -> range -> raytrace -> points transform -> distance noise -
 transform and the noise are folded into the raytrace launch): every node's results of every sensor are identical
 between one rgl_graph_run_batch and per-sensor rgl_graph_run, over two frames with entities moved in between, and
 the raytrace node's fields at the seeded beams of tests/golden/c4_sample.npz match the FP64 oracle to the parity
-bar (tests.helpers.check_golden), also under the binary and the 8-bit quantized node formats.
+bar (tests.helpers.check_golden), also under the binary and the 8-bit quantized node formats. With every yield
+node prefetching into pinned host memory (chunked copies after the batch), results read back in any sensor order
+equal those read from device memory.
 
 C3 (scenes.config3_scene: ~10M triangles, 2,001 instances, 58-sample beams): over three frames in which 20% of the
 instances move by U[0, 0.5] m, FIRST_LAST and STRONGEST results with a refitted TLAS equal those with a rebuilt
@@ -148,6 +150,37 @@ def test_c4_golden_beams_other_node_formats(rgl_lib, fmt):
             full[f][g["beams"][a:b]] = res[f]
         worst = max(worst, check_c4_sensor(full, g, off, hit, k, f"format {fmt} sensor {k}")["max_dist_err_m"])
     say(f"C4 golden beams, format {fmt}: 64 sensors match, max |d - d_FP64| {worst:.2e} m")
+
+
+def test_c4_prefetch_chunks_equal_device_readback(rgl_lib):
+    """The 64-lidar frame with every yield node prefetching: the prefetch copies run after the rest of the batch in
+    chunks of consecutive sensors, each chunk's results complete at its own event. Read back last sensor first, then
+    the rest interleaved from both ends, they equal the results a second set of the same graphs (same noise seeds)
+    gives from device memory without prefetch, over two frames."""
+    rgl = rgl_lib
+    specs, sensors = S.config4_workload()
+    scenes = [sc for sc, _ in W.load_scenes(specs)]
+    sets = [[W.sensor_graph(x.lidar, scenes[x.scene], 0xC0F40000 + k) for k, x in enumerate(sensors)]
+            for _ in range(2)]
+    for x in sets[1]:
+        rgl.check("rgl_node_points_yield_configure_prefetch", x.out.h, True)
+    n = len(sensors)
+    order = [n - 1] + [k for pair in zip(range(0, n // 2), range(n - 2, n // 2 - 2, -1)) for k in pair]
+    order = list(dict.fromkeys(order + list(range(n))))
+    assert sorted(order) == list(range(n))
+    for frame in range(2):
+        rgl.graph_run_batch([x.rays for x in sets[0]])
+        want = [{f: rgl.result(x.out, f) for f in W.FIELDS} for x in sets[0]]
+        rgl.graph_run_batch([x.rays for x in sets[1]])
+        sizes = [W.kept_count(x.out) for x in sets[1]]  # counts are on the host before the prefetch copies end
+        moved = 0
+        for k in order:
+            got = {f: rgl.result(sets[1][k].out, f) for f in W.FIELDS}
+            assert len(got[W.FIELDS[0]]) == sizes[k] == len(want[k][W.FIELDS[0]]) > 0, k
+            for f in W.FIELDS:
+                assert got[f].tobytes() == want[k][f].tobytes(), (frame, k, f)
+                moved += got[f].nbytes
+        say(f"C4 frame {frame}: prefetched results of 64 sensors equal device read-back ({moved / 1e6:.1f} MB)")
 
 
 def move_c3(rgl, ents, base, movable, pos, rng, fraction=0.2):
