@@ -242,9 +242,9 @@ RGL_API rgl_status_t rgl_mesh_create(rgl_mesh_t* out_mesh, const rgl_vec3f* vert
                                      const rgl_vec3i* indices, int32_t index_count);            // core.h:573
 RGL_API rgl_status_t rgl_mesh_set_texture_coords(rgl_mesh_t mesh, const rgl_vec2f* uvs, int32_t uv_count); // core.h:583
 RGL_API rgl_status_t rgl_mesh_set_bone_weights(rgl_mesh_t mesh, const rgl_bone_weights_t* bone_weights,
-                                               int32_t bone_weights_count);                    // core.h:593 (not implemented)
+                                               int32_t bone_weights_count);                    // core.h:593
 RGL_API rgl_status_t rgl_mesh_set_restposes(rgl_mesh_t mesh, const rgl_mat3x4f* restposes,
-                                            int32_t bones_count);                              // core.h:605 (not implemented)
+                                            int32_t bones_count);                              // core.h:605
 RGL_API rgl_status_t rgl_mesh_destroy(rgl_mesh_t mesh);                                         // core.h:612
 RGL_API rgl_status_t rgl_mesh_is_alive(rgl_mesh_t mesh, bool* out_alive);                       // core.h:620
 
@@ -253,7 +253,7 @@ RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t sce
 RGL_API rgl_status_t rgl_entity_destroy(rgl_entity_t entity);                                         // core.h:638
 RGL_API rgl_status_t rgl_entity_set_transform(rgl_entity_t entity, const rgl_mat3x4f* transform);     // core.h:646
 RGL_API rgl_status_t rgl_entity_set_pose_world(rgl_entity_t entity, const rgl_mat3x4f* pose,
-                                               int32_t bones_count);                   // core.h:659 (not implemented)
+                                               int32_t bones_count);                   // core.h:659
 RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id);                              // core.h:666
 RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_texture_t texture);    // core.h:673
 RGL_API rgl_status_t rgl_entity_set_laser_retro(rgl_entity_t entity, float retro);                    // core.h:682

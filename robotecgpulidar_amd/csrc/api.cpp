@@ -436,7 +436,6 @@ RGL_API rgl_status_t rgl_mesh_create(rgl_mesh_t* out_mesh, const rgl_vec3f* vert
 		CHECK_ARG(vertex_count > 0);
 		CHECK_ARG(indices != nullptr);
 		CHECK_ARG(index_count > 0);
-		GraphRunCtx::synchronizeAll();
 		auto m = std::make_shared<rgl::Mesh>(reinterpret_cast<const Vec3f*>(vertices), vertex_count,
 		                                reinterpret_cast<const Vec3i*>(indices), index_count);
 		Registry<rgl::Mesh>::add(m);
@@ -508,12 +507,16 @@ RGL_API rgl_status_t rgl_mesh_is_alive(rgl_mesh_t mesh, bool* out_alive)
 }
 
 // ============================================================================================ entity
+// Calls that change only host state of a scene (entity transforms, ids, laser retro, textures, scene time, new
+// entities) return without waiting for running graphs, where src/api/apiCore.cpp:333-425 drains every graph: the
+// scene's next rebuild runs on the scene stream after the raytrace launches that still read its device arrays
+// (Scene::rebuild). Calls that rewrite device data those launches read (animation, texture coordinates) or free it
+// still drain every graph first.
 RGL_API rgl_status_t rgl_entity_create(rgl_entity_t* out_entity, rgl_scene_t scene, rgl_mesh_t mesh)
 {
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(out_entity != nullptr);
 		CHECK_ARG(mesh != nullptr);
-		GraphRunCtx::synchronizeAll();
 		auto sc = sceneOf(scene);
 		auto e = std::make_shared<rgl::Entity>(Registry<rgl::Mesh>::get(mesh, "Mesh"), sc.get());
 		Registry<rgl::Entity>::add(e);
@@ -542,7 +545,6 @@ RGL_API rgl_status_t rgl_entity_set_transform(rgl_entity_t entity, const rgl_mat
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(transform != nullptr);
-		GraphRunCtx::synchronizeAll();
 		Registry<rgl::Entity>::get(entity, "Entity")->setTransform(toMat(transform));
 	});
 	TAPE_RECORD(rgl_entity_set_transform, A(entity), R(transform, 1));
@@ -567,7 +569,6 @@ RGL_API rgl_status_t rgl_entity_set_id(rgl_entity_t entity, int32_t id)
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(id != RGL_ENTITY_INVALID_ID);
-		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		constexpr int maxId = 1 << 28; // entity ids are 28-bit (src/scene/Entity.cpp:45-48)
 		if (id >= maxId) {
@@ -585,7 +586,6 @@ RGL_API rgl_status_t rgl_entity_set_intensity_texture(rgl_entity_t entity, rgl_t
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
 		CHECK_ARG(texture != nullptr);
-		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		e->texture = Registry<rgl::Texture>::get(texture, "Texture");
 		e->scene->markDirty(false);
@@ -598,7 +598,6 @@ RGL_API rgl_status_t rgl_entity_set_laser_retro(rgl_entity_t entity, float retro
 {
 	rgl_status_t status = safeCall([&]() {
 		CHECK_ARG(entity != nullptr);
-		GraphRunCtx::synchronizeAll();
 		auto e = Registry<rgl::Entity>::get(entity, "Entity");
 		e->laserRetro = retro;
 		e->scene->markDirty(false);
@@ -638,7 +637,6 @@ RGL_API rgl_status_t rgl_texture_create(rgl_texture_t* out_texture, const void* 
 		CHECK_ARG(texels != nullptr);
 		CHECK_ARG(width > 0);
 		CHECK_ARG(height > 0);
-		GraphRunCtx::synchronizeAll();
 		auto t = std::make_shared<rgl::Texture>(texels, width, height);
 		Registry<rgl::Texture>::add(t);
 		*out_texture = reinterpret_cast<rgl_texture_t>(t.get());
@@ -670,7 +668,6 @@ RGL_API rgl_status_t rgl_texture_is_alive(rgl_texture_t texture, bool* out_alive
 RGL_API rgl_status_t rgl_scene_set_time(rgl_scene_t scene, uint64_t nanoseconds)
 {
 	rgl_status_t status = safeCall([&]() {
-		GraphRunCtx::synchronizeAll();
 		auto sc = sceneOf(scene);
 		sc->setTime(nanoseconds);
 		sc->markDirty(false);
@@ -1549,6 +1546,7 @@ RGL_API rgl_status_t rgl_node_raytrace_brute_force(rgl_node_t node, float* out_t
 		const int rays = ray_count;
 		DeviceBuffer a, b, c;
 		n->bruteForce(a.ensureT<float>(rays), b.ensureT<int32_t>(rays), c.ensureT<int32_t>(rays));
+		RGL_HIP(hipStreamSynchronize(n->stream())); // the node's stream does not synchronize with the null stream
 		RGL_HIP(hipMemcpy(out_t, a.get(), sizeof(float) * rays, hipMemcpyDeviceToHost));
 		RGL_HIP(hipMemcpy(out_instance, b.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));
 		RGL_HIP(hipMemcpy(out_prim, c.get(), sizeof(int32_t) * rays, hipMemcpyDeviceToHost));

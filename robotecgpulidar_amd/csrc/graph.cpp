@@ -798,6 +798,7 @@ void RaytraceNode::bruteForce(float* t, int32_t* inst, int32_t* prim)
 	                  sizeof(int32_t) * counts.size(), hipMemcpyHostToDevice));
 	gpu::bruteForceClosest(stream(), view, dc.as<int32_t>(), buildParams(view), t, inst, prim);
 	RGL_HIP(hipGetLastError());
+	RGL_HIP(hipStreamSynchronize(stream())); // the kernel reads `dc`, released on return
 	RGL_HIP(hipStreamSynchronize(stream()));
 }
 

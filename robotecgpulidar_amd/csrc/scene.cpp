@@ -480,9 +480,7 @@ std::vector<int32_t> Scene::instanceTriCounts() const
 
 void Scene::rebuild(hipStream_t s)
 {
-	// The previous rebuild's uploads read the pinned staging arrays reused below.
-	RGL_HIP(hipStreamSynchronize(s));
-	// Raytrace launches still in flight read the device arrays rewritten below.
+	// Raytrace launches still in flight read the device arrays rewritten below: the scene stream waits for them.
 	for (auto& [user, ev] : uses) {
 		RGL_HIP(hipStreamWaitEvent(s, ev->get(), 0));
 	}
@@ -498,8 +496,15 @@ void Scene::rebuild(hipStream_t s)
 		view.overflow = overflowFlag();
 		return;
 	}
-	auto* inst = static_cast<InstanceRec*>(hostInst.ensure(sizeof(InstanceRec) * n));
-	auto* shade = static_cast<InstanceShade*>(hostShade.ensure(sizeof(InstanceShade) * n));
+	const int set = stagingSet;
+	stagingSet ^= 1;
+	if (!staged[set]) {
+		staged[set] = std::make_unique<Event>();
+	} else if (staged[set]->wasRecorded()) {
+		staged[set]->sync(); // the upload before the last one still reads this set
+	}
+	auto* inst = static_cast<InstanceRec*>(hostInst[set].ensure(sizeof(InstanceRec) * n));
+	auto* shade = static_cast<InstanceShade*>(hostShade[set].ensure(sizeof(InstanceShade) * n));
 	for (int i = 0; i < n; ++i) {
 		const Entity& e = *entities[i];
 		Blas& blas = entities[i]->tracedBlas(s);
@@ -528,6 +533,7 @@ void Scene::rebuild(hipStream_t s)
 	RGL_HIP(hipMemcpyAsync(dInst.ensure(sizeof(InstanceRec) * n), inst, sizeof(InstanceRec) * n, hipMemcpyHostToDevice, s));
 	RGL_HIP(hipMemcpyAsync(dShade.ensure(sizeof(InstanceShade) * n), shade, sizeof(InstanceShade) * n,
 	                       hipMemcpyHostToDevice, s));
+	staged[set]->record(s);
 	const int inner = std::max(n - 1, 1);
 	dBoxes.ensure(sizeof(Aabb) * n);
 	gpu::instanceBoxes(s, n, dShade.as<InstanceShade>(), dInst.as<InstanceRec>(), dBoxes.as<Aabb>());

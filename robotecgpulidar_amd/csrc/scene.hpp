@@ -208,7 +208,11 @@ private:
 	int maxBlasDepth = 0;
 	std::unique_ptr<Stream> stream;
 	std::unique_ptr<Event> ready;
-	PinnedBuffer hostInst, hostShade;
+	// Two sets of pinned staging arrays for the instance uploads, used in turn; `staged[k]` records the end of the
+	// upload from set k, so a rebuild waits on the host only while the one before the last still reads its set.
+	PinnedBuffer hostInst[2], hostShade[2];
+	std::unique_ptr<Event> staged[2];
+	int stagingSet = 0;
 	DeviceBuffer dInst, dShade, dBoxes, tlasNodes, tlasLeafPrim, tlasParent, tlasCounters, tlasHeight, tlasScratch;
 	DeviceBuffer dOverflow;
 	DeviceBuffer tlasWide, tlasWideScratch, tlasWideCount; // 4-wide TLAS (when the format is 4-wide)
