@@ -198,6 +198,26 @@ __device__ __forceinline__ int batchItemOf(const Item* items, int itemCount, int
 	}
 	return __builtin_amdgcn_readfirstlane(count - 1);
 }
+
+// Workgroups of a launch are dealt to the 8 XCDs of the device (each with its own L2) in turn: hardware block b
+// runs on XCD b % 8. The logical block of b gives XCD x the chunks x, x + 8, x + 16, ... of `chunk` consecutive
+// logical blocks, so the rays one L2 serves at a time are neighbours (they read the same node and triangle records)
+// while the XCDs still share the work evenly; chunk <= 0 spreads each XCD's share in one contiguous range. Blocks
+// past the last whole round keep their index, so the mapping is a permutation of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcdLogicalBlock(int b, int n, int chunk)
+{
+	if (chunk <= 0) {
+		chunk = max(1, n / kXcds);
+	}
+	const int round = kXcds * chunk;
+	const int whole = n / round * round;
+	if (b >= whole) {
+		return b;
+	}
+	const int x = b % kXcds, j = b / kXcds;
+	return ((j / chunk) * kXcds + x) * chunk + j % chunk;
+}
 #endif
 
 // ------------------------------------------------------------------------------------------------ points

@@ -43,18 +43,33 @@ constexpr int kRtBlock = RGL_RT_BLOCK; // threads per raytrace block (the LDS sh
 #else
 #define RGL_RT_OCCUPANCY
 #endif
-constexpr int kLdsStack = 24;
+#ifndef RGL_LDS_STACK
+#define RGL_LDS_STACK 24
+#endif
+constexpr int kLdsStack = RGL_LDS_STACK;
 constexpr int kOvfStack = 104;
+// Consecutive logical blocks per XCD turn of the raytrace launches (xcdLogicalBlock; <= 0: one range per XCD).
+#ifndef RGL_XCD_CHUNK
+#define RGL_XCD_CHUNK 64
+#endif
+constexpr int kXcdChunk = RGL_XCD_CHUNK;
 constexpr int kPop = INT_MIN;
 constexpr int kBeamSamples = 58;  // 1 primary + 3 layers x 19 vertices (src/gpu/MultiReturn.hpp:19-21)
 constexpr int kBeamLayers = 3;
 constexpr int kBeamVertices = 19;
-// Slab-test widening: a computed slab distance (bound - origin) * invDir is within 4 units of roundoff
-// (u = 2^-24: the subtraction and the product round once each, the reciprocal is within one ulp) of the exact one
-// and keeps its sign, so comparing the computed near distance against the far distance (or the closest hit) grown
-// by 2^-20 = 16u, with the growth itself rounded (one more u), keeps every box the exact test enters:
-// (1 + 4u) / (1 - 4u) < (1 + 16u)(1 - u).
+// Conservative slab test. The distance of the plane p on axis a is computed as one fma, fma(p, m, -c), with per-ray
+// constants (SlabRay) instead of (p - o) * inv. With u = 2^-24, inv the hardware reciprocal (within 2u of 1 / dir),
+// q = fl(o * m) and the slack S = 2^-22 = 4u:
+//  * near planes use m = inv and c = fl(q + |q| S) >= o * inv, so the computed distance is at most
+//    (p - o) * inv rounded once: within (1 + 2u)(1 + u) of the exact one, or <= 0 if that is negative;
+//  * far planes use m = fl(inv * G), G = kSlabGrow = 1 + 16u, and c = fl(q - |q| S) <= o * m, so the computed
+//    distance is at least the exact one times (1 - 2u)(1 + 16u)(1 - u)^2 > 1 + 8u when positive;
+// which plane of an axis is near follows from the sign of inv. The slack moves each axis' interval by 2^-22 |o_a|
+// in space at most, and the growth by 16u relative: every box the exact test enters ([max near, min far] meets
+// [0, tBest]) is entered, since (1 + 2u)(1 + u) < 1 + 8u and the bound tBest is grown by G as well.
+// The closest hit, and therefore every result, does not depend on which extra boxes are entered.
 constexpr float kSlabGrow = 1.0f + 9.5367431640625e-7f;
+constexpr float kSlabSlack = 2.384185791015625e-7f; // 2^-22
 
 // Component selection by a one-hot vector e: e.x*v.x + e.y*v.y + e.z*v.z is exactly the selected component (the
 // products are exact, the other terms are zeros). Selecting by a per-lane index otherwise compiles to branches or
@@ -248,26 +263,53 @@ __device__ void touchLines(const LineSet* s, const void* p, int bytes)
 // finite, so that a box whose near distance is +inf (the empty-slot box of Bvh4FNode) is never entered.
 __device__ __forceinline__ float slabBound(float tBest) { return fminf(tBest * kSlabGrow, FLT_MAX); }
 
-// Conservative slab test: the box is entered if [max(tn, 0), min(tf, tBest)] is non-empty after the far distance
-// and tBest are grown by kSlabGrow. tBound = slabBound(tBest).
-__device__ __forceinline__ void slab(const Vec3f& ro, const Vec3f& ri, const float* f, float tBound, bool& hit,
-                                     float& tNear)
+// Slab-test constants of a ray at one level (world or object space): the distance of a plane p bounding axis a
+// from below is fma(p, mlo.a, -clo.a), from above fma(p, mhi.a, -chi.a) (near or far constants by the sign of the
+// reciprocal direction).
+struct SlabRay
 {
-	const float tx0 = (f[0] - ro.x) * ri.x, tx1 = (f[3] - ro.x) * ri.x;
-	const float ty0 = (f[1] - ro.y) * ri.y, ty1 = (f[4] - ro.y) * ri.y;
-	const float tz0 = (f[2] - ro.z) * ri.z, tz1 = (f[5] - ro.z) * ri.z;
+	Vec3f mlo, clo, mhi, chi;
+};
+
+__device__ __forceinline__ void slabAxis(float o, float inv, float& mlo, float& clo, float& mhi, float& chi)
+{
+	const float invF = inv * kSlabGrow;
+	const float qN = o * inv, qF = o * invF;
+	const float cN = fmaf(fabsf(qN), kSlabSlack, qN);
+	const float cF = fmaf(-fabsf(qF), kSlabSlack, qF);
+	const bool up = inv >= 0.0f; // the lower plane is the near one
+	mlo = up ? inv : invF;
+	clo = up ? cN : cF;
+	mhi = up ? invF : inv;
+	chi = up ? cF : cN;
+}
+
+__device__ __forceinline__ SlabRay slabRay(Vec3f o, Vec3f inv)
+{
+	SlabRay r;
+	slabAxis(o.x, inv.x, r.mlo.x, r.clo.x, r.mhi.x, r.chi.x);
+	slabAxis(o.y, inv.y, r.mlo.y, r.clo.y, r.mhi.y, r.chi.y);
+	slabAxis(o.z, inv.z, r.mlo.z, r.clo.z, r.mhi.z, r.chi.z);
+	return r;
+}
+
+// Conservative slab test of the box (lx, ly, lz) - (hx, hy, hz): entered if [max(tn, 0), min(tf, tBound)] is
+// non-empty, tBound = slabBound(tBest); tNear = tn.
+__device__ __forceinline__ void slab6(const SlabRay& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                      float tBound, bool& hit, float& tNear)
+{
+	const float tx0 = fmaf(lx, r.mlo.x, -r.clo.x), tx1 = fmaf(hx, r.mhi.x, -r.chi.x);
+	const float ty0 = fmaf(ly, r.mlo.y, -r.clo.y), ty1 = fmaf(hy, r.mhi.y, -r.chi.y);
+	const float tz0 = fmaf(lz, r.mlo.z, -r.clo.z), tz1 = fmaf(hz, r.mhi.z, -r.chi.z);
 	const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
 	const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-	hit = fmaxf(tn, 0.0f) <= fminf(tf * kSlabGrow, tBound);
+	hit = fmaxf(tn, 0.0f) <= fminf(tf, tBound);
 	tNear = tn;
 }
 
-// Slab test of a decoded child box.
-__device__ __forceinline__ void slab6(const Vec3f& ro, const Vec3f& ri, float lx, float ly, float lz, float hx, float hy,
-                                      float hz, float tBound, bool& hit, float& tNear)
+__device__ __forceinline__ void slab(const SlabRay& r, const float* f, float tBound, bool& hit, float& tNear)
 {
-	const float f[6] = {lx, ly, lz, hx, hy, hz};
-	slab(ro, ri, f, tBound, hit, tNear);
+	slab6(r, f[0], f[1], f[2], f[3], f[4], f[5], tBound, hit, tNear);
 }
 
 // Node and triangle records are read through global-address-space pointers, so the loads are global (not flat)
@@ -276,17 +318,60 @@ typedef float RglF4 __attribute__((ext_vector_type(4)));
 typedef int RglI4 __attribute__((ext_vector_type(4)));
 typedef unsigned RglU4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) RglF4 GlobalFloat4;
-typedef const __attribute__((address_space(1))) RglI4 GlobalInt4;
-typedef const __attribute__((address_space(1))) RglU4 GlobalUint4;
 template<typename T>
 __device__ __forceinline__ T* globalPtr(const void* p)
 {
 	return (T*)(p);
 }
 
+// Records every active lane of the wave reads alike are fetched with scalar loads (through the scalar cache) instead:
+// vector loads deliver each lane its own copy through the vector memory data path, which node and triangle fetches
+// otherwise keep busy, while one scalar load serves the whole wave. constant-address-space pointers built from
+// readfirstlane values compile to scalar loads; the records are not written while a launch reads them.
+typedef const __attribute__((address_space(4))) RglF4 ConstFloat4;
+__device__ __forceinline__ const void* firstLanePtr(const void* p)
+{
+	const uint64_t a = reinterpret_cast<uint64_t>(p);
+	const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
+	const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
+	return reinterpret_cast<const void*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ bool waveUniform(const void* p)
+{
+	return __ballot(p == firstLanePtr(p)) == __ballot(1);
+}
+template<typename T>
+__device__ __forceinline__ T* scalarPtr(const void* uniformP)
+{
+	return (T*)(uniformP);
+}
+
+// A record of `kQuads` 16-byte quads at p (16-byte aligned), by scalar loads when every active lane reads the same
+// one (about two thirds of the node steps of the 64-lidar frame), else by vector loads. (Serving the lanes of the first
+// lane's record by scalar loads also in divergent steps, alone or one distinct record after another, measured 15-20%
+// slower: the scalar load's latency then adds to the vector load's.)
+template<int kQuads>
+__device__ __forceinline__ void loadQuads(const void* p, RglF4 (&q)[kQuads])
+{
+	if (waveUniform(p)) {
+		ConstFloat4* sp = scalarPtr<ConstFloat4>(firstLanePtr(p));
+#pragma unroll
+		for (int i = 0; i < kQuads; ++i) {
+			q[i] = sp[i];
+		}
+		return;
+	}
+	GlobalFloat4* vp = globalPtr<GlobalFloat4>(p);
+#pragma unroll
+	for (int i = 0; i < kQuads; ++i) {
+		q[i] = vp[i];
+	}
+}
+
 __device__ __forceinline__ LeafTri loadLeafTri(const LeafTri* tris, int leaf)
 {
-	GlobalFloat4* q = globalPtr<GlobalFloat4>(tris + leaf);
+	RglF4 q[3];
+	loadQuads<3>(tris + leaf, q);
 	const RglF4 a = q[0], b = q[1], c = q[2];
 	LeafTri t;
 	t.ax = a.x;
@@ -361,14 +446,14 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		r##b = rb_;                                                                                                    \
 	} while (0)
 
-	const Vec3f worldInv = invDir(d);
-	// Ray of the node steps at the current level (world in the TLAS, object space in a BLAS) and the object-space
-	// ray record of the entered instance; both change only at the top of the node loop (instance entry, and the
-	// return to the TLAS after the pops left the instance).
-	Vec3f ro = o, ri = worldInv;
+	// Slab constants of the node steps at the current level (world in the TLAS, object space in a BLAS) and the
+	// object-space ray record of the entered instance; both change only at the top of the node loop (instance
+	// entry, and the return to the TLAS after the pops left the instance). The world constants are recomputed on
+	// each return from the origin kept in LDS (o is its FP32 rounding) rather than held in registers.
+	SlabRay sr = slabRay(o, invDir(d));
 	TravRay obj;
-	obj.o = o;
-	obj.inv = worldInv;
+	obj.o = Vec3f{0.0f, 0.0f, 0.0f};
+	obj.inv = obj.o;
 	obj.sx = obj.sy = obj.sz = 0.0f;
 	obj.ex = obj.ey = obj.ez = oneHot(0);
 	const void* tlas = kStats ? static_cast<const void*>(sc.tlas2) : sc.tlasNodes;
@@ -391,15 +476,16 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		}
 		int next;
 		if constexpr (kFmt == kBvh2) {
-			GlobalFloat4* np = globalPtr<GlobalFloat4>(static_cast<const BvhNode*>(nodes) + node);
-			const RglF4 q0 = np[0], q1 = np[1], q2 = np[2];
-			const RglI4 q3 = globalPtr<GlobalInt4>((const void*)np)[3];
+			RglF4 q[4];
+			loadQuads<4>(static_cast<const BvhNode*>(nodes) + node, q);
+			const RglF4 q0 = q[0], q1 = q[1], q2 = q[2];
+			const RglI4 q3 = __builtin_bit_cast(RglI4, q[3]);
 			const float b0[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
 			const float b1[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 			bool h0, h1;
 			float n0, n1;
-			slab(ro, ri, b0, tBound, h0, n0);
-			slab(ro, ri, b1, tBound, h1, n1);
+			slab(sr, b0, tBound, h0, n0);
+			slab(sr, b1, tBound, h1, n1);
 			if (h0 && h1) {
 				const bool firstNear = n0 <= n1;
 				RGL_PUSH(firstNear ? q3.y : q3.x);
@@ -418,20 +504,23 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			int r0, r1, r2, r3;
 			if constexpr (kFmt == kBvh4F) {
 				// Empty slots hold boxes no ray enters, so the child count is not read.
-				GlobalFloat4* np = globalPtr<GlobalFloat4>(static_cast<const Bvh4FNode*>(nodes) + node);
-				const RglF4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-				const RglI4 ch = globalPtr<GlobalInt4>((const void*)np)[6];
-				slab6(ro, ri, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBound, h0, k0);
-				slab6(ro, ri, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBound, h1, k1);
-				slab6(ro, ri, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBound, h2, k2);
-				slab6(ro, ri, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBound, h3, k3);
+				RglF4 q[7];
+				loadQuads<7>(static_cast<const Bvh4FNode*>(nodes) + node, q);
+				const RglF4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+				const RglI4 ch = __builtin_bit_cast(RglI4, q[6]);
+				slab6(sr, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tBound, h0, k0);
+				slab6(sr, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tBound, h1, k1);
+				slab6(sr, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tBound, h2, k2);
+				slab6(sr, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tBound, h3, k3);
 				r0 = ch.x;
 				r1 = ch.y;
 				r2 = ch.z;
 				r3 = ch.w;
 			} else {
-				GlobalUint4* up = globalPtr<GlobalUint4>(static_cast<const Bvh4QNode*>(nodes) + node);
-				const RglU4 u0 = up[0], u1 = up[1], u2 = up[2], u3 = up[3];
+				RglF4 q[4];
+				loadQuads<4>(static_cast<const Bvh4QNode*>(nodes) + node, q);
+				const RglU4 u0 = __builtin_bit_cast(RglU4, q[0]), u1 = __builtin_bit_cast(RglU4, q[1]);
+				const RglU4 u2 = __builtin_bit_cast(RglU4, q[2]), u3 = __builtin_bit_cast(RglU4, q[3]);
 				const float ox = __uint_as_float(u0.x), oy = __uint_as_float(u0.y), oz = __uint_as_float(u0.z);
 				// exp byte = e + 127, so (byte << 23) is the bit pattern of 2^e.
 				const float sx = __uint_as_float((u0.w & 0xffu) << 23);
@@ -439,7 +528,7 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				const float sz = __uint_as_float(((u0.w >> 16) & 0xffu) << 23);
 				const int cnt = static_cast<int>(u0.w >> 24);
 #define RGL_QBOX(c, h, k)                                                                                              \
-	slab6(ro, ri, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                             \
+	slab6(sr, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                             \
 	      __fadd_rn(oz, ubyte(u1.z, c) * sz), __fadd_rn(ox, ubyte(u1.w, c) * sx), __fadd_rn(oy, ubyte(u2.x, c) * sy),  \
 	      __fadd_rn(oz, ubyte(u2.y, c) * sz), tBound, h, k)
 				RGL_QBOX(0, h0, k0);
@@ -518,13 +607,18 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 	auto enterInstance = [&](int leafRef) {
 		inst = ~leafRef;
 		instBase = sp;
-		const InstanceRec& ir = sc.instances[inst];
+		static_assert(sizeof(InstanceRec) == 10 * sizeof(RglF4), "instance record read as 10 quads");
+		InstanceRec ir;
+		{
+			RglF4 q[10];
+			loadQuads<10>(sc.instances + inst, q);
+			__builtin_memcpy(&ir, q, sizeof(ir));
+		}
 		if constexpr (kFoot) {
-			touchLines(ls, &ir, sizeof(InstanceRec));
+			touchLines(ls, sc.instances + inst, sizeof(InstanceRec));
 		}
 		obj = objectRay(ir, loadOrigin<kBlock>(), d);
-		ro = obj.o;
-		ri = obj.inv;
+		sr = slabRay(obj.o, obj.inv);
 		nodes = kStats ? static_cast<const void*>(sc.shades[inst].blas2) : ir.blasNodes;
 		tris = ir.blasTris;
 		inBlas = true;
@@ -533,8 +627,12 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 		}
 	};
 	auto returnToTlas = [&]() {
-		ro = o;
-		ri = worldInv;
+		// Opaque copies keep the recomputation here: hoisted out of the loop, its twelve results would take
+		// registers (or scratch) for the whole traversal.
+		Vec3d o64 = loadOrigin<kBlock>();
+		Vec3f dd = d;
+		asm volatile("" : "+v"(o64.x), "+v"(o64.y), "+v"(o64.z), "+v"(dd.x), "+v"(dd.y), "+v"(dd.z));
+		sr = slabRay(round32(o64), invDir(dd));
 		nodes = tlas;
 		inBlas = false;
 	};
@@ -1014,10 +1112,10 @@ __device__ __forceinline__ void applyFusedStages(const RaytraceBatchItem& it, in
 template<int kFmt, bool kMulti, int kBlock = kRtBlock>
 __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items, int itemCount)
 {
-	const int item = batchItemOf(items, itemCount, static_cast<int>(blockIdx.x),
-	                             [](const RaytraceBatchItem& r) { return r.firstBlock; });
+	const int block = xcdLogicalBlock(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), kXcdChunk);
+	const int item = batchItemOf(items, itemCount, block, [](const RaytraceBatchItem& r) { return r.firstBlock; });
 	const RaytraceBatchItem& it = items[item];
-	const int localBlock = static_cast<int>(blockIdx.x) - it.firstBlock;
+	const int localBlock = block - it.firstBlock;
 	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, localBlock);
 	if constexpr (!kMulti) {
 		const int beam = localBlock * kBlock + threadIdx.x;
@@ -1031,13 +1129,13 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 template<int kFmt>
 __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytrace(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
-	raytraceOne<kFmt, false>(sc, p, out, blockIdx.x);
+	raytraceOne<kFmt, false>(sc, p, out, xcdLogicalBlock(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), kXcdChunk));
 }
 
 template<int kFmt>
 __global__ void __launch_bounds__(kRtBlock) RGL_RT_OCCUPANCY kRaytraceMulti(SceneView sc, RaytraceParams p, RaytraceOutputs out)
 {
-	raytraceOne<kFmt, true>(sc, p, out, blockIdx.x);
+	raytraceOne<kFmt, true>(sc, p, out, xcdLogicalBlock(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), kXcdChunk));
 }
 
 template<int kFmt, int kBlock>
