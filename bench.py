@@ -124,8 +124,9 @@ def roofline(b_io: int, footprint: int, b_canonical: int, kernel_ms: float, prof
     records and shading records its rays touch (footprint, each 64 B line once, rgl_graph_traversal_footprint).
     `traffic` is the DRAM bytes rocprof counted for the same kernel (FETCH_SIZE + WRITE_SIZE, scaled by a copy
     kernel of known size); `effective_gbs` is the canonical per-ray figure (every node and triangle fetch of every
-    ray counted) over the same time; `valu_issue_busy` the fraction of SIMD cycles issuing VALU instructions in the
-    committed counter run (the traversal's actual limiter)."""
+    ray counted) over the same time; `limiter` the traversal's actual limiter from the committed counter run: the
+    fraction of the launch the vector memory data return path (TD) is busy, most of it waiting on L1/L2 returns of
+    divergent node fetches."""
     t = kernel_ms * 1e-3
     b_alg = b_io + footprint
     achieved = b_alg / t / 1e9
@@ -136,8 +137,13 @@ def roofline(b_io: int, footprint: int, b_canonical: int, kernel_ms: float, prof
            "effective_gbs": round((b_io + b_canonical) / t / 1e9, 2), "canonical_bytes_per_launch": b_io + b_canonical}
     if traffic:
         out["dram_frac"] = round(traffic / t / 1e9 / HBM_PEAK_GBS, 4)
-    if profile and "valu_issue_busy" in profile:
-        out["valu_issue_busy"] = profile["valu_issue_busy"]
+    if profile and "td_busy" in profile:
+        out["limiter"] = {"unit": "vector memory data path (TD) busy", "frac": profile["td_busy"],
+                          "td_stalled_on_cache": profile["td_tc_stall"], "ta_busy": profile["ta_busy"],
+                          "vector_loads_per_wave": profile["vector_loads_per_wave"],
+                          "scalar_loads_per_wave": profile["scalar_loads_per_wave"],
+                          "valu_instructions_per_wave": profile["valu_instructions_per_wave"],
+                          "source": profile["source"]}
     return out
 
 
