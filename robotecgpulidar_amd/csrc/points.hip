@@ -106,8 +106,11 @@ __device__ __forceinline__ void scatterField(const FieldCopy& f, int base, const
 	}
 }
 
-typedef unsigned Word3 __attribute__((ext_vector_type(3)));
-typedef unsigned Word2 __attribute__((ext_vector_type(2)));
+// Three- and two-word groups of 12- and 8-byte records, which are only 4-byte aligned: the vector types would
+// otherwise carry their natural 16- and 8-byte alignment, which the compiler may assume when it splits or merges
+// the accesses.
+typedef unsigned Word3 __attribute__((ext_vector_type(3), aligned(4)));
+typedef unsigned Word2 __attribute__((ext_vector_type(2), aligned(4)));
 
 // 4-byte accesses through global-address-space pointers: global (not flat) loads and stores count only on the vector
 // memory counter, so reading the LDS-staged field list does not wait for the data loads in flight.

@@ -9,6 +9,9 @@ bar (tests.helpers.check_golden), also under the binary and the 8-bit quantized 
 node prefetching into pinned host memory (chunked copies after the batch), results read back in any sensor order
 equal those read from device memory.
 
+C2 (scenes.config2_scene: the 2.0M-triangle urban scene): the lidar's angular ray noise and distance noise with
+velocity distortion on every 64th ray match the FP64 oracle fed the node's replayed noise stream.
+
 C3 (scenes.config3_scene: ~10M triangles, 2,001 instances, 58-sample beams): over three frames in which 20% of the
 instances move by U[0, 0.5] m, FIRST_LAST and STRONGEST results with a refitted TLAS equal those with a rebuilt
 one bit for bit; and on a 2,001-instance scene of ~1M triangles, refitted traversal equals brute force over every
@@ -30,7 +33,7 @@ import pytest
 from robotecgpulidar_amd import scenes as S
 from robotecgpulidar_amd import workloads as W
 from tests.golden import cases
-from tests.helpers import check_golden
+from tests.helpers import check_golden, compose64
 
 pytestmark = pytest.mark.gpu
 
@@ -181,6 +184,63 @@ def test_c4_prefetch_chunks_equal_device_readback(rgl_lib):
                 assert got[f].tobytes() == want[k][f].tobytes(), (frame, k, f)
                 moved += got[f].nbytes
         say(f"C4 frame {frame}: prefetched results of 64 sensors equal device read-back ({moved / 1e6:.1f} MB)")
+
+
+def test_c2_urban_noise_matches_oracle(rgl_lib, oracle):
+    """C2's noise on the 2.0M-triangle urban scene: every 64th ray of the 128 x 2048 lidar (4,096 rays) through
+    pose -> range -> firing time offsets -> angular ray noise (sigma 0.05 deg about the sensor's Z) -> raytrace with
+    velocity distortion at 10 m/s -> distance noise (sigma 0.02 m + 0.001 d). The rays the raytrace node traced
+    (RAY_POSE) equal the oracle's FP64 noise rotation of the composed rays with the same normal variates (the
+    node's seeded stream replayed on the CPU); the oracle then traces those rays with distortion and applies the
+    distance noise in FP64, and the GPU's hits, distances and points match it to the parity bar."""
+    rgl = rgl_lib
+    spec, lidar = S.config2_scene(), S.config2_lidar()
+    S.load_into_rgl(spec)
+    sel = np.arange(0, len(lidar.rays), 64)
+    rays, offsets = lidar.rays[sel], lidar.time_offsets[sel]
+    n = rgl.Node
+    ang, dist = n(), n()
+    rgl.check("rgl_node_gaussian_noise_angular_ray", ang.ref, 0.0, W.ANGULAR_SIGMA, rgl.RGL_AXIS_Z)
+    rgl.check("rgl_node_gaussian_noise_distance", dist.ref, 0.0, 0.02, 0.001)
+    rt = rgl.node_raytrace(n())
+    lin, angv = np.array([10.0, 0.0, 0.0], np.float32), np.zeros(3, np.float32)
+    rgl.check("rgl_node_raytrace_configure_velocity", rt.h, lin.ctypes.data, angv.ctypes.data)
+    rgl.check("rgl_node_raytrace_configure_distortion", rt.h, True)
+    pose_out = rgl.node_points_yield(n(), [24])  # RAY_POSE of the traced rays, beside the raytrace node
+    out = rgl.node_points_yield(n(), [XYZ, DIST, IS_HIT])
+    chain = [rgl.node_rays_from_mat3x4f(n(), rays), rgl.node_rays_transform(n(), lidar.pose),
+             rgl.node_rays_set_range(n(), np.array([lidar.range], np.float32)),
+             rgl.node_rays_set_time_offsets(n(), offsets), ang, rt, dist, out]
+    rgl.chain(*chain)
+    rgl.add_child(rt, pose_out)
+    rgl.check("rgl_node_gaussian_noise_set_seed", ang.h, C.c_uint64(0xC0F2A))
+    rgl.check("rgl_node_gaussian_noise_set_seed", dist.h, C.c_uint64(0xC0F2D))
+    rgl.graph_run(chain[0])
+    got = {f: rgl.result(out, f) for f in (XYZ, DIST, IS_HIT)}
+    traced = rgl.result(pose_out, 24).reshape(-1, 3, 4)
+
+    def variates(node):
+        seed, run, tag = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        rgl.check("rgl_node_gaussian_noise_get_stream", node.h, C.byref(seed), C.byref(run), C.byref(tag))
+        return oracle.normals(seed.value, run.value - 1, tag.value, len(rays))
+
+    look_at = np.linalg.inv(np.vstack([lidar.pose.astype(np.float64), [0, 0, 0, 1]]))[:3]
+    world = compose64(lidar.pose, rays)
+    noisy = oracle.noise_angular_rays(world, variates(ang), 0.0, W.ANGULAR_SIGMA, rgl.RGL_AXIS_Z, look_at)
+    np.testing.assert_allclose(traced, noisy, rtol=1e-6, atol=2e-6)
+    ref = oracle.raytrace(spec, traced.astype(np.float64), ray_origin_to_world=lidar.pose,
+                          ranges=np.array([lidar.range], np.float32), time_offsets=offsets, distortion=True,
+                          lin_vel=(10.0, 0.0, 0.0), fields=[XYZ, DIST, IS_HIT])
+    amb = ref["ambiguous"] != 0
+    mism = got[IS_HIT] != ref[IS_HIT]
+    assert not (mism & ~amb).any() and mism.sum() <= 2, int(mism.sum())
+    both = (got[IS_HIT] == 1) & (ref[IS_HIT] == 1) & ~amb
+    xyz, d = oracle.noise_distance(ref[XYZ][both], ref[DIST][both], variates(dist)[both], 0.0, 0.02, 0.001, look_at)
+    np.testing.assert_allclose(got[DIST][both], d, rtol=1e-5, atol=1e-3)
+    np.testing.assert_allclose(got[XYZ][both], xyz, atol=2e-3)
+    say(f"C2 noise on the urban scene: {both.sum()} hits compared, max |d - d_FP64| "
+        f"{np.abs(got[DIST][both] - d).max():.2e} m, traced rays within "
+        f"{np.abs(traced - noisy).max():.1e} of the FP64 noise rotation")
 
 
 def move_c3(rgl, ents, base, movable, pos, rng, fraction=0.2):
