@@ -615,6 +615,13 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 	};
 	Track first, second, last, strongest, secondStrongest;
 	float sample0Distance = q.farDistance;
+	// Every (first, second) distance pair and (strongest, second strongest) intensity pair the reduction can reach
+	// when each comparison whose operands are within FP32's reach of each other may go either way (at most kStates
+	// each; `overflow` if more). The returned SECOND and SECOND_STRONGEST lie within the range of the second
+	// elements.
+	constexpr size_t kStates = 512;
+	std::vector<std::pair<double, double>> dStates, iStates;
+	bool statesOverflow = false;
 	// Beam reduction compares samples strictly and in sample order (nodeKernels.cu:369-397): a comparison whose
 	// operands are within FP32's reach of each other may go the other way on the GPU and change the selection.
 	// Distances carry ~eps relative error; intensities texel*cos(incidence) carry ~eps relative plus about
@@ -668,6 +675,56 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 				intTie = true;
 			}
 		}
+		if (samples > 1 && out.secondRange != nullptr) {
+			// lt(a, b): the outcomes a < b can take on the GPU (bit 0: false possible, bit 1: true possible).
+			auto outcomes = [&](double a, double b, bool nearAB) {
+				const bool exact = a < b;
+				return nearAB ? 3 : (exact ? 2 : 1);
+			};
+			auto advance = [&](std::vector<std::pair<double, double>>& st, double x, double base, bool byIntensity) {
+				if (st.empty()) {
+					st.push_back({x, x});
+					return;
+				}
+				std::vector<std::pair<double, double>> next;
+				auto add = [&](double f, double g) {
+					for (auto& e : next) {
+						if (e.first == f && e.second == g) {
+							return;
+						}
+					}
+					next.push_back({f, g});
+				};
+				for (const auto& [f, g] : st) {
+					// Distances: (x < f) -> (x, f); else if (f < x && x < g) -> (f, x). Intensities mirror it
+					// with > (the strongest tracker).
+					auto near2 = [&](double a, double b2) {
+						return byIntensity ? nearI(a, b2, 255.0, 255.0) : nearD(a, b2);
+					};
+					const int o1 = byIntensity ? outcomes(f, x, near2(f, x)) : outcomes(x, f, near2(x, f));
+					if (o1 & 2) {
+						add(x, f);
+					}
+					if (o1 & 1) {
+						const int o2 = byIntensity ? outcomes(x, f, near2(x, f)) : outcomes(f, x, near2(f, x));
+						const int o3 = byIntensity ? outcomes(g, x, near2(g, x)) : outcomes(x, g, near2(x, g));
+						if ((o2 & 2) && (o3 & 2)) {
+							add(f, x);
+						}
+						if ((o2 & 1) || (o3 & 1)) {
+							add(f, g);
+						}
+					}
+				}
+				if (next.size() > kStates) {
+					statesOverflow = true;
+					next.resize(kStates);
+				}
+				st.swap(next);
+			};
+			advance(dStates, c.distance, 0.0, false);
+			advance(iStates, c.intensity, c.sh.base, true);
+		}
 		if (first.sample < 0) {
 			first = second = last = strongest = secondStrongest = c;
 			if (gMistake == 1) {
@@ -696,6 +753,25 @@ void beam(const World& w, const OrcRequest& q, OrcOutputs& out, int b)
 		out.ambiguous[b] =
 		    static_cast<int8_t>((ambiguous ? 1 : 0) | (distTie ? 2 : 0) | (intTie ? 4 : 0) | (texelEdge ? 8 : 0) |
 		                        (instanceTie ? 16 : 0));
+	}
+	if (out.secondRange != nullptr) {
+		double dLo = INFINITY, dHi = -INFINITY, iLo = INFINITY, iHi = -INFINITY;
+		for (const auto& e : dStates) {
+			dLo = std::min(dLo, e.second);
+			dHi = std::max(dHi, e.second);
+		}
+		for (const auto& e : iStates) {
+			iLo = std::min(iLo, e.second);
+			iHi = std::max(iHi, e.second);
+		}
+		if (statesOverflow || samples == 1) {
+			dLo = iLo = -INFINITY;
+			dHi = iHi = INFINITY;
+		}
+		out.secondRange[4 * b + 0] = static_cast<float>(dLo);
+		out.secondRange[4 * b + 1] = static_cast<float>(dHi);
+		out.secondRange[4 * b + 2] = static_cast<float>(iLo);
+		out.secondRange[4 * b + 3] = static_cast<float>(iHi);
 	}
 	for (int r = 0; r < R; ++r) {
 		const int rp = b * R + r;

@@ -99,6 +99,10 @@ typedef struct
 	double* primaryT;      /* FP64 ray parameter of the primary sample's closest hit, -1 if none */
 	int32_t* primaryInst;  /* instance of that hit, -1 if none */
 	int32_t* primaryPrim;  /* triangle of that hit, -1 if none */
+	/* per beam, 4 floats: the least and greatest SECOND distance and SECOND_STRONGEST intensity the beam reduction
+	   can return when every comparison within FP32's reach of a tie may go either way (+-inf if unbounded: more
+	   than 512 reachable tracker states, single-sample beams, no hit) */
+	float* secondRange;
 } OrcOutputs;
 
 /* Runs the whole raytrace node (raygen, closest hit, miss, beam reduction) for every beam. Returns 0 on success. */

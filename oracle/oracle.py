@@ -62,7 +62,7 @@ _CTYPE = {np.float32: C.c_float, np.int32: C.c_int32, np.uint32: C.c_uint32, np.
 class OrcOutputs(C.Structure):
     _fields_ = [(n, C.POINTER(_CTYPE[dt])) for n, _, dt, _ in OUT_FIELDS] + [
         ("ambiguous", C.POINTER(C.c_int8)), ("primaryT", C.POINTER(C.c_double)), ("primaryInst", i32p),
-        ("primaryPrim", i32p),
+        ("primaryPrim", i32p), ("secondRange", f32p),
     ]
 
 
@@ -197,7 +197,7 @@ def raytrace(spec, rays_world, ray_origin_to_world=None, ranges=((0.0, 3.4028234
     """Runs the raytrace node semantics on the CPU. rays_world: (N, 3, 4) rays after the rays chain; a float64
     array is traced as given (FP64 world rays, direction = third column), anything else as float32 rays with the
     reference's float rounding of the direction. Returns a dict of per-return-point arrays keyed by rgl field id,
-    plus 'ambiguous', 'primaryT', 'primaryInst', 'primaryPrim'."""
+    plus 'ambiguous', 'primaryT', 'primaryInst', 'primaryPrim' and 'second_range'."""
     lib = load()
     keep = _Keep()
     sc = make_scene(spec, keep, prev_transforms, displacements) if world is None else None
@@ -251,6 +251,10 @@ def raytrace(spec, rays_world, ray_origin_to_world=None, ranges=((0.0, 3.4028234
         a = np.empty(n, dtype=dt)
         res[name] = a
         setattr(out, name, _p(a, _CTYPE[dt]))
+    # per beam: [min, max] SECOND distance and [min, max] SECOND_STRONGEST intensity over every resolution of the
+    # reduction's near-tied comparisons (+-inf where unbounded)
+    res["second_range"] = np.empty((n, 4), np.float32)
+    out.secondRange = _p(res["second_range"], C.c_float)
     if world is not None:
         status = lib.orc_world_raytrace(world.handle, C.byref(q), C.byref(out))
     else:

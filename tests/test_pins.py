@@ -149,6 +149,41 @@ def test_beam_reduction_closed_forms_are_distinct():
     assert e["tilted_back"][5] < e["tilted_back"][1] * (1 - 1e-4)
 
 
+def second_ranges(oracle, name, mode):
+    import robotecgpulidar_amd.rgl as R
+    n = plane_normal(name)
+    case = Case(plane_scene(n), S.f32(S.identity()[None]), divergence=(2 * HALF_H, 2 * HALF_V),
+                return_mode=getattr(R, "RGL_RETURN_" + mode), default_intensity=50.0)
+    return run_oracle(oracle, case, fields=[DIST, INTEN, IS_HIT]), expected_returns(n)
+
+
+@pytest.mark.parametrize("name", list(NORMALS))
+def test_tie_resolution_ranges_hold_the_closed_forms(oracle, name):
+    """The oracle's range of SECOND / SECOND_STRONGEST over every resolution of near-tied comparisons (used by the
+    GPU multi-return checks) contains the closed-form values on the planes, and so does the returned value."""
+    for mode, t, col in (("FIRST_SECOND", 3, 0), ("STRONGEST_SECOND_STRONGEST", 5, 2)):
+        r, exp = second_ranges(oracle, name, mode)
+        lo, hi = float(r["second_range"][0, col]), float(r["second_range"][0, col + 1])
+        got = float(r[DIST][1] if t == 3 else r[INTEN][1])
+        assert lo - 1e-5 * abs(lo) <= exp[t] <= hi + 1e-5 * abs(hi), (name, t, lo, exp[t], hi)
+        assert lo <= got <= hi, (name, t, lo, got, hi)
+
+
+def test_tie_resolution_ranges_catch_planted_mistake(oracle, planted):
+    """The ranges come from a tracker model of their own: with the SECOND trackers seeded wrongly (mistake 1),
+    the returned values leave the ranges (on the perpendicular plane, where SECOND stays on the nearest, primary
+    sample)."""
+    planted(1)
+    outside = 0
+    for name in NORMALS:
+        for mode, t, col in (("FIRST_SECOND", 3, 0), ("STRONGEST_SECOND_STRONGEST", 5, 2)):
+            r, _ = second_ranges(oracle, name, mode)
+            lo, hi = float(r["second_range"][0, col]), float(r["second_range"][0, col + 1])
+            got = float(r[DIST][1] if t == 3 else r[INTEN][1])
+            outside += not (lo <= got <= hi)
+    assert outside > 0
+
+
 @pytest.mark.parametrize("mistake", [1, 2, 6])
 def test_beam_reduction_catches_planted_mistakes(oracle, planted, mistake):
     planted(mistake)
