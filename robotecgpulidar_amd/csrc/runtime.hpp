@@ -186,11 +186,15 @@ public:
 	{
 		RGL_HIP(hipEventRecord(e, s));
 		recorded = true;
+		reached = false;
 	}
+	// Waits for the last recorded work; once it has been seen complete, later waits return at once (result queries
+	// wait on the same events many times per frame).
 	void sync()
 	{
-		if (recorded) {
+		if (recorded && !reached) {
 			RGL_HIP(hipEventSynchronize(e));
+			reached = true;
 		}
 	}
 	bool wasRecorded() const { return recorded; }
@@ -198,6 +202,7 @@ public:
 private:
 	hipEvent_t e = nullptr;
 	bool recorded = false;
+	bool reached = false;
 };
 
 // ------------------------------------------------------------------------------------------------ fields

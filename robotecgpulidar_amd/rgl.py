@@ -251,13 +251,23 @@ def last_error() -> str:
     return (s.value or b"").decode()
 
 
+_FNS: dict = {}
+
+
+def _fn(name: str):
+    fn = _FNS.get(name)
+    if fn is None:
+        fn = _FNS[name] = getattr(load(), name)
+    return fn
+
+
 def call(name: str, *args) -> int:
     """Calls an API function and returns its status (no exception)."""
-    return getattr(load(), name)(*args)
+    return _fn(name)(*args)
 
 
 def check(name: str, *args) -> None:
-    st = call(name, *args)
+    st = _fn(name)(*args)
     if st != RGL_SUCCESS:
         raise RGLError(name, st, last_error())
 

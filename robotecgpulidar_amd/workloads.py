@@ -32,10 +32,14 @@ class SensorGraph:
     nodes: list            # every node of the chain, in order
 
 
+_COUNT = C.c_int32()
+_COUNT_REF = C.byref(_COUNT)
+
+
 def kept_count(out) -> int:
-    count = C.c_int32()
-    rgl.check("rgl_graph_get_result_size", out.h, rgl.XYZ_VEC3_F32, C.byref(count), None)
-    return count.value
+    """Points the output node holds after the last run (waits for the run's counts, not for its data)."""
+    rgl.check("rgl_graph_get_result_size", out.h, rgl.XYZ_VEC3_F32, _COUNT_REF, None)
+    return _COUNT.value
 
 
 def sensor_graph(lidar: S.Lidar, scene, seed: int) -> SensorGraph:
