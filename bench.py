@@ -199,10 +199,12 @@ def run_batch64(args):
     outs = [g.out for g in graphs]
     rays_per_frame = sum(len(x.lidar.rays) for x in sensors)
 
+    counts = rgl.ResultSizes(outs, rgl.XYZ_VEC3_F32)  # every sensor's kept count in one call, in both modes
+
     def plain_frame():
         for e in entries:
             rgl.check("rgl_graph_run", e.h)
-        return sum(kept_count(o) for o in outs)
+        return int(counts().sum())
 
     submit_s = []
 
@@ -210,7 +212,7 @@ def run_batch64(args):
         t0 = time.perf_counter()
         rgl.graph_run_batch(entries)
         submit_s.append(time.perf_counter() - t0)
-        return sum(kept_count(o) for o in outs)
+        return int(counts().sum())
 
     def timed(fn, steps):
         lat = []
@@ -665,7 +667,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=8, help="untimed frames (the first six also settle the autotuner)")
     ap.add_argument("--workload", choices=["batch64", "urban", "serve", "multireturn", "radar"], default="batch64")
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the serving-loop measurement")
-    ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q"], default="bvh4",
+    ap.add_argument("--bvh-format", choices=["bvh2", "bvh4", "bvh4q", "bvh4h"], default="bvh4",
                     help="node format: binary FP32, 4-wide FP32, or 4-wide 8-bit quantized")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU-port frames for cpu_baseline (0 = skip)")
@@ -688,7 +690,7 @@ def main():
         sys.exit(2)
     rgl.load()
     rgl.check("rgl_configure_logging", 6, None, False)
-    fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8}[args.bvh_format]
+    fmt = {"bvh2": rgl.BVH2, "bvh4": rgl.BVH4_F32, "bvh4q": rgl.BVH4_Q8, "bvh4h": rgl.BVH4_Q16}[args.bvh_format]
     rgl.set_bvh_format(fmt)
     if args.workload in ("serve", "multireturn", "radar"):
         run = {"serve": run_serve, "multireturn": run_multireturn, "radar": run_radar}[args.workload]
@@ -700,7 +702,8 @@ def main():
         rgl.cleanup()
         result["serving"] = run_serve(args)
     result["config"]["bvh_format"] = {rgl.BVH2: "binary FP32 (64 B nodes)", rgl.BVH4_F32: "4-wide FP32 (128 B nodes)",
-                                      rgl.BVH4_Q8: "4-wide 8-bit quantized (64 B nodes)"}[fmt]
+                                      rgl.BVH4_Q8: "4-wide 8-bit quantized (64 B nodes)",
+                                      rgl.BVH4_Q16: "4-wide 16-bit quantized (80 B nodes)"}[fmt]
     rgl.cleanup()
     print(json.dumps(result))
 

@@ -29,8 +29,8 @@ RGL_API rgl_status_t rgl_scene_set_tlas_refit(rgl_scene_t scene, bool refit);
 
 // BVH node format used by traversal in every scene: 0 = binary with FP32 child boxes (64 B nodes), 1 = 4-wide with
 // FP32 child boxes (128 B), 2 = 4-wide with child boxes quantized to 8 bits relative to the node box and rounded
-// outward (64 B). All formats index the same leaves, so results are identical; they differ in speed only. Scenes are
-// rebuilt on their next use.
+// outward (64 B), 3 = the same with 16-bit bounds (80 B). All formats index the same leaves, so results are
+// identical; they differ in speed only. Scenes are rebuilt on their next use.
 RGL_API rgl_status_t rgl_configure_bvh_format(int32_t format);
 
 // Reports the BVH node format in use.
@@ -71,6 +71,12 @@ RGL_API rgl_status_t rgl_scene_destroy(rgl_scene_t scene);
 // identical to those of rgl_graph_run on the same graph and state; results are read with
 // rgl_graph_get_result_size / rgl_graph_get_result_data as usual. Graphs may raytrace different scenes.
 RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* nodes, int32_t node_count);
+
+// Point counts of `field` at nodes[0..node_count) (rgl_graph_get_result_size of each node, in one call; sizes[i]
+// receives the count of nodes[i]). A batch's counts arrive on the host together, so reading them for many sensors
+// costs one wait and one call instead of one per node; the first failing node fails the call.
+RGL_API rgl_status_t rgl_graph_get_result_sizes(const rgl_node_t* nodes, int32_t node_count, rgl_field_t field,
+                                                int32_t* out_counts);
 
 // Autotuning of rgl_graph_run_batch's launch shapes (on by default): while untuned, successive batches alternate
 // between the raytrace block sizes (128 or 64 threads) and the compaction tiles (8 or 4 points per thread), each

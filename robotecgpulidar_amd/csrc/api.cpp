@@ -1121,6 +1121,23 @@ RGL_API rgl_status_t rgl_graph_run_batch(const rgl_node_t* raw_nodes, int32_t no
 	});
 }
 
+RGL_API rgl_status_t rgl_graph_get_result_sizes(const rgl_node_t* raw_nodes, int32_t node_count, rgl_field_t field,
+                                                int32_t* out_counts)
+{
+	return safeCall([&]() {
+		CHECK_ARG(node_count >= 0);
+		CHECK_ARG((raw_nodes != nullptr && out_counts != nullptr) || node_count == 0);
+		for (int32_t i = 0; i < node_count; ++i) {
+			CHECK_ARG(raw_nodes[i] != nullptr);
+			auto pn = nodeAs<IPointsNode>(raw_nodes[i], "IPointsNode");
+			pn->waitForCounts();
+			checkOverflow(pn);
+			(void) pn->fieldPointSize(field); // an unknown field fails like rgl_graph_get_result_size
+			out_counts[i] = pn->hostCount();
+		}
+	});
+}
+
 RGL_API rgl_status_t rgl_graph_batch_set_kernel_timing(bool enable)
 {
 	return safeCall([&]() { GraphBatch::instance().setTiming(enable); });

@@ -565,23 +565,69 @@ __global__ void kScanAdd(int n, int32_t* out, const int32_t* blockSums)
 
 __device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
 
-// Smallest exponent e (scale 2^e) with fl(lo + 255 * 2^e) >= hi.
-__device__ int quantExponent(float lo, float hi)
+// Smallest exponent e (scale 2^e) with lo + levels * 2^e >= hi in exact arithmetic (lo is an FP32 value; the sum is
+// evaluated in FP64, where it is exact for the exponents that matter here).
+__device__ int quantExponent(float lo, double hi, double levels)
 {
 	if (!(hi > lo)) {
 		return -126;
 	}
-	int e = max(-126, min(127, ilogbf((hi - lo) / 255.0f)));
-	while (e < 127 && __fadd_rn(lo, 255.0f * pow2f(e)) < hi) {
+	int e = max(-126, min(127, ilogb((hi - lo) / levels)));
+	while (e < 127 && static_cast<double>(lo) + levels * pow2f(e) < hi) {
 		++e;
 	}
-	while (e > -126 && __fadd_rn(lo, 255.0f * pow2f(e - 1)) >= hi) {
+	while (e > -126 && static_cast<double>(lo) + levels * pow2f(e - 1) >= hi) {
 		--e;
 	}
 	return e;
 }
 
-template<bool kQuant>
+// Quantized child bounds of one 4-wide node (Bvh4QNode with Q = uint8_t, Bvh4HNode with Q = uint16_t). Each decoded
+// bound lies outside the child's FP32 bound by at least a margin of 2^-20 of the node's extent on that axis (checked
+// in exact arithmetic), which covers the rounding of the traversal's folded decode (levels * 2^e < 2 * extent).
+template<typename Node, typename Q>
+__device__ void quantizeChildren(Node& o, const float (&clo)[4][3], const float (&chi)[4][3], int cnt)
+{
+	constexpr int kLevels = sizeof(Q) == 1 ? 255 : 65535;
+	for (int a = 0; a < 3; ++a) {
+		float lo0 = FLT_MAX, hi0 = -FLT_MAX;
+		for (int c = 0; c < cnt; ++c) {
+			lo0 = fminf(lo0, clo[c][a]);
+			hi0 = fmaxf(hi0, chi[c][a]);
+		}
+		const double margin = ldexp(static_cast<double>(hi0) - lo0, -20);
+		float lo = static_cast<float>(static_cast<double>(lo0) - margin);
+		if (static_cast<double>(lo) > static_cast<double>(lo0) - margin) {
+			lo = nextafterf(lo, -INFINITY);
+		}
+		const int e = quantExponent(lo, static_cast<double>(hi0) + margin, kLevels);
+		const double sc = pow2f(e);
+		o.origin[a] = lo;
+		o.exp[a] = static_cast<uint8_t>(e + 127);
+		for (int c = 0; c < 4; ++c) {
+			if (c >= cnt) {
+				o.qlo[a][c] = static_cast<Q>(kLevels);
+				o.qhi[a][c] = 0;
+				continue;
+			}
+			const double want = static_cast<double>(clo[c][a]) - margin, wantHi = static_cast<double>(chi[c][a]) + margin;
+			int ql = static_cast<int>(floor((want - lo) / sc));
+			ql = max(0, min(kLevels, ql));
+			while (ql > 0 && static_cast<double>(lo) + ql * sc > want) {
+				--ql;
+			}
+			int qh = static_cast<int>(ceil((wantHi - lo) / sc));
+			qh = max(0, min(kLevels, qh));
+			while (qh < kLevels && static_cast<double>(lo) + qh * sc < wantHi) {
+				++qh;
+			}
+			o.qlo[a][c] = static_cast<Q>(ql);
+			o.qhi[a][c] = static_cast<Q>(qh);
+		}
+	}
+}
+
+template<int kFmt>
 __global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags, const int32_t* rank, void* out)
 {
 	const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -616,7 +662,7 @@ __global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags
 		}
 	}
 	const int slot = rank[k];
-	if (!kQuant) {
+	if constexpr (kFmt == kBvh4F) {
 		Bvh4FNode o;
 		for (int c = 0; c < 4; ++c) {
 			for (int a = 0; a < 3; ++a) {
@@ -630,44 +676,24 @@ __global__ void kCollapse4(int inner, const BvhNode* nodes, const int32_t* flags
 		static_cast<Bvh4FNode*>(out)[slot] = o;
 		return;
 	}
-	Bvh4QNode o;
-	for (int a = 0; a < 3; ++a) {
-		float lo = FLT_MAX, hi = -FLT_MAX;
-		for (int c = 0; c < cnt; ++c) {
-			lo = fminf(lo, clo[c][a]);
-			hi = fmaxf(hi, chi[c][a]);
-		}
-		const int e = quantExponent(lo, hi);
-		const float sc = pow2f(e);
-		o.origin[a] = lo;
-		o.exp[a] = static_cast<uint8_t>(e + 127);
+	auto finish = [&](auto& o) {
 		for (int c = 0; c < 4; ++c) {
-			if (c >= cnt) {
-				o.qlo[a][c] = 255;
-				o.qhi[a][c] = 0;
-				continue;
-			}
-			// Lower bound rounded down, upper bound rounded up, both checked in the decoder's FP32 arithmetic.
-			int ql = static_cast<int>(floor((static_cast<double>(clo[c][a]) - lo) / sc));
-			ql = max(0, min(255, ql));
-			while (ql > 0 && __fadd_rn(lo, static_cast<float>(ql) * sc) > clo[c][a]) {
-				--ql;
-			}
-			int qh = static_cast<int>(ceil((static_cast<double>(chi[c][a]) - lo) / sc));
-			qh = max(0, min(255, qh));
-			while (qh < 255 && __fadd_rn(lo, static_cast<float>(qh) * sc) < chi[c][a]) {
-				++qh;
-			}
-			o.qlo[a][c] = static_cast<uint8_t>(ql);
-			o.qhi[a][c] = static_cast<uint8_t>(qh);
+			o.child[c] = c < cnt ? ref[c] : ref[0];
 		}
+		o.count = static_cast<uint8_t>(cnt);
+	};
+	if constexpr (kFmt == kBvh4Q) {
+		Bvh4QNode o;
+		quantizeChildren<Bvh4QNode, uint8_t>(o, clo, chi, cnt);
+		finish(o);
+		o.pad[0] = o.pad[1] = 0;
+		static_cast<Bvh4QNode*>(out)[slot] = o;
+	} else {
+		Bvh4HNode o;
+		quantizeChildren<Bvh4HNode, uint16_t>(o, clo, chi, cnt);
+		finish(o);
+		static_cast<Bvh4HNode*>(out)[slot] = o;
 	}
-	for (int c = 0; c < 4; ++c) {
-		o.child[c] = c < cnt ? ref[c] : ref[0];
-	}
-	o.count = static_cast<uint8_t>(cnt);
-	o.pad[0] = o.pad[1] = 0;
-	static_cast<Bvh4QNode*>(out)[slot] = o;
 }
 
 // ---------------------------------------------------------------------------------------- PLOC topology
@@ -995,12 +1021,11 @@ void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* s
 	const int inner = std::max(n - 1, 1);
 	const int32_t* flags = static_cast<const int32_t*>(scratch);
 	const int32_t* rank = flags + inner;
-	if (format == kBvh4Q) {
-		hipLaunchKernelGGL(kCollapse4<true>, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, nodes, flags, rank,
-		                   out);
-	} else {
-		hipLaunchKernelGGL(kCollapse4<false>, dim3(blocksFor(inner)), dim3(kBlock), 0, stream, inner, nodes, flags, rank,
-		                   out);
+	const dim3 grid(blocksFor(inner)), block(kBlock);
+	switch (format) {
+		case kBvh4Q: hipLaunchKernelGGL(kCollapse4<kBvh4Q>, grid, block, 0, stream, inner, nodes, flags, rank, out); break;
+		case kBvh4H: hipLaunchKernelGGL(kCollapse4<kBvh4H>, grid, block, 0, stream, inner, nodes, flags, rank, out); break;
+		default: hipLaunchKernelGGL(kCollapse4<kBvh4F>, grid, block, 0, stream, inner, nodes, flags, rank, out); break;
 	}
 }
 

@@ -31,8 +31,9 @@ enum BvhFormat : int32_t
 	kBvh2 = 0,   // BvhNode: binary, FP32 child boxes, 64 B
 	kBvh4F = 1,  // Bvh4FNode: 4-wide, FP32 child boxes, 128 B
 	kBvh4Q = 2,  // Bvh4QNode: 4-wide, child boxes quantized to 8 bits relative to the node box, 64 B
+	kBvh4H = 3,  // Bvh4HNode: 4-wide, child boxes quantized to 16 bits relative to the node box, 80 B
 };
-constexpr int kBvhFormats = 3;
+constexpr int kBvhFormats = 4;
 
 // 4-wide node with FP32 child boxes stored per axis ([axis][child]) so the four slab tests load contiguous lanes.
 // Children occupy slots [0, count); child references follow the BvhNode convention. An empty slot holds the point
@@ -47,10 +48,11 @@ struct alignas(16) Bvh4FNode
 };
 static_assert(sizeof(Bvh4FNode) == 128, "BVH4 FP32 node must be 128 B");
 
-// 4-wide node with quantized child boxes. A child bound on axis a decodes, in FP32, as
-// origin[a] + q * 2^(exp[a] - 127), one exact product and one rounded sum; the builder picks q so that the decoded
-// box contains the child's FP32 box (lower bounds rounded down, upper bounds rounded up, each checked in the same
-// FP32 arithmetic), so traversal with decoded boxes culls nothing the exact boxes would keep.
+// 4-wide node with quantized child boxes. A child bound on axis a stands for the plane origin[a] + q * 2^(exp[a] - 127)
+// (exact real value); the builder picks q so that this plane lies outside the child's FP32 bound by a margin of 2^-20
+// of the node's extent on that axis (lower bounds rounded down, upper bounds rounded up, checked in exact arithmetic).
+// Traversal evaluates the plane distances with the decode folded into its slab constants, whose extra rounding the
+// margin covers, so traversal with quantized boxes culls nothing the exact boxes would keep.
 struct alignas(16) Bvh4QNode
 {
 	float origin[3];
@@ -62,6 +64,18 @@ struct alignas(16) Bvh4QNode
 	int32_t pad[2];
 };
 static_assert(sizeof(Bvh4QNode) == 64, "BVH4 quantized node must be 64 B");
+
+// The same with 16-bit bounds (q in [0, 65535]): boxes within 2^-16 of the node extent of the exact ones.
+struct alignas(16) Bvh4HNode
+{
+	float origin[3];
+	uint8_t exp[3];
+	uint8_t count;
+	uint16_t qlo[3][4]; // [axis][child]
+	uint16_t qhi[3][4];
+	int32_t child[4];
+};
+static_assert(sizeof(Bvh4HNode) == 80, "BVH4 16-bit node must be 80 B");
 
 // Leaf triangle in BVH leaf order, 48 B: object-space vertices A, B, C. w of the first vec4 holds the
 // triangle's index in the mesh index buffer (bit pattern of an int32), which the shading stage uses for UVs

@@ -32,7 +32,7 @@ void refitLbvh(hipStream_t stream, int n, const Aabb* boxes, const int32_t* leaf
                BvhNode* nodes, int32_t* counters, int32_t* height);
 // 4-wide nodes from a binary LBVH over n leaves (built by buildLbvh). bvh4Ranks marks the kept (even-depth) internal
 // nodes in `scratch` (bvh4ScratchBytes(n)) and writes their number to *countDev; bvh4Collapse then writes that many
-// nodes of `format` (kBvh4F or kBvh4Q) to `out`. Leaf references keep the binary tree's convention.
+// nodes of `format` (kBvh4F, kBvh4Q or kBvh4H) to `out`. Leaf references keep the binary tree's convention.
 size_t bvh4ScratchBytes(int n);
 void bvh4Ranks(hipStream_t stream, int n, const int32_t* parentSlot, void* scratch, int32_t* countDev);
 void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* scratch, int format, void* out);

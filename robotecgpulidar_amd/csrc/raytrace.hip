@@ -309,6 +309,16 @@ __device__ __forceinline__ void slab6(const SlabRay& r, float lx, float ly, floa
 	tNear = tn;
 }
 
+// The same test from the six plane distances (lower and upper plane per axis).
+__device__ __forceinline__ void slabT(float tx0, float tx1, float ty0, float ty1, float tz0, float tz1, float tBound,
+                                      bool& hit, float& tNear)
+{
+	const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+	const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+	hit = fmaxf(tn, 0.0f) <= fminf(tf, tBound);
+	tNear = tn;
+}
+
 __device__ __forceinline__ void slab(const SlabRay& r, const float* f, float tBound, bool& hit, float& tNear)
 {
 	slab6(r, f[0], f[1], f[2], f[3], f[4], f[5], tBound, hit, tNear);
@@ -473,7 +483,10 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 			st->nodes++;
 		}
 		if constexpr (kFoot) {
-			const int nodeBytes = kFmt == kBvh4F ? sizeof(Bvh4FNode) : (kFmt == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+			const int nodeBytes = kFmt == kBvh4F   ? sizeof(Bvh4FNode)
+			                      : kFmt == kBvh4Q ? sizeof(Bvh4QNode)
+			                      : kFmt == kBvh4H ? sizeof(Bvh4HNode)
+			                                       : sizeof(BvhNode);
 			touchLines(ls, static_cast<const char*>(nodes) + static_cast<size_t>(node) * nodeBytes, nodeBytes);
 		}
 		int next;
@@ -519,33 +532,71 @@ __device__ bool traceClosest(const SceneView& sc, Vec3f o, Vec3f d, float tmax, 
 				r2 = ch.z;
 				r3 = ch.w;
 			} else {
-				RglF4 q[4];
-				loadQuads<4>(static_cast<const Bvh4QNode*>(nodes) + node, q);
+				// Quantized bounds (8-bit Bvh4QNode, 16-bit Bvh4HNode): the node origin, one power-of-two scale per axis
+				// (exp byte = e + 127, so byte << 23 is the bit pattern of 2^e), the child count, then per axis and child
+				// a lower and an upper level q.
+				constexpr bool kWide16 = kFmt == kBvh4H;
+				constexpr int kQ = kWide16 ? 5 : 4;
+				RglF4 q[kQ];
+				loadQuads<kQ>(static_cast<const char*>(nodes) + static_cast<size_t>(node) * (kWide16 ? sizeof(Bvh4HNode) : sizeof(Bvh4QNode)), q);
 				const RglU4 u0 = __builtin_bit_cast(RglU4, q[0]), u1 = __builtin_bit_cast(RglU4, q[1]);
 				const RglU4 u2 = __builtin_bit_cast(RglU4, q[2]), u3 = __builtin_bit_cast(RglU4, q[3]);
 				const float ox = __uint_as_float(u0.x), oy = __uint_as_float(u0.y), oz = __uint_as_float(u0.z);
-				// exp byte = e + 127, so (byte << 23) is the bit pattern of 2^e.
 				const float sx = __uint_as_float((u0.w & 0xffu) << 23);
 				const float sy = __uint_as_float(((u0.w >> 8) & 0xffu) << 23);
 				const float sz = __uint_as_float(((u0.w >> 16) & 0xffu) << 23);
 				const int cnt = static_cast<int>(u0.w >> 24);
+				// Dequantization folded into the slab constants: the plane origin + q * 2^e at distance
+				// fma(plane, m, -c) equals q * (2^e m) + (origin m - c), so per node and axis A = 2^e m (exact: a
+				// power-of-two scaling) and B = fma(origin, m, -c) are formed once and each bound costs one conversion
+				// and one fma. Against the plane's distance computed directly this adds an absolute error of at most
+				// u |B| (u = 2^-24), and |B| <= |t| + L * 2^e |m| (L = 255 or 65535 levels): the |t| part stays inside
+				// the widening of the plain test, and the rest is a shift of the plane by at most u * L * 2^e in space,
+				// which the builder's outward margin of 2^-20 of the node extent (with L * 2^e < 2 * extent) exceeds.
+				const float aLx = sx * sr.mlo.x, aHx = sx * sr.mhi.x, bLx = fmaf(ox, sr.mlo.x, -sr.clo.x);
+				const float bHx = fmaf(ox, sr.mhi.x, -sr.chi.x);
+				const float aLy = sy * sr.mlo.y, aHy = sy * sr.mhi.y, bLy = fmaf(oy, sr.mlo.y, -sr.clo.y);
+				const float bHy = fmaf(oy, sr.mhi.y, -sr.chi.y);
+				const float aLz = sz * sr.mlo.z, aHz = sz * sr.mhi.z, bLz = fmaf(oz, sr.mlo.z, -sr.clo.z);
+				const float bHz = fmaf(oz, sr.mhi.z, -sr.chi.z);
+				if constexpr (kWide16) {
+					// Words: u1 = lo x (children 0-1, 2-3), lo y; u2 = lo z, hi x; u3 = hi y, hi z.
+					auto lv = [](uint32_t w0, uint32_t w1, int c) {
+						const uint32_t w = c < 2 ? w0 : w1;
+						return static_cast<float>((c & 1) ? (w >> 16) : (w & 0xffffu));
+					};
+#define RGL_HBOX(c, h, k)                                                                                              \
+	slabT(fmaf(lv(u1.x, u1.y, c), aLx, bLx), fmaf(lv(u2.z, u2.w, c), aHx, bHx), fmaf(lv(u1.z, u1.w, c), aLy, bLy),       \
+	      fmaf(lv(u3.x, u3.y, c), aHy, bHy), fmaf(lv(u2.x, u2.y, c), aLz, bLz), fmaf(lv(u3.z, u3.w, c), aHz, bHz), tBound, \
+	      h, k)
+					RGL_HBOX(0, h0, k0);
+					RGL_HBOX(1, h1, k1);
+					RGL_HBOX(2, h2, k2);
+					RGL_HBOX(3, h3, k3);
+#undef RGL_HBOX
+					const RglI4 ch = __builtin_bit_cast(RglI4, q[kQ - 1]);
+					r0 = ch.x;
+					r1 = ch.y;
+					r2 = ch.z;
+					r3 = ch.w;
+				} else {
 #define RGL_QBOX(c, h, k)                                                                                              \
-	slab6(sr, __fadd_rn(ox, ubyte(u1.x, c) * sx), __fadd_rn(oy, ubyte(u1.y, c) * sy),                             \
-	      __fadd_rn(oz, ubyte(u1.z, c) * sz), __fadd_rn(ox, ubyte(u1.w, c) * sx), __fadd_rn(oy, ubyte(u2.x, c) * sy),  \
-	      __fadd_rn(oz, ubyte(u2.y, c) * sz), tBound, h, k)
-				RGL_QBOX(0, h0, k0);
-				RGL_QBOX(1, h1, k1);
-				RGL_QBOX(2, h2, k2);
-				RGL_QBOX(3, h3, k3);
+	slabT(fmaf(ubyte(u1.x, c), aLx, bLx), fmaf(ubyte(u1.w, c), aHx, bHx), fmaf(ubyte(u1.y, c), aLy, bLy),               \
+	      fmaf(ubyte(u2.x, c), aHy, bHy), fmaf(ubyte(u1.z, c), aLz, bLz), fmaf(ubyte(u2.y, c), aHz, bHz), tBound, h, k)
+					RGL_QBOX(0, h0, k0);
+					RGL_QBOX(1, h1, k1);
+					RGL_QBOX(2, h2, k2);
+					RGL_QBOX(3, h3, k3);
 #undef RGL_QBOX
+					r0 = static_cast<int>(u2.z);
+					r1 = static_cast<int>(u2.w);
+					r2 = static_cast<int>(u3.x);
+					r3 = static_cast<int>(u3.y);
+				}
 				// Empty slots decode to inverted boxes, which the slab test does not reject.
 				h1 = h1 && cnt > 1;
 				h2 = h2 && cnt > 2;
 				h3 = h3 && cnt > 3;
-				r0 = static_cast<int>(u2.z);
-				r1 = static_cast<int>(u2.w);
-				r2 = static_cast<int>(u3.x);
-				r3 = static_cast<int>(u3.y);
 			}
 			const float kMiss = INFINITY;
 			k0 = h0 ? k0 : kMiss;
@@ -1312,6 +1363,7 @@ void raytrace(hipStream_t stream, const SceneView& scene, const RaytraceParams& 
 	switch (scene.format) {
 		case kBvh4F: launchRaytrace<kBvh4F>(stream, blocks, multi, scene, params, out); break;
 		case kBvh4Q: launchRaytrace<kBvh4Q>(stream, blocks, multi, scene, params, out); break;
+		case kBvh4H: launchRaytrace<kBvh4H>(stream, blocks, multi, scene, params, out); break;
 		default: launchRaytrace<kBvh2>(stream, blocks, multi, scene, params, out); break;
 	}
 }
@@ -1331,6 +1383,7 @@ void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemC
 	switch (format) {
 		case kBvh4F: launchRaytraceBatch<kBvh4F>(stream, totalBlocks, multi, variant, items, itemCount); break;
 		case kBvh4Q: launchRaytraceBatch<kBvh4Q>(stream, totalBlocks, multi, variant, items, itemCount); break;
+		case kBvh4H: launchRaytraceBatch<kBvh4H>(stream, totalBlocks, multi, variant, items, itemCount); break;
 		default: launchRaytraceBatch<kBvh2>(stream, totalBlocks, multi, variant, items, itemCount); break;
 	}
 }
@@ -1357,6 +1410,7 @@ void traversalFootprint(hipStream_t stream, const SceneView& scene, const Raytra
 	switch (scene.format) {
 		case kBvh4F: hipLaunchKernelGGL(kTraversalFootprint<kBvh4F>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
 		case kBvh4Q: hipLaunchKernelGGL(kTraversalFootprint<kBvh4Q>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
+		case kBvh4H: hipLaunchKernelGGL(kTraversalFootprint<kBvh4H>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
 		default: hipLaunchKernelGGL(kTraversalFootprint<kBvh2>, dim3(blocks), dim3(kRtBlock), 0, stream, scene, params, ls); break;
 	}
 }

@@ -76,7 +76,12 @@ void Mesh::releaseAccel() { blas.release(); }
 namespace {
 size_t nodeBytes(int format)
 {
-	return format == kBvh4F ? sizeof(Bvh4FNode) : (format == kBvh4Q ? sizeof(Bvh4QNode) : sizeof(BvhNode));
+	switch (format) {
+		case kBvh4F: return sizeof(Bvh4FNode);
+		case kBvh4Q: return sizeof(Bvh4QNode);
+		case kBvh4H: return sizeof(Bvh4HNode);
+		default: return sizeof(BvhNode);
+	}
 }
 } // namespace
 
@@ -85,7 +90,7 @@ int64_t estimatedBlasBytes(int64_t t, int format)
 	// Binary nodes 64 B, leaf triangles 48 B, leaf order 4 B, parent links 8 B, fit counters and heights 8 B,
 	// primitive boxes 24 B per triangle; at most one 4-wide node per binary internal node (an upper bound: about half
 	// of them are kept in practice).
-	const int64_t wideBytes = format == kBvh4F ? 128 * t : (format == kBvh4Q ? 64 * t : 0);
+	const int64_t wideBytes = format == kBvh2 ? 0 : static_cast<int64_t>(nodeBytes(format)) * t;
 	return t * (64 + 48 + 4 + 8 + 8 + 24) + wideBytes;
 }
 
@@ -136,7 +141,7 @@ const void* Blas::nodesFor(hipStream_t s, int format)
 	if (format == kBvh2) {
 		return nodes.get();
 	}
-	if (format != kBvh4F && format != kBvh4Q) {
+	if (format != kBvh4F && format != kBvh4Q && format != kBvh4H) {
 		throw std::invalid_argument("unknown BVH node format " + std::to_string(format));
 	}
 	if (!wideBuilt[format]) {
@@ -353,7 +358,7 @@ int Scene::bvhFormat() { return formatSetting(); }
 
 void Scene::setBvhFormat(int format)
 {
-	if (format != kBvh2 && format != kBvh4F && format != kBvh4Q) {
+	if (format < 0 || format >= kBvhFormats) {
 		throw std::invalid_argument("unknown BVH node format " + std::to_string(format));
 	}
 	if (format != formatSetting()) {

@@ -186,6 +186,7 @@ _PROTOTYPES = [
     ("rgl_scene_create", [P(C.c_void_p)]),
     ("rgl_scene_destroy", [C.c_void_p]),
     ("rgl_graph_run_batch", [P(c_node), C.c_int32]),
+    ("rgl_graph_get_result_sizes", [P(c_node), C.c_int32, C.c_int32, i32p]),
     ("rgl_graph_batch_set_kernel_timing", [C.c_bool]),
     ("rgl_graph_batch_get_kernel_timing", [P(C.c_double), i32p]),
     ("rgl_graph_batch_configure_autotune", [C.c_bool]),
@@ -211,7 +212,7 @@ _PROTOTYPES = [
     ("rgl_server_get_stats", [C.c_void_p, C.c_void_p]),
     ("rgl_server_reset_stats", [C.c_void_p]),
 ]
-BVH2, BVH4_F32, BVH4_Q8 = 0, 1, 2
+BVH2, BVH4_F32, BVH4_Q8, BVH4_Q16 = 0, 1, 2, 3
 VOID_FUNCS = {"rgl_get_last_error_string"}
 
 _lib = None
@@ -490,6 +491,21 @@ def graph_run_batch(nodes) -> None:
     """Runs the graphs of `nodes` (one node per graph) as one batch."""
     arr = (c_node * max(len(nodes), 1))(*[n.h.value for n in nodes])
     check("rgl_graph_run_batch", arr, len(nodes))
+
+
+class ResultSizes:
+    """Point counts of one field at a fixed list of nodes, read with one rgl_graph_get_result_sizes call."""
+
+    def __init__(self, nodes, field: int):
+        self.n = len(nodes)
+        self.arr = (c_node * max(self.n, 1))(*[x.h.value for x in nodes])
+        self.out = np.zeros(max(self.n, 1), np.int32)
+        self.ptr = self.out.ctypes.data_as(i32p)
+        self.field = int(field)
+
+    def __call__(self) -> np.ndarray:
+        check("rgl_graph_get_result_sizes", self.arr, self.n, self.field, self.ptr)
+        return self.out[: self.n]
 
 
 def batch_kernel_timing(enable: bool | None = None):

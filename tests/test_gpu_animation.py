@@ -110,7 +110,7 @@ def jitter(v, seed, amp):
     return (v + rng.uniform(-amp, amp, v.shape)).astype(np.float32)
 
 
-@pytest.mark.parametrize("fmt", [R.BVH2, R.BVH4_F32, R.BVH4_Q8])
+@pytest.mark.parametrize("fmt", [R.BVH2, R.BVH4_F32, R.BVH4_Q8, R.BVH4_Q16])
 def test_animated_scene_matches_oracle_with_velocities(rgl_lib, oracle, fmt):
     rgl = rgl_lib
     default_fmt = rgl.get_bvh_format()
@@ -180,7 +180,7 @@ def test_animated_scene_matches_oracle_with_velocities(rgl_lib, oracle, fmt):
         rgl.set_bvh_format(default_fmt)
 
 
-@pytest.mark.parametrize("fmt", [R.BVH2, R.BVH4_F32, R.BVH4_Q8])
+@pytest.mark.parametrize("fmt", [R.BVH2, R.BVH4_F32, R.BVH4_Q8, R.BVH4_Q16])
 def test_refitted_animation_equals_fresh_mesh(rgl_lib, fmt):
     """The animated entity's BLAS is refitted (the tree of the first vertices with boxes of the current ones), a
     fresh mesh is built from scratch: closest hits do not depend on the tree, so results are bit-identical."""

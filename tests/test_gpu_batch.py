@@ -197,3 +197,28 @@ def test_batch_kernel_timing(rgl_lib):
     ms, launches = rgl.batch_kernel_timing()
     rgl.batch_kernel_timing(False)
     assert launches == 3 and ms > 0.0
+
+
+def test_result_sizes_in_one_call(rgl_lib):
+    """rgl_graph_get_result_sizes gives every node's count as rgl_graph_get_result_size does, after batched and solo
+    runs, and fails like it on a node that holds no points."""
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    graphs = make_set(rgl, scenes)
+    outs = [g.out for g in graphs]
+    sizes = rgl.ResultSizes(outs, XYZ)
+    for frame in range(2):
+        if frame == 0:
+            rgl.graph_run_batch([g.rays for g in graphs])
+        else:
+            for g in graphs:
+                rgl.graph_run(g.rays)
+        one_by_one = [rgl.result_size(n, XYZ)[0] for n in outs]
+        assert sizes().tolist() == one_by_one
+        assert sum(one_by_one) > 1000
+    empty = (rgl.c_node * 1)()
+    assert rgl.call("rgl_graph_get_result_sizes", empty, 0, XYZ, None) == rgl.RGL_SUCCESS
+    counts = (C.c_int32 * 2)()
+    pair = (rgl.c_node * 2)(graphs[0].out.h.value, graphs[0].rays.h.value)  # a rays node holds no points
+    assert rgl.call("rgl_graph_get_result_sizes", pair, 2, XYZ, counts) != rgl.RGL_SUCCESS
+    assert counts[0] == one_by_one[0]

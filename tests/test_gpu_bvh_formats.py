@@ -1,7 +1,8 @@
-"""BVH node formats: the 4-wide FP32 and the 4-wide 8-bit-quantized trees must give results bit-identical to the
-binary FP32 tree (which the parity tests pin against the FP64 oracle and brute force), on random triangle soups,
-flat and single-triangle meshes, far-from-origin coordinates, the C1 and urban scenes, batches, and TLAS refits.
-Quantized child boxes are rounded outward in the decoder's own FP32 arithmetic, so no reachable triangle is culled.
+"""BVH node formats: the 4-wide FP32 and the 4-wide 8-bit- and 16-bit-quantized trees must give results
+bit-identical to the binary FP32 tree (which the parity tests pin against the FP64 oracle and brute force), on random
+triangle soups, flat and single-triangle meshes, far-from-origin coordinates, the C1 and urban scenes, batches, and
+TLAS refits. Quantized child bounds lie outside the exact ones by a margin that covers the traversal's folded decode,
+so no reachable triangle is culled.
 """
 import numpy as np
 import pytest
@@ -15,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 XYZ, IS_HIT, ENT, DIST, NORMAL = 1, 4, 7, 8, 22
 FIELDS = [XYZ, DIST, ENT, NORMAL, IS_HIT]
-FORMATS = [R.BVH2, R.BVH4_F32, R.BVH4_Q8]
+FORMATS = [R.BVH2, R.BVH4_F32, R.BVH4_Q8, R.BVH4_Q16]
 
 
 @pytest.fixture
@@ -81,6 +82,7 @@ def test_formats_identical_and_oracle_on_c1(lib, oracle):
     assert_identical(res, "c1")
     orc = run_oracle(oracle, case, fields=FIELDS)
     compare(res[R.BVH4_Q8], orc, len(lidar.rays), "c1 quantized")
+    compare(res[R.BVH4_Q16], orc, len(lidar.rays), "c1 16-bit quantized")
 
 
 def test_formats_identical_on_urban_scene(lib):
