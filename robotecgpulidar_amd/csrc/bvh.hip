@@ -865,7 +865,131 @@ __global__ void __launch_bounds__(kPlocBlock) kPloc(int n, const Aabb* boxes, co
 
 inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
+// Trace order of lidar rays: rays are binned into square tiles of their direction's (azimuth, elevation) in the
+// sensor frame, tiles of kTraceTileCells x kTraceTileCells cells of the mean angular area per ray (so a tile holds
+// about that many rays), listed elevation band by elevation band; a stable sort by tile keeps index order inside a
+// tile. A channel-by-channel lidar pattern then becomes runs of consecutive rays of a few adjacent channels.
+#ifndef RGL_TRACE_TILE_CELLS
+#define RGL_TRACE_TILE_CELLS 6
+#endif
+constexpr float kTraceTileCells = RGL_TRACE_TILE_CELLS;
+
+__device__ __forceinline__ void rayAngles(const Mat3x4f& r, float& az, float& el)
+{
+	const float dx = r.rc[0][2], dy = r.rc[1][2], dz = r.rc[2][2];
+	az = atan2f(dy, dx);
+	el = atan2f(dz, sqrtf(dx * dx + dy * dy));
+}
+
+// bounds: [0] min azimuth, [1] min elevation, [3] max azimuth, [4] max elevation (ordered-float encoding).
+__global__ void kAngleBounds(int n, const Mat3x4f* rays, uint32_t* bounds)
+{
+	__shared__ float smin[2][kBlock], smax[2][kBlock];
+	float lo[2] = {FLT_MAX, FLT_MAX}, hi[2] = {-FLT_MAX, -FLT_MAX};
+	for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+		float a[2];
+		rayAngles(rays[i], a[0], a[1]);
+		for (int k = 0; k < 2; ++k) {
+			lo[k] = fminf(lo[k], a[k]);
+			hi[k] = fmaxf(hi[k], a[k]);
+		}
+	}
+	for (int k = 0; k < 2; ++k) {
+		smin[k][threadIdx.x] = lo[k];
+		smax[k][threadIdx.x] = hi[k];
+	}
+	__syncthreads();
+	for (int st = kBlock / 2; st > 0; st >>= 1) {
+		if (threadIdx.x < st) {
+			for (int k = 0; k < 2; ++k) {
+				smin[k][threadIdx.x] = fminf(smin[k][threadIdx.x], smin[k][threadIdx.x + st]);
+				smax[k][threadIdx.x] = fmaxf(smax[k][threadIdx.x], smax[k][threadIdx.x + st]);
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		for (int k = 0; k < 2; ++k) {
+			atomicMin(&bounds[k], floatToOrdered(smin[k][0]));
+			atomicMax(&bounds[3 + k], floatToOrdered(smax[k][0]));
+		}
+	}
+}
+
+// Key = (elevation band, azimuth block) of the ray's tile; 0 for every ray when the directions span no area (one
+// channel or one azimuth: index order is already coherent there).
+__global__ void kTileKeys(int n, const Mat3x4f* rays, const uint32_t* bounds, uint64_t* keys, int32_t* vals)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const float az0 = orderedToFloat(bounds[0]), el0 = orderedToFloat(bounds[1]);
+	const float azSpan = orderedToFloat(bounds[3]) - az0, elSpan = orderedToFloat(bounds[4]) - el0;
+	uint64_t key = 0;
+	const float cellArea = azSpan * elSpan / static_cast<float>(n);
+	if (cellArea > 0.0f) {
+		const float tile = kTraceTileCells * sqrtf(cellArea);
+		float az, el;
+		rayAngles(rays[i], az, el);
+		const uint32_t band = static_cast<uint32_t>(fminf(floorf((el - el0) / tile), 1048575.0f));
+		const uint32_t block = static_cast<uint32_t>(fminf(floorf((az - az0) / tile), 1048575.0f));
+		key = (static_cast<uint64_t>(band) << 20) | block;
+	}
+	keys[i] = key;
+	vals[i] = i;
+}
+
+void radixSortPairs(hipStream_t stream, int n, uint64_t* keys, uint64_t* keysAlt, int32_t* vals, int32_t* valsAlt,
+                    uint32_t* hist)
+{
+	const int tiles = (n + kSortTile - 1) / kSortTile;
+	uint64_t* kIn = keys;
+	uint64_t* kOut = keysAlt;
+	int32_t* vIn = vals;
+	int32_t* vOut = valsAlt;
+	for (int pass = 0; pass < 8; ++pass) {
+		const int shift = pass * 8;
+		hipLaunchKernelGGL(kSortHistogram, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, shift, tiles, hist);
+		hipLaunchKernelGGL(kSortScan, dim3(1), dim3(1024), 0, stream, 256 * tiles, hist);
+		hipLaunchKernelGGL(kSortScatter, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, vIn, kOut, vOut, shift, tiles,
+		                   hist);
+		std::swap(kIn, kOut);
+		std::swap(vIn, vOut);
+	}
+	// 8 passes: the sorted pairs are back in (keys, vals).
+}
+
 } // namespace
+
+size_t directionOrderScratchBytes(int n)
+{
+	const size_t tiles = (static_cast<size_t>(n) + kSortTile - 1) / kSortTile;
+	return 2 * sizeof(uint64_t) * n + sizeof(int32_t) * n + sizeof(uint32_t) * 256 * tiles + 2048;
+}
+
+void directionOrder(hipStream_t stream, int n, const Mat3x4f* rays, int32_t* order, void* scratch)
+{
+	if (n <= 0) {
+		return;
+	}
+	const int tiles = (n + kSortTile - 1) / kSortTile;
+	char* s = static_cast<char*>(scratch);
+	auto take = [&](size_t bytes) {
+		char* p = s;
+		s += (bytes + 255) & ~static_cast<size_t>(255);
+		return p;
+	};
+	uint64_t* keys = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * n));
+	uint64_t* keysAlt = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * n));
+	int32_t* valsAlt = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+	uint32_t* hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 256 * tiles));
+	uint32_t* bounds = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 8));
+	hipLaunchKernelGGL(kInitBounds, dim3(1), dim3(64), 0, stream, bounds);
+	hipLaunchKernelGGL(kAngleBounds, dim3(std::min(blocksFor(n), 1024)), dim3(kBlock), 0, stream, n, rays, bounds);
+	hipLaunchKernelGGL(kTileKeys, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, rays, bounds, keys, order);
+	radixSortPairs(stream, n, keys, keysAlt, order, valsAlt, hist);
+}
 
 size_t lbvhScratchBytes(int n)
 {
@@ -912,20 +1036,7 @@ void buildLbvh(hipStream_t stream, int n, const Aabb* boxes, bool leafRefIsPrim,
 	hipLaunchKernelGGL(kMorton, dim3(blocksFor(n)), dim3(kBlock), 0, stream, n, boxes, bounds, keys, leafPrim,
 	                   leafRefIsPrim ? 1 : 0);
 
-	uint64_t* kIn = keys;
-	uint64_t* kOut = keysAlt;
-	int32_t* vIn = leafPrim;
-	int32_t* vOut = valsAlt;
-	for (int pass = 0; pass < 8; ++pass) {
-		const int shift = pass * 8;
-		hipLaunchKernelGGL(kSortHistogram, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, shift, tiles, hist);
-		hipLaunchKernelGGL(kSortScan, dim3(1), dim3(1024), 0, stream, 256 * tiles, hist);
-		hipLaunchKernelGGL(kSortScatter, dim3(tiles), dim3(kBlock), 0, stream, n, kIn, vIn, kOut, vOut, shift, tiles,
-		                   hist);
-		std::swap(kIn, kOut);
-		std::swap(vIn, vOut);
-	}
-	// 8 passes: sorted data is back in (keys, leafPrim).
+	radixSortPairs(stream, n, keys, keysAlt, leafPrim, valsAlt, hist);
 	if (n <= kPlocMax) {
 		// Small trees (every TLAS, most meshes): agglomerative topology, built in one workgroup.
 		Aabb* b0 = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * n));

@@ -169,6 +169,7 @@ struct RaytraceOutputs
 struct RaytraceParams
 {
 	const Mat3x4f* rays;          // rays of the rays-node chain (world frame unless rayPreTransformed)
+	const int32_t* order;         // single-sample launches: thread t traces beam order[t] (nullptr: beam t)
 	int32_t rayCount;
 	int32_t rayPreTransformed;    // 1: the world-frame ray is rayPreTransform * rays[i], composed in FP64
 	Mat3x4f rayPreTransform;

@@ -417,10 +417,29 @@ static void requireNoInputs(const Node& n, const std::vector<Node::Ptr>& inputs)
 }
 
 // ------------------------------------------------------------------------------------------------ rays nodes
+// Rays are traced tile by tile of their directions once there are enough of them to fill several waves: lidar
+// patterns list rays row by row (one elevation, many azimuths), and a wave of 64 consecutive rays then spans a thin
+// strip of directions; the rays of a compact patch (a few channels x a few azimuths) descend the same BVH nodes
+// longer. The order is computed once per update.
 void FromMat3x4fRaysNode::setParameters(const Mat3x4f* r, int n)
 {
+	constexpr int kOrderMinRays = 4096;
 	RGL_HIP(hipMemcpy(buf.ensure(sizeof(Mat3x4f) * n), r, sizeof(Mat3x4f) * n, hipMemcpyHostToDevice));
 	count = n;
+	ordered = n >= kOrderMinRays;
+	if (ordered) {
+		// One stream and one scratch buffer serve every rays node (API calls come from one thread); the scratch only
+		// grows, so repeated updates allocate nothing. Both live until the process ends (never destroyed during
+		// static destruction, after which the HIP runtime may be gone).
+		static Stream* s = new Stream();
+		static DeviceBuffer* scratch = new DeviceBuffer();
+		gpu::directionOrder(s->get(), n, buf.as<Mat3x4f>(), order.ensureT<int32_t>(n),
+		                    scratch->ensure(gpu::directionOrderScratchBytes(n)));
+		RGL_HIP(hipGetLastError());
+		s->sync();
+	} else {
+		order.release();
+	}
 }
 void FromMat3x4fRaysNode::validateImpl() { requireNoInputs(*this, inputs); }
 
@@ -626,6 +645,7 @@ RaytraceParams RaytraceNode::buildParams(const SceneView&) const
 	RaytraceParams p{};
 	const IRaysNode::DeferredRays dr = raysNode->deferredRays();
 	p.rays = dr.rays;
+	p.order = raysNode->traceOrder();
 	p.rayPreTransformed = dr.hasPre ? 1 : 0;
 	p.rayPreTransform = dr.pre;
 	p.rayCount = raysNode->rayCount();

@@ -116,6 +116,9 @@ struct IRaysNode : virtual Node
 	virtual const float* timeOffsets() const { return nullptr; }
 	virtual int timeOffsetsCount() const { return 0; }
 	virtual Mat3x4f cumulativeRayTransform() const { return Mat3x4f::identity(); }
+	// Order in which a raytrace launch traces the rays (a permutation of [0, rayCount) grouping rays of close
+	// directions), or nullptr for index order. Results do not depend on it: every ray writes its own index.
+	virtual const int32_t* traceOrder() const { return nullptr; }
 	// The rays as a raytrace launch consumes them: `rays`, with `pre` still to be applied when `hasPre` (a transform
 	// the chain left to the raytrace kernel).
 	struct DeferredRays
@@ -139,6 +142,7 @@ struct IRaysNodeSingleInput : IRaysNode
 	const float* timeOffsets() const override { return input->timeOffsets(); }
 	int timeOffsetsCount() const override { return input->timeOffsetsCount(); }
 	Mat3x4f cumulativeRayTransform() const override { return input->cumulativeRayTransform(); }
+	const int32_t* traceOrder() const override { return input->traceOrder(); }
 
 protected:
 	std::shared_ptr<IRaysNode> input;
@@ -321,6 +325,7 @@ struct FromMat3x4fRaysNode : IRaysNode
 	BatchStage batchStage() const override { return BatchStage::Passive; }
 	int rayCount() const override { return count; }
 	const Mat3x4f* rays() const override { return buf.as<Mat3x4f>(); }
+	const int32_t* traceOrder() const override { return ordered ? order.as<int32_t>() : nullptr; }
 
 protected:
 	void validateImpl() override;
@@ -328,6 +333,8 @@ protected:
 
 private:
 	DeviceBuffer buf;
+	DeviceBuffer order;
+	bool ordered = false;
 	int count = 0;
 };
 

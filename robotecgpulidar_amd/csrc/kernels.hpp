@@ -33,6 +33,11 @@ void refitLbvh(hipStream_t stream, int n, const Aabb* boxes, const int32_t* leaf
 // 4-wide nodes from a binary LBVH over n leaves (built by buildLbvh). bvh4Ranks marks the kept (even-depth) internal
 // nodes in `scratch` (bvh4ScratchBytes(n)) and writes their number to *countDev; bvh4Collapse then writes that many
 // nodes of `format` (kBvh4F, kBvh4Q or kBvh4H) to `out`. Leaf references keep the binary tree's convention.
+// Trace order of n rays: a permutation of [0, n) that lists rays tile by tile of their directions' (azimuth,
+// elevation), so that the rays one wave traces point into a compact patch of directions and still form runs of
+// consecutive indices (`scratch`: directionOrderScratchBytes(n)).
+size_t directionOrderScratchBytes(int n);
+void directionOrder(hipStream_t stream, int n, const Mat3x4f* rays, int32_t* order, void* scratch);
 size_t bvh4ScratchBytes(int n);
 void bvh4Ranks(hipStream_t stream, int n, const int32_t* parentSlot, void* scratch, int32_t* countDev);
 void bvh4Collapse(hipStream_t stream, int n, const BvhNode* nodes, const void* scratch, int format, void* out);

@@ -1119,20 +1119,26 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 
 constexpr int kBeamsPerMultiBlock = kRtBlock / 64; // divergent beams: one wave each
 
+// Traces the beam(s) of one thread (single-sample: thread t of the launch traces beam order[t]) and returns the beam
+// index of a single-sample thread, or -1.
 template<int kFmt, bool kMulti, int kBlock = kRtBlock>
-__device__ __forceinline__ void raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
-                                            int localBlock)
+__device__ __forceinline__ int raytraceOne(const SceneView& sc, const RaytraceParams& p, const RaytraceOutputs& out,
+                                           int localBlock)
 {
 	if constexpr (kMulti) {
 		const int beam = localBlock * kBeamsPerMultiBlock + static_cast<int>(threadIdx.x >> 6);
 		if (beam < p.rayCount) {
 			raytraceBeamWave<kFmt>(sc, p, out, beam);
 		}
+		return -1;
 	} else {
-		const int beam = localBlock * kBlock + threadIdx.x;
-		if (beam < p.rayCount) {
-			raytraceBeam<kFmt, kBlock>(sc, p, out, beam);
+		const int t = localBlock * kBlock + threadIdx.x;
+		if (t >= p.rayCount) {
+			return -1;
 		}
+		const int beam = p.order != nullptr ? p.order[t] : t;
+		raytraceBeam<kFmt, kBlock>(sc, p, out, beam);
+		return beam;
 	}
 }
 
@@ -1169,10 +1175,9 @@ __device__ __forceinline__ void raytraceBatchOne(const RaytraceBatchItem* items,
 	const int item = batchItemOf(items, itemCount, block, [](const RaytraceBatchItem& r) { return r.firstBlock; });
 	const RaytraceBatchItem& it = items[item];
 	const int localBlock = block - it.firstBlock;
-	raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, localBlock);
+	const int beam = raytraceOne<kFmt, kMulti, kBlock>(it.sc, it.p, it.out, localBlock);
 	if constexpr (!kMulti) {
-		const int beam = localBlock * kBlock + threadIdx.x;
-		if (it.fusedCount > 0 && beam < it.p.rayCount) {
+		if (it.fusedCount > 0 && beam >= 0) {
 			applyFusedStages(it, beam);
 		}
 	}
