@@ -408,7 +408,8 @@ def serve_closed_loop(entries, batching: bool, slots: int, budget: int, duration
     tickets = [server.submit(e) for e in entries]
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < duration_s:
-        server.step()
+        if server.step() == 0:
+            continue  # nothing finished: no ticket changed
         for i, t in enumerate(tickets):
             if server.poll(t)[0] == rgl.Server.DONE:
                 tickets[i] = server.submit(entries[i])

@@ -127,9 +127,10 @@ RGL_API rgl_status_t rgl_node_points_radar_bin(rgl_node_t* node, float range_ste
 
 // ---------------------------------------------------------------------------------------------------- serving
 // A request server doing continuous batching of sensor frames (one graph per sensor) on the calling thread.
-// Requests queue in FIFO order; rgl_server_step retires finished batches and launches every admissible queued
-// request into each free slot as one batch (rgl_graph_run_batch semantics), so up to `slots` batches overlap on the
-// device. A request is admissible when its graph has no earlier request in flight and the acceleration structures
+// Requests queue in FIFO order; rgl_server_step retires finished batches and launches the admissible queued requests
+// into the free slots, spread evenly over them (at most `max_batch_graphs` per batch; rgl_graph_run_batch semantics),
+// so up to `slots` batches overlap on the device and the requests of a finished batch run again while the others
+// still occupy it. A request is admissible when its graph has no earlier request in flight and the acceleration structures
 // of the scenes it raytraces fit `hbm_budget_bytes` (<= 0: unlimited) after evicting least recently used idle
 // scenes (evicted scenes rebuild on their next use); a request whose scenes alone exceed the budget fails (state 3).
 // With `batching` false every request runs alone through

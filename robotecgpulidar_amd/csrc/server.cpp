@@ -218,6 +218,10 @@ int Server::step()
 			retire(s);
 		}
 	}
+	// Queued requests are spread evenly over the free slots (at most maxBatchGraphs per batch), so that several
+	// batches are in flight: the requests of the batch that finishes first return to the queue and run again while
+	// the others still occupy the device, instead of every request waiting for one large batch.
+	int freeSlots = static_cast<int>(std::count_if(slots.begin(), slots.end(), [](const Slot& x) { return !x.active; }));
 	for (Slot& s : slots) {
 		if (s.active || queue.empty()) {
 			continue;
@@ -225,7 +229,9 @@ int Server::step()
 		std::set<Scene*> pinned = pinnedScenes();
 		std::set<const GraphRunCtx*> chosen;
 		std::vector<Request> batch;
-		const int limit = batching ? maxBatchGraphs : 1;
+		const int share = static_cast<int>((queue.size() + static_cast<size_t>(freeSlots) - 1) / static_cast<size_t>(freeSlots));
+		--freeSlots;
+		const int limit = batching ? std::min(maxBatchGraphs, std::max(share, 1)) : 1;
 		for (auto it = queue.begin(); it != queue.end() && static_cast<int>(batch.size()) < limit;) {
 			if (!it->entry->hasCtx()) {
 				GraphRunCtx::createAndAttach(it->entry);
