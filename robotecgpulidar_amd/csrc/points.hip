@@ -344,12 +344,44 @@ __device__ __forceinline__ int findItem(const Item* items, int itemCount, Key ke
 	return batchItemOf(items, itemCount, static_cast<int>(blockIdx.x), key);
 }
 
+// Tile counts of many graphs: one wave per tile of kBlock * kItems_ points, each lane summing 16-byte groups of four
+// flags (tiles start at multiples of 1024 flags, so the groups are aligned; the group that crosses the live count is
+// read flag by flag), then a wave reduction. No LDS or block barrier: with a block per tile the launch was bound by
+// block scheduling rather than by its 4 bytes per point.
 template<int kItems_>
-__global__ void kTileCountsBatch(const CompactItem* __restrict__ items, int itemCount)
+__global__ void kTileCountsBatch(const CompactItem* __restrict__ items, int itemCount, int totalTiles)
 {
-	const int i = findItem(items, itemCount, [](const CompactItem& c) { return c.firstTile; });
+	const int tile = static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
+	if (tile >= totalTiles) {
+		return; // the whole wave
+	}
+	const int lane = static_cast<int>(threadIdx.x & 63);
+	const int i = batchItemOf(items, itemCount, tile, [](const CompactItem& c) { return c.firstTile; });
 	const CompactItem& it = items[i];
-	tileCount<kItems_>(it.capacity, it.countIn, it.flags, static_cast<int>(blockIdx.x) - it.firstTile, it.tileCounts);
+	const int n = liveCount(it.capacity, it.countIn);
+	const int local = tile - it.firstTile;
+	constexpr int kTile = kBlock * kItems_;
+	const int base = local * kTile;
+	const bool aligned = (reinterpret_cast<uintptr_t>(it.flags) & 15u) == 0; // device buffers are; anything else reads flag by flag
+	int c = 0;
+#pragma unroll
+	for (int g = 0; g < kTile / 256; ++g) {
+		const int p = base + g * 256 + 4 * lane;
+		if (aligned && p + 3 < n) {
+			const int4 v = *reinterpret_cast<const int4*>(it.flags + p);
+			c += (v.x != 0) + (v.y != 0) + (v.z != 0) + (v.w != 0);
+		} else {
+			for (int k = 0; k < 4; ++k) {
+				c += (p + k < n && it.flags[p + k] != 0) ? 1 : 0;
+			}
+		}
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		c += __shfl_down(c, off, 64);
+	}
+	if (lane == 0) {
+		it.tileCounts[local] = c;
+	}
 }
 
 __global__ void kTileScanBatch(const CompactItem* __restrict__ items)
@@ -635,12 +667,13 @@ void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, i
 	if (itemCount <= 0 || totalTiles <= 0) {
 		return;
 	}
+	const int countBlocks = (totalTiles + kWaves - 1) / kWaves;
 	if (variant == 1) {
-		hipLaunchKernelGGL(kTileCountsBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kTileCountsBatch<4>, dim3(countBlocks), dim3(kBlock), 0, stream, items, itemCount, totalTiles);
 		hipLaunchKernelGGL(kTileScanBatch, dim3(itemCount), dim3(kBlock), 0, stream, items);
 		hipLaunchKernelGGL(kCompactScatterBatch<4>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	} else {
-		hipLaunchKernelGGL(kTileCountsBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
+		hipLaunchKernelGGL(kTileCountsBatch<kItems>, dim3(countBlocks), dim3(kBlock), 0, stream, items, itemCount, totalTiles);
 		hipLaunchKernelGGL(kTileScanBatch, dim3(itemCount), dim3(kBlock), 0, stream, items);
 		hipLaunchKernelGGL(kCompactScatterBatch<kItems>, dim3(totalTiles), dim3(kBlock), 0, stream, items, itemCount);
 	}
