@@ -493,6 +493,18 @@ def graph_run_batch(nodes) -> None:
     check("rgl_graph_run_batch", arr, len(nodes))
 
 
+class BatchRun:
+    """rgl_graph_run_batch over a fixed list of nodes, with the node array built once (a C++ caller passes a static
+    array; rebuilding the ctypes array every frame costs the host tens of microseconds)."""
+
+    def __init__(self, nodes):
+        self.n = len(nodes)
+        self.arr = (c_node * max(self.n, 1))(*[x.h.value for x in nodes])
+
+    def __call__(self) -> None:
+        check("rgl_graph_run_batch", self.arr, self.n)
+
+
 class ResultSizes:
     """Point counts of one field at a fixed list of nodes, read with one rgl_graph_get_result_sizes call."""
 
