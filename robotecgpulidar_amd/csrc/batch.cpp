@@ -148,6 +148,7 @@ void GraphBatch::fusePointStages(RaytraceNode* rt, RaytraceBatchItem& it)
 
 namespace {
 constexpr double kTuneMinUnits = 100000.0;
+constexpr int64_t kTraceOrderMinLaunchRays = int64_t(1) << 20; // about three occupancy generations of 64-ray waves
 // Capacity bytes per prefetch launch: about a third of a millisecond of the host link, so the host starts copying the
 // first sensors' points while the last ones are still on the way.
 #ifndef RGL_PREFETCH_CHUNK_MIB
@@ -528,6 +529,21 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 						break;
 					}
 					default: throw std::logic_error("batch: unexpected stage kind");
+				}
+			}
+			// Direction-tile trace order only in launches of several occupancy generations: there it keeps the rays of
+			// a wave on the same nodes (64-lidar batch: -1.8% raytrace time), while a launch of about one generation
+			// lasts as long as its slowest waves, and grouping costly patches of directions into the same waves
+			// lengthens those (single 262k-ray lidar: +17% with the order).
+			{
+				int64_t launchRays = 0;
+				for (const auto& it : L.rt) {
+					launchRays += it.p.rayCount;
+				}
+				if (launchRays < kTraceOrderMinLaunchRays) {
+					for (auto& it : L.rt) {
+						it.p.order = nullptr;
+					}
 				}
 			}
 			scheduled += group.size();

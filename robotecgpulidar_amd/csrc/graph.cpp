@@ -728,7 +728,8 @@ RaytraceBatchItem RaytraceNode::batchItem(hipStream_t s)
 
 void RaytraceNode::enqueueImpl()
 {
-	const RaytraceBatchItem it = batchItem(stream());
+	RaytraceBatchItem it = batchItem(stream());
+	it.p.order = nullptr; // a single sensor's launch spans about one occupancy generation: index order (see batch.cpp)
 	if (timing) {
 		if (timingUsed == timingEvents.size()) {
 			hipEvent_t a, b;
