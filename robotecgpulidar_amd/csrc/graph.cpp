@@ -64,6 +64,7 @@ void Node::addChild(const Ptr& child)
 	}
 	outputs.push_back(child);
 	child->inputs.push_back(shared_from_this());
+	++structureEpoch();
 	std::stable_sort(outputs.begin(), outputs.end(), [](const Ptr& a, const Ptr& b) { return a->priority > b->priority; });
 	const int32_t maxChild = outputs.front()->priority;
 	if (maxChild > priority) {
@@ -97,6 +98,7 @@ void Node::removeChild(const Ptr& child)
 	child->inputs.erase(pit);
 	dirty = true;
 	child->dirty = true;
+	++structureEpoch();
 }
 
 // src/graph/Node.cpp:186-214
@@ -181,6 +183,7 @@ void Node::validate()
 	}
 	validateImpl();
 	dirty = false;
+	++structureEpoch();
 }
 
 void Node::enqueue()
@@ -506,7 +509,18 @@ void TransformRaysNode::pointOp(PointOpItem& it, FieldCopyList&)
 	it.out0 = out.ensureT<Mat3x4f>(std::max(n, 1));
 	it.m0 = transform;
 }
+// Depends on the graph's structure and on validated field sets only, so it is cached per structure epoch (batch
+// planning asks for it several times per node and frame).
 bool TransformRaysNode::deferToRaytrace() const
+{
+	if (deferEpoch != structureEpoch()) {
+		deferCached = computeDeferToRaytrace();
+		deferEpoch = structureEpoch();
+	}
+	return deferCached;
+}
+
+bool TransformRaysNode::computeDeferToRaytrace() const
 {
 	if (outputs.empty() || input->deferredRays().hasPre) {
 		return false;

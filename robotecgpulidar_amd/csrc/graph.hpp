@@ -70,8 +70,19 @@ struct Node : std::enable_shared_from_this<Node>
 	void waitForCounts();
 	// Waits for all of the last run's device work if there was one (before the node's buffers are replaced).
 	void waitForResultsIfRun();
-	void markDirty() { dirty = true; }
+	void markDirty()
+	{
+		dirty = true;
+		++structureEpoch();
+	}
 	bool isValid() const { return !dirty; }
+	// Advances whenever any node is marked dirty, linked, unlinked or validated: a value derived from graph structure
+	// or validated node state may be cached for as long as the epoch is unchanged.
+	static uint64_t& structureEpoch()
+	{
+		static uint64_t epoch = 0;
+		return epoch;
+	}
 
 	virtual std::string getName() const = 0;
 	hipStream_t stream() const;
@@ -411,8 +422,11 @@ protected:
 	void enqueueImpl() override;
 
 private:
+	bool computeDeferToRaytrace() const;
 	Mat3x4f transform = Mat3x4f::identity();
 	DeviceBuffer out;
+	mutable uint64_t deferEpoch = ~uint64_t(0);
+	mutable bool deferCached = false;
 };
 
 // Seeded noise: stream identity of one noise node.
