@@ -1058,15 +1058,16 @@ __device__ __forceinline__ void raytraceBeamWave(const SceneView& sc, const Rayt
 		}
 	}
 	// Trackers hold (sample, key): distance for FIRST / SECOND / LAST, intensity for STRONGEST / SECOND_STRONGEST.
+	// The samples that hit are visited in sample order through the ballot of their lanes; the sample index is
+	// wave-uniform, so each value is read from its lane with v_readlane into a scalar register (a cross-lane shuffle
+	// would be an LDS permute and a wait per value in this sequential loop).
 	int fS = -1, sS = -1, lS = -1, stS = -1, ssS = -1;
 	float fD = 0.0f, sD = 0.0f, lD = 0.0f, stI = 0.0f, ssI = 0.0f;
-	const float d0 = __shfl(dist, 0, 64);
-	for (int k = 0; k < kBeamSamples; ++k) {
-		if (__shfl(isHit, k, 64) == 0) {
-			continue;
-		}
-		const float d = __shfl(dist, k, 64);
-		const float I = __shfl(inten, k, 64);
+	const float d0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dist), 0));
+	for (uint64_t pending = __ballot(isHit != 0); pending != 0; pending &= pending - 1) {
+		const int k = static_cast<int>(__builtin_ctzll(pending));
+		const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dist), k));
+		const float I = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inten), k));
 		if (fS < 0) {
 			fS = sS = lS = stS = ssS = k;
 			fD = sD = lD = d;
