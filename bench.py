@@ -15,7 +15,8 @@ compact -> yield), run plain.
 `roofline` is computed for the (batched) raytrace kernel from HIP-event durations on the stream it runs on: the
 bytes it must move (ray poses, output fields, and the unique BVH and scene records its rays touch) over its time,
 against 8 TB/s HBM. `accuracy` compares the benchmark frame's own hit distances with the FP64 oracle at 65,536
-committed beams. `cpu_baseline` times the FP32 CPU port of the same graph (oracle/cpu_port.cpp) on whole 64-lidar
+committed beams; the urban and multi-return lines carry the same figures for their configurations from the committed
+golden fixtures (C2 / C1 and C3 at reduced ray counts). `cpu_baseline` times the FP32 CPU port of the same graph (oracle/cpu_port.cpp) on whole 64-lidar
 frames on the host's CPU share.
 """
 from __future__ import annotations
@@ -150,6 +151,36 @@ def roofline(b_io: int, footprint: int, b_canonical: int, kernel_ms: float, prof
 def kernel_profile(name: str):
     tf = ROOT / "profiles" / name
     return json.loads(tf.read_text()) if tf.exists() else None
+
+
+def golden_accuracy(names):
+    """Accuracy of a workload's configuration against the FP64 oracle, from the committed golden fixtures of the same
+    configuration at reduced ray count (tests/golden/cases.py): each case runs once through the library in a scene
+    of its own, and tests.helpers.check_golden reports hit-mask differences (all on beams the oracle marks
+    geometrically ambiguous), the distances compared and the largest |d - d_FP64|; a case that misses the parity bar
+    reports the failed check instead of numbers."""
+    sys.path.insert(0, str(ROOT))
+    from tests.golden import cases  # noqa: E402  (fixture definitions; no oracle code)
+    from tests.helpers import Graph, check_golden  # noqa: E402
+    meta = json.loads((GOLDEN / "golden.json").read_text())
+    out = {}
+    for name in names:
+        rgl.cleanup()
+        rgl.check("rgl_configure_logging", 6, None, False)
+        case, fields = cases.CASES[name]()
+        S.load_into_rgl(case.spec)
+        gpu = Graph(rgl, case, fields).run()
+        g = np.load(GOLDEN / f"{name}.npz")
+        hit = np.unpackbits(g["hit_bits"])[:meta[name]["return_points"]].astype(np.int32)
+        values = {int(k[1:]): g[k] for k in g.files if k.startswith("f")}
+        try:
+            stats = check_golden(gpu, hit, g["entity"], g["ambiguous"], g["sample"], values, case.return_mode, name,
+                                 min_compared=0)
+        except AssertionError as e:
+            stats = {"parity": f"failed: {e}"}
+        out[name] = {"rays": int(len(case.rays)), **stats, "reference": f"tests/golden/{name}.npz (FP64 oracle)"}
+    rgl.cleanup()
+    return out
 
 
 def c4_accuracy(graphs):
@@ -696,10 +727,16 @@ def main():
     rgl.set_bvh_format(fmt)
     if args.workload in ("serve", "multireturn", "radar"):
         run = {"serve": run_serve, "multireturn": run_multireturn, "radar": run_radar}[args.workload]
-        print(json.dumps(run(args)))
+        result = run(args)
+        if args.workload == "multireturn":
+            result["accuracy"] = golden_accuracy(["c3_first_last", "c3_strongest"])
+        print(json.dumps(result))
         rgl.cleanup()
         return
     result = run_batch64(args) if args.workload == "batch64" else run_urban(args)
+    if args.workload == "urban":
+        rgl.cleanup()
+        result["accuracy"] = golden_accuracy(["urban", "c1"])
     if args.workload == "batch64" and not args.no_serve:
         rgl.cleanup()
         result["serving"] = run_serve(args)
