@@ -110,3 +110,36 @@ def test_ray_poses_follow_the_transform_when_requested(rgl_lib):
     want = np.einsum("ij,njk->nik", p[:, :3], lidar.astype(np.float64))
     want[:, :, 3] += p[:, 3]
     np.testing.assert_allclose(got, want, rtol=0, atol=4e-6 * max(1.0, float(np.abs(want).max())))
+
+
+def test_deferral_follows_graph_changes(rgl_lib):
+    """The decision to defer a transform into the raytrace launch is cached per graph structure: adding a yield that
+    requests RAY_POSE must materialize the transform at the next run (poses equal to the transform of the rays), and
+    removing it must defer it again: the third run's points are bit-identical to the first's, and the materialized
+    run differs from them only by the FP32 rounding of the composed origin."""
+    rgl = rgl_lib
+    _, scenes, _ = build_world(rgl)
+    lidar = S.spinning_lidar(8, 64, -10.0, 10.0)
+    pose = S.f32(S.trs((1.0, 2.0, 3.0), (10.0, 20.0, 30.0), (1, 1, 1)))
+    rays = rgl.node_rays_from_mat3x4f(rgl.Node(), lidar)
+    tf = rgl.node_rays_transform(rgl.Node(), pose)
+    rt = rgl.node_raytrace(rgl.Node(), scenes[1])
+    out = rgl.node_points_yield(rgl.Node(), [XYZ, DIST])
+    rgl.chain(rays, tf, rt, out)
+    rgl.graph_run(rays)
+    first = {f: rgl.result(out, f) for f in (XYZ, DIST)}
+    poses = rgl.node_points_yield(rgl.Node(), [RAY_POSE])
+    rgl.add_child(rt, poses)
+    rgl.graph_run(rays)
+    got = rgl.result(poses, RAY_POSE).reshape(-1, 3, 4).astype(np.float64)
+    p = pose.astype(np.float64)
+    want = np.einsum("ij,njk->nik", p[:, :3], lidar.astype(np.float64))
+    want[:, :, 3] += p[:, 3]
+    np.testing.assert_allclose(got, want, rtol=0, atol=4e-6 * max(1.0, float(np.abs(want).max())))
+    second = {f: rgl.result(out, f) for f in (XYZ, DIST)}
+    rgl.check("rgl_graph_node_remove_child", rt.h, poses.h)
+    rgl.graph_run(rays)
+    third = {f: rgl.result(out, f) for f in (XYZ, DIST)}
+    for f in (XYZ, DIST):
+        assert first[f].tobytes() == third[f].tobytes(), f
+        np.testing.assert_allclose(second[f], first[f], rtol=1e-5, atol=1e-4)
