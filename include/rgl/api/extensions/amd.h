@@ -82,7 +82,9 @@ RGL_API rgl_status_t rgl_graph_get_result_sizes(const rgl_node_t* nodes, int32_t
 // between the raytrace block sizes (128 or 64 threads) and the compaction tiles (8 or 4 points per thread), each
 // launch of at least 100k rays / points timed with HIP events; after three timings per variant the fastest per
 // element is kept. Variants never change results. configure_autotune(true) restarts tuning; force_variants fixes
-// the variants (0 or 1 each) and disables tuning; get_variants reports the variants in use (-1 while tuning).
+// the variants and disables tuning (raytrace 0, 1, or 2 = a persistent grid of 64-thread blocks claiming runs of
+// blocks from a counter, which the tuner does not consider: it measured slower than hardware block dispatch;
+// compaction 0 or 1); get_variants reports the variants in use (-1 while tuning).
 RGL_API rgl_status_t rgl_graph_batch_configure_autotune(bool enable);
 RGL_API rgl_status_t rgl_graph_batch_force_variants(int32_t raytrace_variant, int32_t compact_variant);
 RGL_API rgl_status_t rgl_graph_batch_get_variants(int32_t* out_raytrace_variant, int32_t* out_compact_variant);

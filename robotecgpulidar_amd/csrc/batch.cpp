@@ -41,7 +41,7 @@ struct GraphBatch::Plan
 		std::vector<RadarPostprocessPointsNode*> rpNodes;
 		std::vector<Node*> other;
 		int rtBlocks = 0, rtmBlocks = 0, opBlocks = 0, cpTiles = 0, rpBlocks = 0;
-		size_t rtOff = 0, rtmOff = 0, opOff = 0, cpOff = 0, rpOff = 0;
+		size_t rtOff = 0, rtmOff = 0, opOff = 0, cpOff = 0, rpOff = 0, claimOff = 0;
 		void clear()
 		{
 			rt.clear();
@@ -237,7 +237,7 @@ void GraphBatch::setAutotune(bool enable)
 	harvest(true);
 	autotune = enable;
 	forcedRt = forcedCp = 0;
-	rtTune = Tuner{gpu::kRaytraceVariants};
+	rtTune = Tuner{gpu::kRaytraceTunedVariants};
 	cpTune = Tuner{gpu::kCompactVariants};
 }
 
@@ -610,6 +610,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 		for (int l = 0; l <= maxLevel; ++l) {
 			Plan::Level& L = pl.levels[l];
 			L.rtOff = reserve(sizeof(RaytraceBatchItem) * L.rt.size());
+			L.claimOff = reserve(sizeof(uint32_t)); // block claims of the persistent raytrace launch, zero at start
 			L.rtmOff = reserve(sizeof(RaytraceBatchItem) * L.rtm.size());
 			L.opOff = reserve(sizeof(PointOpItem) * L.op.size());
 			L.cpOff = reserve(sizeof(CompactItem) * L.cp.size());
@@ -645,6 +646,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			if (!L.rt.empty()) {
 				std::memcpy(host + L.rtOff, L.rt.data(), sizeof(RaytraceBatchItem) * L.rt.size());
 			}
+			std::memset(host + L.claimOff, 0, sizeof(uint32_t));
 			if (!L.rtm.empty()) {
 				std::memcpy(host + L.rtmOff, L.rtm.data(), sizeof(RaytraceBatchItem) * L.rtm.size());
 			}
@@ -721,7 +723,8 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 							timeStart(s, pt);
 						}
 						gpu::raytraceBatch(s, reinterpret_cast<const RaytraceBatchItem*>(dev + L.rtOff),
-						                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false, rtVariant);
+						                   static_cast<int>(L.rt.size()), L.rtBlocks, Scene::bvhFormat(), false, rtVariant,
+						                   reinterpret_cast<uint32_t*>(dev + L.claimOff));
 						RGL_HIP(hipGetLastError());
 						if (tune) {
 							RGL_HIP(hipEventRecord(pt.b, s));
@@ -788,7 +791,7 @@ void GraphBatch::run(const std::vector<Node::Ptr>& entryNodes)
 			for (int l = 0; l <= maxLevel; ++l) {
 				const Plan::Level& L = pl.levels[l];
 				for (int64_t v : {static_cast<int64_t>(L.rt.size()), static_cast<int64_t>(L.rtBlocks),
-				                  static_cast<int64_t>(L.rtOff), static_cast<int64_t>(L.rtm.size()),
+				                  static_cast<int64_t>(L.rtOff), static_cast<int64_t>(L.claimOff), static_cast<int64_t>(L.rtm.size()),
 				                  static_cast<int64_t>(L.rtmBlocks), static_cast<int64_t>(L.rtmOff),
 				                  static_cast<int64_t>(L.op.size()), static_cast<int64_t>(L.opBlocks),
 				                  static_cast<int64_t>(L.opOff), static_cast<int64_t>(L.cp.size()),

@@ -308,7 +308,7 @@ private:
 	size_t timingUsed = 0;
 	bool autotune = true;
 	int forcedRt = 0, forcedCp = 0;
-	Tuner rtTune{2}, cpTune{2};
+	Tuner rtTune{2}, cpTune{2}; // tuned raytrace and compaction launch variants (gpu::kRaytraceTunedVariants, kCompactVariants)
 	std::vector<PendingTiming> pending;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> tuneEvents; // free pairs
 	// Captured kernel sequences, keyed by their launch signature; a signature is captured the second time it is

@@ -67,18 +67,21 @@ void countNonzeroKeys(hipStream_t stream, const unsigned long long* keys, size_t
 void bruteForceClosest(hipStream_t stream, const SceneView& scene, const int32_t* triCounts, const RaytraceParams& params,
                        float* outT, int32_t* outInstance, int32_t* outPrim);
 void transformRays(hipStream_t stream, int n, const Mat3x4f* in, Mat3x4f* out, const Mat3x4f& tf);
-// Tunable variants of the batched launches (chosen by the batch autotuner): raytrace single-sample blocks of
-// 128 (0) or 64 (1) threads; compaction tiles of 8 (0) or 4 (1) points per thread.
-constexpr int kRaytraceVariants = 2;
+// Launch variants of the batched launches: raytrace single-sample blocks of 128 (0) or 64 (1) threads, chosen by the
+// batch autotuner, or (2, only when forced) a persistent grid of 64-thread blocks that claim runs of 64-ray blocks;
+// compaction tiles of 8 (0) or 4 (1) points per thread.
+constexpr int kRaytraceVariants = 3;
+constexpr int kRaytraceTunedVariants = 2;
 constexpr int kCompactVariants = 2;
 // Blocks of a raytrace launch over rayCount beams: one beam per thread, or one wave per divergent beam.
 int raytraceBlocks(int rayCount, bool multi, int variant);
 // True if the beams of `p` are traced with the 58-sample divergence model.
 bool beamIsMulti(const RaytraceParams& p);
 // Batched launches: `items` live in device memory; every item's scene uses node format `format`; with `multi` every
-// item has divergent beams (blocks from raytraceBlocks(n, true)), otherwise none has.
+// item has divergent beams (blocks from raytraceBlocks(n, true)), otherwise none has. The persistent variant (2) takes
+// `claims`, a device counter that must be zero when the launch starts.
 void raytraceBatch(hipStream_t stream, const RaytraceBatchItem* items, int itemCount, int totalBlocks, int format,
-                   bool multi, int variant);
+                   bool multi, int variant, uint32_t* claims = nullptr);
 int pointOpBlocks(int capacity);
 void pointOpsBatch(hipStream_t stream, const PointOpItem* items, int itemCount, int totalBlocks);
 void compactBatch(hipStream_t stream, const CompactItem* items, int itemCount, int totalTiles, int variant);

@@ -1,5 +1,5 @@
-"""Launch-shape variants of rgl_graph_run_batch (raytrace block size, compaction tile) never change results, and the
-autotuner settles on one of each after timing them on live batches."""
+"""Launch-shape variants of rgl_graph_run_batch (raytrace block size or the persistent grid, compaction tile) never
+change results, and the autotuner settles on one of each after timing them on live batches."""
 import ctypes as C
 
 import numpy as np
@@ -23,7 +23,7 @@ def variants(rgl):
     return a.value, b.value
 
 
-@pytest.mark.parametrize("rt,cp", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("rt,cp", [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (2, 1)])
 def test_forced_variants_equal_solo(lib, rt, cp):
     rgl = lib
     _, scenes, _ = build_world(rgl)
