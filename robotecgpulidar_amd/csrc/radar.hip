@@ -6,9 +6,10 @@
 // graph). Here it stays on the device: the clustering is sequential in point order (a point joins the first
 // cluster, in creation order, that accepts it), so one wave walks the points and tests 64 clusters per step, with
 // cluster bounds and list ends in LDS; the merge phase tests 64 cluster pairs per step in the reference's pair
-// order. Per-cluster sums then run with one lane per cluster.
+// order. Per-cluster sums then run with the wave spread over one cluster's members at a time.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 
 #include "device_types.hpp"
 #include "kernels.hpp"
@@ -111,6 +112,27 @@ __device__ __forceinline__ float fmin2(float a, float b) { return b < a ? b : a;
 __device__ __forceinline__ float fmax2(float a, float b) { return a < b ? b : a; } // std::max
 __device__ __forceinline__ float lanef(float v, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j)); }
 __device__ __forceinline__ int lanei(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+// Predicates of __builtin_amdgcn_fcmpf (LLVM's compare codes): ordered >=, ordered <=, unordered.
+constexpr int kOGE = 3, kOLE = 5, kUNO = 8;
+
+// Bounds `nb` of a cluster with bounds `o` after adding point (d, a, r, e) (RadarCluster::addPoint,
+// src/graph/RadarPostprocessPointsNode.cpp:274-291), or of a new cluster holding only that point when `fresh`
+// (the constructor, :264-272): fmin2(v, v) = fmax2(v, v) = v, a NaN included. Radial-speed bounds holding a NaN take
+// the point's speed.
+__device__ __forceinline__ void addPoint(const float* o, float* nb, bool fresh, float d, float a, float r, float e)
+{
+	const float v[4] = {d, a, r, e}; // pairs (kDMin, kDMax), (kAMin, kAMax), (kRMin, kRMax), (kEMin, kEMax)
+	for (int k = 0; k < 4; ++k) {
+		const float lo = fresh ? v[k] : o[2 * k], hi = fresh ? v[k] : o[2 * k + 1];
+		if (2 * k == kRMin) {
+			nb[2 * k] = isnan(lo) ? v[k] : fmin2(lo, v[k]);
+			nb[2 * k + 1] = isnan(hi) ? v[k] : fmax2(hi, v[k]);
+		} else {
+			nb[2 * k] = fmin2(lo, v[k]);
+			nb[2 * k + 1] = fmax2(hi, v[k]);
+		}
+	}
+}
 
 // The node's distance scopes, scope s held by lane s. Must be used by all 64 lanes together.
 struct LaneScopes
@@ -145,14 +167,17 @@ struct LaneScopes
 // so after merging (ci, cj) the next first mergeable pair is either (a, ci) with a < ci or in row ci or later: the
 // scan resumes there and finds the same pair. A cluster's member order is the reference's: the phase-1 clusters it
 // absorbed form a chain, each contributing its points in index order.
-// Phase 3: per cluster (one lane each) the scattered-field sums in member order, the directional centre (:379-398
-// of RadarCluster), RCS, power, noise and SNR (:128-174).
+// Phase 3: per cluster the scattered-field sums over its members (the wave across one cluster at a time), the
+// directional centre (:379-398 of RadarCluster), then per cluster (one lane each) RCS, power, noise and SNR
+// (:128-174).
 __device__ void clusterOne(const RadarItem& it)
 {
 	__shared__ float b[kBounds][kRadarMaxClusters];
 	__shared__ int32_t segHead[kRadarMaxClusters], segTail[kRadarMaxClusters]; // per current cluster
 	__shared__ int32_t segNext[kRadarMaxClusters], size[kRadarMaxClusters];    // per phase-1 cluster
 	__shared__ int32_t offset[kRadarMaxClusters];
+	__shared__ float4 sums[kRadarMaxClusters];      // per final cluster: scattered-field sums (U re, im, R re, im)
+	__shared__ int32_t centres[kRadarMaxClusters];  // per final cluster: its centre point
 	const int lane = threadIdx.x;
 	const int capacity = it.capacity;
 	// Uniform across the wave; readfirstlane keeps the loops below on scalar branches.
@@ -180,6 +205,7 @@ __device__ void clusterOne(const RadarItem& it)
 	int32_t errors = 0;
 	float rb[kBounds]; // bounds of cluster `lane` while it is one of the first 64
 	int rsize = 0;     // its size
+	uint64_t held = 0; // lanes holding a cluster: min(clusters, 64) low bits
 	for (int q = 0; q < kBounds; ++q) {
 		rb[q] = 0.0f;
 	}
@@ -195,96 +221,97 @@ __device__ void clusterOne(const RadarItem& it)
 		}
 		float pdt, prt, pat;
 		const int ps = scopes.find(pd, pdt, prt, pat);
+		// The ends of each point's acceptance windows, by the same additions the per-cluster test would make.
+		const float dHi = pd + pdt, dLo = pd - pdt, aHi = pa + pat, aLo = pa - pat, rHi = pr + prt, rLo = pr - prt;
 		int32_t myCluster = -1, myRank = 0; // lane j records point base + j
-		const int m = min(64, n - base);
-		for (int j = 0; j < m; ++j) {
-			if (lanei(ps, j) < 0) {
-				continue;
-			}
-			const float d = lanef(pd, j), a = lanef(pa, j), r = lanef(pr, j), e = lanef(pe, j);
-			const float dt = lanef(pdt, j), rt = lanef(prt, j), at = lanef(pat, j);
-			auto accepts = [&](float dlo, float dhi, float alo, float ahi, float rlo, float rhi) {
-				const bool dOk = d + dt >= dlo && d - dt <= dhi;
-				const bool aOk = a + at >= alo && a - at <= ahi;
-				const bool rOk = (isnan(rlo) || isnan(rhi)) || (r + rt >= rlo && r - rt <= rhi);
-				return dOk && aOk && rOk;
-			};
-			int found = -1;
-			{
-				const bool cand = lane < clusters && accepts(rb[kDMin], rb[kDMax], rb[kAMin], rb[kAMax], rb[kRMin], rb[kRMax]);
-				const uint64_t mask = __ballot(cand);
-				if (mask != 0) {
-					found = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+		// isCandidate for point j against clusters 0..63 (one per lane) as a lane mask: one compare per bound, the
+		// masks combined on the scalar unit. Radial-speed bounds holding a NaN accept any speed.
+		auto heldAccepting = [&](int j) -> uint64_t {
+			const float sdHi = lanef(dHi, j), sdLo = lanef(dLo, j), saHi = lanef(aHi, j), saLo = lanef(aLo, j);
+			const float srHi = lanef(rHi, j), srLo = lanef(rLo, j);
+			const uint64_t dOk = __builtin_amdgcn_fcmpf(sdHi, rb[kDMin], kOGE) & __builtin_amdgcn_fcmpf(sdLo, rb[kDMax], kOLE);
+			const uint64_t aOk = __builtin_amdgcn_fcmpf(saHi, rb[kAMin], kOGE) & __builtin_amdgcn_fcmpf(saLo, rb[kAMax], kOLE);
+			const uint64_t rOk = __builtin_amdgcn_fcmpf(rb[kRMin], rb[kRMax], kUNO) |
+			                     (__builtin_amdgcn_fcmpf(srHi, rb[kRMin], kOGE) & __builtin_amdgcn_fcmpf(srLo, rb[kRMax], kOLE));
+			return dOk & aOk & rOk & held;
+		};
+		// Points outside every scope join no cluster; the others are walked in order, one per step. The inner loop
+		// takes the common step (the point joins a cluster held in registers, whose lane grows its bounds) without
+		// leaving it; a point that no such cluster accepts goes through the general step below.
+		uint64_t pending = __builtin_amdgcn_ballot_w64(pi < n && ps >= 0);
+		while (pending != 0) {
+			int j = __builtin_ctzll(pending);
+			for (uint64_t hit = heldAccepting(j); hit != 0;) {
+				const int c = __builtin_ctzll(hit);
+				const float d = lanef(pd, j), a = lanef(pa, j), r = lanef(pr, j), e = lanef(pe, j);
+				const int rank = lanei(rsize, c);
+				if (lane == c) {
+					addPoint(rb, rb, false, d, a, r, e);
+					rsize = rank + 1;
 				}
+				myCluster = lane == j ? c : myCluster;
+				myRank = lane == j ? rank : myRank;
+				pending &= pending - 1;
+				if (pending == 0) {
+					break;
+				}
+				j = __builtin_ctzll(pending);
+				hit = heldAccepting(j);
 			}
+			if (pending == 0) {
+				break;
+			}
+			// General step for point j: the first accepting cluster among those held in LDS, else a new cluster.
+			int found = -1;
 			for (int cb = 64; cb < clusters && found < 0; cb += 64) {
 				const int c = cb + lane;
 				const int cc = c < clusters ? c : 0;
-				const bool cand = c < clusters && accepts(b[kDMin][cc], b[kDMax][cc], b[kAMin][cc], b[kAMax][cc],
-				                                          b[kRMin][cc], b[kRMax][cc]);
-				const uint64_t mask = __ballot(cand);
+				const float sdHi = lanef(dHi, j), sdLo = lanef(dLo, j), saHi = lanef(aHi, j), saLo = lanef(aLo, j);
+				const float srHi = lanef(rHi, j), srLo = lanef(rLo, j);
+				const float rlo = b[kRMin][cc], rhi = b[kRMax][cc];
+				const bool rOk = __builtin_isunordered(rlo, rhi) || (srHi >= rlo && srLo <= rhi);
+				const bool cand = c < clusters && sdHi >= b[kDMin][cc] && sdLo <= b[kDMax][cc] && saHi >= b[kAMin][cc] &&
+				                  saLo <= b[kAMax][cc] && rOk;
+				const uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
 				if (mask != 0) {
-					found = cb + __ffsll(static_cast<unsigned long long>(mask)) - 1;
+					found = cb + __builtin_ctzll(mask);
 				}
 			}
 			if (found < 0 && clusters == kRadarMaxClusters) {
 				errors |= kRadarErrTooManyClusters;
 				break;
 			}
-			const int c = found >= 0 ? found : clusters;
-			int rank;
-			if (c < 64) {
-				rank = found >= 0 ? lanei(rsize, c) : 0;
+			const float d = lanef(pd, j), a = lanef(pa, j), r = lanef(pr, j), e = lanef(pe, j);
+			const bool fresh = found < 0;
+			const int c = fresh ? clusters : found;
+			const int rank = fresh ? 0 : __builtin_amdgcn_readfirstlane(lane == 0 ? size[c] : 0);
+			if (c < 64) { // a new cluster held in registers
 				if (lane == c) {
-					if (found >= 0) {
-						rb[kDMin] = fmin2(rb[kDMin], d);
-						rb[kDMax] = fmax2(rb[kDMax], d);
-						rb[kAMin] = fmin2(rb[kAMin], a);
-						rb[kAMax] = fmax2(rb[kAMax], a);
-						rb[kRMin] = isnan(rb[kRMin]) ? r : fmin2(rb[kRMin], r);
-						rb[kRMax] = isnan(rb[kRMax]) ? r : fmax2(rb[kRMax], r);
-						rb[kEMin] = fmin2(rb[kEMin], e);
-						rb[kEMax] = fmax2(rb[kEMax], e);
-					} else {
-						rb[kDMin] = rb[kDMax] = d;
-						rb[kAMin] = rb[kAMax] = a;
-						rb[kRMin] = rb[kRMax] = r;
-						rb[kEMin] = rb[kEMax] = e;
-					}
-					rsize = rank + 1;
+					addPoint(rb, rb, true, d, a, r, e);
+					rsize = 1;
 				}
-			} else {
-				rank = found >= 0 ? __builtin_amdgcn_readfirstlane(lane == 0 ? size[c] : 0) : 0;
-				if (lane == 0) {
-					if (found >= 0) {
-						b[kDMin][c] = fmin2(b[kDMin][c], d);
-						b[kDMax][c] = fmax2(b[kDMax][c], d);
-						b[kAMin][c] = fmin2(b[kAMin][c], a);
-						b[kAMax][c] = fmax2(b[kAMax][c], a);
-						b[kRMin][c] = isnan(b[kRMin][c]) ? r : fmin2(b[kRMin][c], r);
-						b[kRMax][c] = isnan(b[kRMax][c]) ? r : fmax2(b[kRMax][c], r);
-						b[kEMin][c] = fmin2(b[kEMin][c], e);
-						b[kEMax][c] = fmax2(b[kEMax][c], e);
-					} else {
-						b[kDMin][c] = b[kDMax][c] = d;
-						b[kAMin][c] = b[kAMax][c] = a;
-						b[kRMin][c] = b[kRMax][c] = r;
-						b[kEMin][c] = b[kEMax][c] = e;
-					}
-					size[c] = rank + 1;
+			} else if (lane == 0) {
+				float lb[kBounds];
+				for (int q = 0; q < kBounds; ++q) {
+					lb[q] = b[q][c];
 				}
+				addPoint(lb, lb, fresh, d, a, r, e);
+				for (int q = 0; q < kBounds; ++q) {
+					b[q][c] = lb[q];
+				}
+				size[c] = rank + 1;
 			}
-			if (found < 0) {
+			if (fresh) {
 				if (lane == 0) {
 					segHead[c] = segTail[c] = c;
 					segNext[c] = -1;
 				}
 				++clusters;
+				held = clusters >= 64 ? ~0ull : (1ull << clusters) - 1;
 			}
-			if (lane == j) {
-				myCluster = c;
-				myRank = rank;
-			}
+			myCluster = lane == j ? c : myCluster;
+			myRank = lane == j ? rank : myRank;
+			pending &= pending - 1;
 			__builtin_amdgcn_wave_barrier();
 		}
 		if (pi < n) {
@@ -420,35 +447,79 @@ __device__ void clusterOne(const RadarItem& it)
 	__threadfence(); // members written above are read by other lanes below
 	__syncthreads();
 
+	// Cluster sums, one cluster at a time across the wave: lane l takes list positions l, l + 64, ... of the cluster's
+	// members (four positions' loads issued together), and the 64 partial sums meet in a butterfly. The centre is the
+	// first point in list order at the smallest |azimuth - mean| + |elevation - mean| (the first member when no
+	// distance is below FLT_MAX): each lane keeps its own first minimum, the wave takes the least (distance,
+	// position) pair. One lane per cluster walking its list would chain two dependent global loads per member.
+	const float4* terms = reinterpret_cast<const float4*>(it.terms);
+	for (int c = 0; c < clusters && errors == 0; ++c) {
+		const float meanA = (b[kAMin][c] + b[kAMax][c]) / 2.0f;
+		const float meanE = (b[kEMin][c] + b[kEMax][c]) / 2.0f;
+		Cf au{0.0f, 0.0f}, ar{0.0f, 0.0f};
+		float best = FLT_MAX;
+		int bestPos = INT_MAX, bestPt = -1;
+		int pos = 0; // list position of the segment's first member
+		for (int o = segHead[c]; o >= 0; o = segNext[o]) {
+			const int first = offset[o], count = size[o];
+			for (int k0 = 0; k0 < count; k0 += 4 * 64) {
+				int pt[4];
+				float4 t[4];
+				float v[4];
+				for (int u = 0; u < 4; ++u) {
+					const int k = k0 + u * 64 + lane;
+					pt[u] = k < count ? members[first + k] : -1;
+				}
+				for (int u = 0; u < 4; ++u) {
+					if (pt[u] >= 0) {
+						t[u] = terms[pt[u]];
+						v[u] = fabsf(azimuth[pt[u]] - meanA) + fabsf(elevation[pt[u]] - meanE);
+					}
+				}
+				for (int u = 0; u < 4; ++u) {
+					if (pt[u] >= 0) {
+						au = au + Cf{t[u].x, t[u].y};
+						ar = ar + Cf{t[u].z, t[u].w};
+						if (v[u] < best) {
+							best = v[u];
+							bestPos = pos + k0 + u * 64 + lane;
+							bestPt = pt[u];
+						}
+					}
+				}
+			}
+			pos += count;
+		}
+		// Butterfly: both partners add the same two values, so every lane ends with the same sums.
+		for (int m = 1; m < 64; m <<= 1) {
+			au.re += __shfl_xor(au.re, m);
+			au.im += __shfl_xor(au.im, m);
+			ar.re += __shfl_xor(ar.re, m);
+			ar.im += __shfl_xor(ar.im, m);
+			const float ob = __shfl_xor(best, m);
+			const int op = __shfl_xor(bestPos, m), opt = __shfl_xor(bestPt, m);
+			if (ob < best || (ob == best && op < bestPos)) {
+				best = ob;
+				bestPos = op;
+				bestPt = opt;
+			}
+		}
+		if (lane == 0) {
+			sums[c] = make_float4(au.re, au.im, ar.re, ar.im);
+			centres[c] = bestPt >= 0 ? bestPt : members[offset[segHead[c]]];
+		}
+	}
+	__syncthreads();
+
 	const RadarPostParams& p = it.p;
 	const float lambda = 299792458.0f / p.frequency;
 	const float lambdaSqrtDbsm = 10.0f * log10f(lambda * lambda);
 	const float fourPiCubedDb = 10.0f * log10f(powf(4.0f * kPi, 3.0f));
 	bool rcsNan = false;
 	for (int c = lane; c < clusters && errors == 0; c += 64) {
-		const float meanA = (b[kAMin][c] + b[kAMax][c]) / 2.0f;
-		const float meanE = (b[kEMin][c] + b[kEMax][c]) / 2.0f;
-		Cf au{0.0f, 0.0f}, ar{0.0f, 0.0f};
-		float best = FLT_MAX;
-		int centre = -1;
-		for (int o = segHead[c]; o >= 0; o = segNext[o]) {
-			const int end = offset[o] + size[o];
-#pragma unroll 4
-			for (int k = offset[o]; k < end; ++k) {
-				const int pt = members[k];
-				const float4 t = reinterpret_cast<const float4*>(it.terms)[pt];
-				au = au + Cf{t.x, t.y};
-				ar = ar + Cf{t.z, t.w};
-				const float v = fabsf(azimuth[pt] - meanA) + fabsf(elevation[pt] - meanE);
-				if (centre < 0) {
-					centre = pt;
-				}
-				if (v < best) {
-					best = v;
-					centre = pt;
-				}
-			}
-		}
+		const float4 s = sums[c];
+		const Cf au{s.x, s.y}, ar{s.z, s.w};
+		const int centre = centres[c];
 		const float mu = hypotf(au.re, au.im), mr = hypotf(ar.re, ar.im);
 		const float rcsDb = 10.0f * log10f(4.0f * kPi * (mu * mu + mr * mr));
 		const float d = distance[centre];

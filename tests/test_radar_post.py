@@ -6,7 +6,9 @@ flat-plate cross-section sigma = 4 pi A^2 / lambda^2 at normal incidence (invari
 polarisation about the ray), and its received power against the radar equation.
 GPU: the library's node on the same known answers through rgl_node_points_from_array, and on raytraced radar frames
 against the oracle: identical clusters (count, membership through the centre point's fields), RCS within 1e-3 dB,
-power within 1e-3 dB plus the node's noise, which is replayed from the library's seeded normal stream.
+power within 1e-3 dB plus the node's noise, which is replayed from the library's seeded normal stream; more clusters
+than a wave has lanes (joins to clusters kept in LDS, NaN radial-speed bounds) against the oracle, and the node's
+cluster limit reported as an error.
 """
 import ctypes as C
 import math
@@ -136,6 +138,58 @@ def test_gpu_clustering_known_answers(rgl_lib):
             np.testing.assert_array_equal(rgl.result(radar, XYZ), rec["xyz"][centres])
         rgl.check("rgl_graph_node_remove_child", src.h, radar.h)
         rgl.check("rgl_graph_destroy", src.h)
+
+
+def many_clusters(k, seed):
+    """k clusters far apart in azimuth (one point each, in order), then a second point for each in shuffled order;
+    every 7th cluster starts with a NaN radial speed, which the joining point's speed replaces."""
+    rng = np.random.default_rng(seed)
+    first_r = np.where(np.arange(k) % 7 == 3, np.nan, 1.0)
+    order = rng.permutation(k)
+    d = np.concatenate([np.full(k, 5.0), np.full(k, 5.5)])
+    a = np.concatenate([10.0 * np.arange(k), 10.0 * order + 0.5])
+    r = np.concatenate([first_r, np.full(k, 1.2)])
+    return d, a, r
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [63, 64, 65, 300])
+def test_gpu_clustering_beyond_register_clusters(rgl_lib, k):
+    """More clusters than one per lane: later points join clusters held in LDS, in the reference's order."""
+    rgl = rgl_lib
+    d, a, r = many_clusters(k, seed=k)
+    rec, fields = records(d, a, r, seed=k)
+    cl = RP.cluster(d, a, r, rec["e"], SCOPES)
+    assert len(cl) == k and all(len(c.idx) == 2 for c in cl)
+    centres = [RP.centre_index(c, a, rec["e"]) for c in cl]
+    radar = rgl.node_points_radar_postprocess(rgl.Node(), SCOPES, *PARAMS)
+    src = rgl.node_points_from_array(rgl.Node(), rec, fields)
+    rgl.add_child(src, radar)
+    rgl.graph_run(radar)
+    assert rgl.result_size(radar, DIST)[0] == k
+    np.testing.assert_array_equal(rgl.result(radar, DIST), rec["d"][centres])
+    np.testing.assert_array_equal(rgl.result(radar, XYZ), rec["xyz"][centres])
+    rgl.check("rgl_graph_node_remove_child", src.h, radar.h)
+    rgl.check("rgl_graph_destroy", src.h)
+
+
+@pytest.mark.gpu
+def test_gpu_clustering_cluster_limit(rgl_lib):
+    """1025 clusters exceed the node's limit of 1024: the run reports it instead of returning a partial result."""
+    rgl = rgl_lib
+    k = 1025
+    rec, fields = records(np.full(k, 5.0), 10.0 * np.arange(k), np.ones(k), seed=1)
+    radar = rgl.node_points_radar_postprocess(rgl.Node(), SCOPES, *PARAMS)
+    src = rgl.node_points_from_array(rgl.Node(), rec, fields)
+    rgl.add_child(src, radar)
+    st = rgl.call("rgl_graph_run", radar.h)
+    if st == 0:
+        cnt, size = C.c_int32(), C.c_int32()
+        st = rgl.call("rgl_graph_get_result_size", radar.h, DIST, C.byref(cnt), C.byref(size))
+    assert st == rgl.RGL_INVALID_PIPELINE, st
+    assert "at most 1024 clusters" in rgl.last_error()
+    rgl.check("rgl_graph_node_remove_child", src.h, radar.h)
+    rgl.check("rgl_graph_destroy", src.h)
 
 
 def radar_frame_graph(rgl, scene_spec, frames=2):
