@@ -206,6 +206,10 @@ __device__ void clusterOne(const RadarItem& it)
 	float rb[kBounds]; // bounds of cluster `lane` while it is one of the first 64
 	int rsize = 0;     // its size
 	uint64_t held = 0; // lanes holding a cluster: min(clusters, 64) low bits
+	// -inf and +inf in registers the optimizer cannot see into, so that a median against them stays one instruction
+	// rather than being turned back into a min/max with operand canonicalisation.
+	float negInf = -INFINITY, posInf = INFINITY;
+	asm volatile("" : "+v"(negInf), "+v"(posInf));
 	for (int q = 0; q < kBounds; ++q) {
 		rb[q] = 0.0f;
 	}
@@ -246,12 +250,25 @@ __device__ void clusterOne(const RadarItem& it)
 				const float d = lanef(pd, j), a = lanef(pa, j), r = lanef(pr, j), e = lanef(pe, j);
 				const int rank = lanei(rsize, c);
 				if (lane == c) {
-					addPoint(rb, rb, false, d, a, r, e);
+					// Distance and azimuth passed ordered compares against these bounds, so none of the four is a
+					// NaN and a median against -inf / +inf gives std::min / std::max (a tie of +0 and -0 may keep
+					// either zero, which no comparison, difference or midpoint made from the bounds tells apart).
+					rb[kDMin] = __builtin_amdgcn_fmed3f(rb[kDMin], d, negInf);
+					rb[kDMax] = __builtin_amdgcn_fmed3f(rb[kDMax], d, posInf);
+					rb[kAMin] = __builtin_amdgcn_fmed3f(rb[kAMin], a, negInf);
+					rb[kAMax] = __builtin_amdgcn_fmed3f(rb[kAMax], a, posInf);
+					// A cluster's two radial-speed bounds are NaN together or not at all, and a point with a NaN speed
+					// passes the radial test only against NaN bounds: so "a NaN bound takes the speed, else std::min"
+					// is "keep the bound if it is <= the speed, else take the speed" (>= for the upper bound).
+					rb[kRMin] = rb[kRMin] <= r ? rb[kRMin] : r;
+					rb[kRMax] = rb[kRMax] >= r ? rb[kRMax] : r;
+					rb[kEMin] = fmin2(rb[kEMin], e);
+					rb[kEMax] = fmax2(rb[kEMax], e);
 					rsize = rank + 1;
 				}
 				myCluster = lane == j ? c : myCluster;
 				myRank = lane == j ? rank : myRank;
-				pending &= pending - 1;
+				pending ^= 1ull << j;
 				if (pending == 0) {
 					break;
 				}
