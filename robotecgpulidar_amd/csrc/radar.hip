@@ -115,6 +115,34 @@ __device__ __forceinline__ int lanei(int v, int j) { return __builtin_amdgcn_rea
 // Predicates of __builtin_amdgcn_fcmpf (LLVM's compare codes): ordered >=, ordered <=, unordered.
 constexpr int kOGE = 3, kOLE = 5, kUNO = 8;
 
+// Data-parallel-primitive controls (DPP, GFX9 encoding): shift right by 1/2/4/8 lanes within rows of 16, broadcast
+// lane 15 of a row to the next row, lane 31 to rows 2 and 3, and shift the whole wave right by one lane.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// v from the lane the DPP control names, or `id` where there is none (or the row is masked off).
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dppOr(float id, float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(v), kCtrl, kRowMask, 0xf, false));
+}
+
+// Inclusive scan of the wave's values, minimum or maximum (NaN-free values): within each row of 16, then the totals
+// of rows 0 and 2 into rows 1 and 3, then that of rows 0..1 into rows 2 and 3.
+template <bool kMax>
+__device__ __forceinline__ float scanWave(float v)
+{
+	const float id = kMax ? -INFINITY : INFINITY;
+	auto op = [](float a, float b) { return kMax ? (a < b ? b : a) : (b < a ? b : a); };
+	v = op(v, dppOr<kRowShr1>(id, v));
+	v = op(v, dppOr<kRowShr2>(id, v));
+	v = op(v, dppOr<kRowShr4>(id, v));
+	v = op(v, dppOr<kRowShr8>(id, v));
+	v = op(v, dppOr<kRowBcast15, 0xa>(id, v));
+	v = op(v, dppOr<kRowBcast31, 0xc>(id, v));
+	return v;
+}
+
 // Bounds `nb` of a cluster with bounds `o` after adding point (d, a, r, e) (RadarCluster::addPoint,
 // src/graph/RadarPostprocessPointsNode.cpp:274-291), or of a new cluster holding only that point when `fresh`
 // (the constructor, :264-272): fmin2(v, v) = fmax2(v, v) = v, a NaN included. Radial-speed bounds holding a NaN take
@@ -243,8 +271,145 @@ __device__ void clusterOne(const RadarItem& it)
 		// takes the common step (the point joins a cluster held in registers, whose lane grows its bounds) without
 		// leaving it; a point that no such cluster accepts goes through the general step below.
 		uint64_t pending = __builtin_amdgcn_ballot_w64(pi < n && ps >= 0);
+		// Chunk step. Bounds only widen (minima fall, maxima rise), and a cluster's radial-speed bounds stay NaN until
+		// a finite speed joins and stay finite after; so, as long as those bounds do not turn finite, the points a
+		// cluster accepts only grow in number. Let f0(p) be the first cluster accepting point p at the start of the
+		// step. If every earlier point q of the chunk joined f0(q), the walk's clusters before p are the starting
+		// ones grown by those points, and p joins f0(p) unless a cluster below f0(p) that earlier points grew now
+		// accepts it (clusters not grown still refuse it), f0(p) does not exist (p opens a cluster), or an earlier
+		// point turned a grown cluster's radial-speed bounds finite. The step finds the first point where one of
+		// these happens, from an inclusive scan of each grown cluster's members over the wave, and commits every
+		// point before it. The walk then takes that point, and the step is tried again on the rest of the chunk
+		// while a try commits at least kMinRun points.
+		constexpr int kMinRun = 16;
+		auto below = [](int q) { return q >= 64 ? ~0ull : (1ull << q) - 1; };
+		// The members' values, or the identity of each minimum (+inf) and maximum (-inf) for other lanes. A member's
+		// distance and azimuth passed ordered compares, so they are not NaN, nor are radial speeds against finite
+		// bounds; NaN elevations and speeds are left out, as std::min / std::max against a number leave them out.
+		auto memberValues = [&](bool member, float* v) {
+			const bool rNum = member && !isnan(pr), eNum = member && !isnan(pe);
+			v[kDMin] = member ? pd : INFINITY;
+			v[kDMax] = member ? pd : -INFINITY;
+			v[kAMin] = member ? pa : INFINITY;
+			v[kAMax] = member ? pa : -INFINITY;
+			v[kRMin] = rNum ? pr : INFINITY;
+			v[kRMax] = rNum ? pr : -INFINITY;
+			v[kEMin] = eNum ? pe : INFINITY;
+			v[kEMax] = eNum ? pe : -INFINITY;
+		};
+		// Bounds x grown by the values v. NaN radial-speed bounds take the speed among v when `rTook` says there is
+		// one (there is at most one: a second would follow the point that turned the bounds finite); NaN elevation
+		// bounds stay NaN, as in the walk.
+		auto grow = [&](const float* x, const float* v, bool rTook, float* nb) {
+			const bool rNan = __builtin_isunordered(x[kRMin], x[kRMax]);
+			for (int q = 0; q < kBounds; q += 2) {
+				nb[q] = fmin2(x[q], v[q]);
+				nb[q + 1] = fmax2(x[q + 1], v[q + 1]);
+			}
+			nb[kRMin] = rNan ? (rTook ? v[kRMin] : x[kRMin]) : nb[kRMin];
+			nb[kRMax] = rNan ? (rTook ? v[kRMax] : x[kRMax]) : nb[kRMax];
+		};
+		auto accepts = [&](const float* nb) {
+			const bool rOk = __builtin_isunordered(nb[kRMin], nb[kRMax]) | ((rHi >= nb[kRMin]) & (rLo <= nb[kRMax]));
+			return (dHi >= nb[kDMin]) & (dLo <= nb[kDMax]) & (aHi >= nb[kAMin]) & (aLo <= nb[kAMax]) & rOk;
+		};
+		auto chunkStep = [&](uint64_t pending) -> uint64_t {
+			const bool live = (pending >> lane) & 1;
+			// f0 cluster by cluster for all points at once (until every live point has one), or point by point
+			// against all clusters at once, whichever takes fewer steps.
+			int f0 = -1;
+			if (clusters <= __builtin_popcountll(pending)) {
+				for (int k = 0; k < clusters; ++k) {
+					float x[kBounds];
+					for (int q = 0; q < kBounds; ++q) {
+						x[q] = lanef(rb[q], k);
+					}
+					f0 = ((f0 < 0) & accepts(x)) ? k : f0;
+					if ((__builtin_amdgcn_ballot_w64(f0 < 0) & pending) == 0) {
+						break;
+					}
+				}
+			} else {
+				for (uint64_t t = pending; t != 0; t &= t - 1) {
+					const int j = __builtin_ctzll(t);
+					const uint64_t hit = heldAccepting(j);
+					f0 = lane == j ? (hit != 0 ? __builtin_ctzll(hit) : -1) : f0;
+				}
+			}
+			const uint64_t none = __builtin_amdgcn_ballot_w64(f0 < 0) & pending;
+			int stop = none != 0 ? __builtin_ctzll(none) : 64; // the first point the walk must take
+			for (uint64_t todo = pending & below(stop); todo != 0 && __builtin_ctzll(todo) < stop;) {
+				const int k = lanei(f0, __builtin_ctzll(todo));
+				const bool member = live && f0 == k;
+				todo &= ~__builtin_amdgcn_ballot_w64(member);
+				float x[kBounds], v[kBounds];
+				for (int q = 0; q < kBounds; ++q) {
+					x[q] = lanef(rb[q], k);
+				}
+				// A speed that turns NaN bounds finite ends the run after its point; no lane before it has one
+				// among its earlier members, so the checks below keep the NaN bounds.
+				if (__builtin_isunordered(x[kRMin], x[kRMax])) {
+					const uint64_t finite = __builtin_amdgcn_ballot_w64(member && !isnan(pr));
+					stop = finite != 0 ? min(stop, __builtin_ctzll(finite) + 1) : stop;
+				}
+				memberValues(member, v);
+				for (int q = 0; q < kBounds; q += 2) { // members before this lane: inclusive scan shifted by one
+					v[q] = dppOr<kWaveShr1>(INFINITY, scanWave<false>(v[q]));
+					v[q + 1] = dppOr<kWaveShr1>(-INFINITY, scanWave<true>(v[q + 1]));
+				}
+				float nb[kBounds];
+				grow(x, v, false, nb);
+				const uint64_t taken = __builtin_amdgcn_ballot_w64(live & (f0 > k) & accepts(nb));
+				stop = taken != 0 ? min(stop, __builtin_ctzll(taken)) : stop;
+			}
+			// Commit the points before `stop`: each grown cluster's bounds, size, and its members' ranks.
+			const uint64_t done = pending & below(stop);
+			const bool committed = (done >> lane) & 1;
+			for (uint64_t todo = done; todo != 0;) {
+				const int k = lanei(f0, __builtin_ctzll(todo));
+				const bool member = committed && f0 == k;
+				const uint64_t mk = __builtin_amdgcn_ballot_w64(member);
+				todo &= ~mk;
+				float x[kBounds], v[kBounds];
+				for (int q = 0; q < kBounds; ++q) {
+					x[q] = lanef(rb[q], k);
+				}
+				const bool rTook = __builtin_amdgcn_ballot_w64(member && !isnan(pr)) != 0;
+				memberValues(member, v);
+				for (int q = 0; q < kBounds; q += 2) { // the members' minima and maxima: the scans' last lane
+					v[q] = lanef(scanWave<false>(v[q]), 63);
+					v[q + 1] = lanef(scanWave<true>(v[q + 1]), 63);
+				}
+				float nb[kBounds];
+				grow(x, v, rTook, nb);
+				const int size0 = lanei(rsize, k);
+				if (lane == k) {
+					for (int q = 0; q < kBounds; ++q) {
+						rb[q] = nb[q];
+					}
+					rsize = size0 + __builtin_popcountll(mk);
+				}
+				const int before = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mk >> 32),
+				                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u)));
+				myCluster = member ? k : myCluster;
+				myRank = member ? size0 + before : myRank;
+			}
+			return done;
+		};
+		bool tryChunk = true;
 		while (pending != 0) {
+			tryChunk = tryChunk && clusters <= 64;
+			if (tryChunk) {
+				const uint64_t done = chunkStep(pending);
+				pending &= ~done;
+				tryChunk = __builtin_popcountll(done) >= kMinRun;
+				if (pending == 0) {
+					break;
+				}
+			}
+			// The walk: one point while chunk steps are being tried, else the rest of the chunk.
 			int j = __builtin_ctzll(pending);
+			bool joined = false;
 			for (uint64_t hit = heldAccepting(j); hit != 0;) {
 				const int c = __builtin_ctzll(hit);
 				const float d = lanef(pd, j), a = lanef(pa, j), r = lanef(pr, j), e = lanef(pe, j);
@@ -269,7 +434,8 @@ __device__ void clusterOne(const RadarItem& it)
 				myCluster = lane == j ? c : myCluster;
 				myRank = lane == j ? rank : myRank;
 				pending ^= 1ull << j;
-				if (pending == 0) {
+				joined = true;
+				if (pending == 0 || tryChunk) {
 					break;
 				}
 				j = __builtin_ctzll(pending);
@@ -277,6 +443,9 @@ __device__ void clusterOne(const RadarItem& it)
 			}
 			if (pending == 0) {
 				break;
+			}
+			if (joined && tryChunk) {
+				continue;
 			}
 			// General step for point j: the first accepting cluster among those held in LDS, else a new cluster.
 			int found = -1;

@@ -173,6 +173,51 @@ def test_gpu_clustering_beyond_register_clusters(rgl_lib, k):
     rgl.check("rgl_graph_destroy", src.h)
 
 
+def interleaved(n, seed, nan_speeds):
+    """n points drawn from 12 objects in random order, spread over both scopes and up to the thresholds, so that
+    clusters grow towards each other, points of one object land in several clusters, and a later point can be taken
+    by a cluster that an earlier one widened; with nan_speeds, radial speeds are NaN for whole objects, for random
+    points or for all."""
+    rng = np.random.default_rng(seed)
+    k = 12
+    cd, ca, cr = rng.uniform(1.0, 19.0, k), rng.uniform(0.0, 90.0, k), rng.uniform(-6.0, 6.0, k)
+    obj = rng.integers(0, k, n)
+    if seed % 2:  # runs of one object, as a scan crosses it
+        obj = np.repeat(obj[: (n + 15) // 16], 16)[:n]
+    d = cd[obj] + rng.normal(0.0, 0.8, n)
+    a = ca[obj] + rng.normal(0.0, 1.5, n)
+    r = cr[obj] + rng.normal(0.0, 0.8, n)
+    if nan_speeds == "objects":
+        r[obj % 3 == 0] = np.nan
+    elif nan_speeds == "points":  # clusters opened by a NaN speed, whose bounds a later finite speed replaces
+        r[rng.random(n) < 0.3] = np.nan
+    elif nan_speeds == "all":
+        r[:] = np.nan
+    return d.astype(np.float32), a.astype(np.float32), r.astype(np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,nan_speeds", [(0, None), (1, None), (2, None), (3, None), (4, "objects"),
+                                             (5, "objects"), (6, "all"), (7, "all"), (8, "points"), (9, "points")])
+def test_gpu_clustering_interleaved_matches_oracle(rgl_lib, seed, nan_speeds):
+    """Interleaved, overlapping clusters: the same clusters (count, centres bit for bit) as the oracle."""
+    rgl = rgl_lib
+    d, a, r = interleaved(3000, seed, nan_speeds)
+    rec, fields = records(d, a, r, seed=seed)
+    cl = RP.cluster(d, a, r, rec["e"], SCOPES)
+    centres = [RP.centre_index(c, a, rec["e"]) for c in cl]
+    radar = rgl.node_points_radar_postprocess(rgl.Node(), SCOPES, *PARAMS)
+    src = rgl.node_points_from_array(rgl.Node(), rec, fields)
+    rgl.add_child(src, radar)
+    rgl.graph_run(radar)
+    print(f"seed {seed}: {len(cl)} clusters, sizes {sorted((len(c.idx) for c in cl), reverse=True)[:8]}")
+    assert rgl.result_size(radar, DIST)[0] == len(cl)
+    np.testing.assert_array_equal(rgl.result(radar, DIST), rec["d"][centres])
+    np.testing.assert_array_equal(rgl.result(radar, XYZ), rec["xyz"][centres])
+    rgl.check("rgl_graph_node_remove_child", src.h, radar.h)
+    rgl.check("rgl_graph_destroy", src.h)
+
+
 @pytest.mark.gpu
 def test_gpu_clustering_cluster_limit(rgl_lib):
     """1025 clusters exceed the node's limit of 1024: the run reports it instead of returning a partial result."""
