@@ -4,9 +4,10 @@
 //
 // The reference runs the clustering on the host after copying every input field back (a synchronisation inside the
 // graph). Here it stays on the device: the clustering is sequential in point order (a point joins the first
-// cluster, in creation order, that accepts it), so one wave walks the points and tests 64 clusters per step, with
-// cluster bounds and list ends in LDS; the merge phase tests 64 cluster pairs per step in the reference's pair
-// order. Per-cluster sums then run with the wave spread over one cluster's members at a time.
+// cluster, in creation order, that accepts it), so one wave assigns 64 points per step where the order provably
+// does not change the answer and walks the others one at a time, testing 64 clusters per step; the merge phase
+// tests 64 cluster pairs per step in the reference's pair order. Per-cluster sums then run with the wave spread over
+// one cluster's members at a time.
 #include <algorithm>
 #include <cfloat>
 #include <climits>
@@ -187,9 +188,9 @@ struct LaneScopes
 // One wave (a 64-thread block) clusters one node's detections.
 // Phase 1: every point, in order, joins the first accepting cluster or opens a new one
 // (src/graph/RadarPostprocessPointsNode.cpp:74-98, RadarCluster::isCandidate :324-348). Points are fetched 64 at a
-// time with coalesced loads, their scopes found in parallel, and broadcast from registers; the bounds and sizes of
-// clusters 0..63 live in the registers of lane c, later clusters in LDS. Each point records its cluster and its
-// rank in it.
+// time with coalesced loads and their scopes found in parallel; the bounds and sizes of clusters 0..63 live in the
+// registers of lane c, later clusters in LDS. A chunk step assigns a run of the 64 points at once, the walk takes
+// the rest one by one (both described where they run). Each point records its cluster and its rank in it.
 // Phase 2: the reference merges the first mergeable pair in (i, j) order and rescans from (0, 1) (:100-117,
 // canMergeWith :350-377, takeIndicesFrom :379-393). Pairs of untouched clusters that were found unmergeable stay so,
 // so after merging (ci, cj) the next first mergeable pair is either (a, ci) with a < ci or in row ci or later: the
