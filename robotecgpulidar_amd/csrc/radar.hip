@@ -339,29 +339,67 @@ __device__ void clusterOne(const RadarItem& it)
 			}
 			const uint64_t none = __builtin_amdgcn_ballot_w64(f0 < 0) & pending;
 			int stop = none != 0 ? __builtin_ctzll(none) : 64; // the first point the walk must take
+			// Each grown cluster's bounds with all its live members are kept in its lane (tb, tsize) in case every
+			// point commits; otherwise the commit below recomputes them up to `stop`. Ranks hold either way.
+			float tb[kBounds];
+			for (int q = 0; q < kBounds; ++q) {
+				tb[q] = rb[q];
+			}
+			int tsize = rsize, trank = 0;
 			for (uint64_t todo = pending & below(stop); todo != 0 && __builtin_ctzll(todo) < stop;) {
 				const int k = lanei(f0, __builtin_ctzll(todo));
 				const bool member = live && f0 == k;
-				todo &= ~__builtin_amdgcn_ballot_w64(member);
-				float x[kBounds], v[kBounds];
+				const uint64_t mk = __builtin_amdgcn_ballot_w64(member);
+				todo &= ~mk;
+				float x[kBounds], v[kBounds], nb[kBounds];
 				for (int q = 0; q < kBounds; ++q) {
 					x[q] = lanef(rb[q], k);
 				}
 				// A speed that turns NaN bounds finite ends the run after its point; no lane before it has one
 				// among its earlier members, so the checks below keep the NaN bounds.
+				const uint64_t finite = __builtin_amdgcn_ballot_w64(member && !isnan(pr));
 				if (__builtin_isunordered(x[kRMin], x[kRMax])) {
-					const uint64_t finite = __builtin_amdgcn_ballot_w64(member && !isnan(pr));
 					stop = finite != 0 ? min(stop, __builtin_ctzll(finite) + 1) : stop;
 				}
 				memberValues(member, v);
-				for (int q = 0; q < kBounds; q += 2) { // members before this lane: inclusive scan shifted by one
-					v[q] = dppOr<kWaveShr1>(INFINITY, scanWave<false>(v[q]));
-					v[q + 1] = dppOr<kWaveShr1>(-INFINITY, scanWave<true>(v[q + 1]));
+				for (int q = 0; q < kBounds; q += 2) { // inclusive scans over the wave
+					v[q] = scanWave<false>(v[q]);
+					v[q + 1] = scanWave<true>(v[q + 1]);
 				}
-				float nb[kBounds];
-				grow(x, v, false, nb);
-				const uint64_t taken = __builtin_amdgcn_ballot_w64(live & (f0 > k) & accepts(nb));
-				stop = taken != 0 ? min(stop, __builtin_ctzll(taken)) : stop;
+				float total[kBounds];
+				for (int q = 0; q < kBounds; ++q) {
+					total[q] = lanef(v[q], 63);
+				}
+				grow(x, total, finite != 0, nb);
+				const int size0 = lanei(rsize, k);
+				if (lane == k) {
+					for (int q = 0; q < kBounds; ++q) {
+						tb[q] = nb[q];
+					}
+					tsize = size0 + __builtin_popcountll(mk);
+				}
+				const int before = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mk >> 32),
+				                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u)));
+				trank = member ? size0 + before : trank;
+				// Would the cluster, grown by the members before a point whose f0 lies above it, take that point?
+				if (__builtin_amdgcn_ballot_w64(live & (f0 > k)) != 0) {
+					for (int q = 0; q < kBounds; q += 2) { // members before this lane
+						v[q] = dppOr<kWaveShr1>(INFINITY, v[q]);
+						v[q + 1] = dppOr<kWaveShr1>(-INFINITY, v[q + 1]);
+					}
+					grow(x, v, false, nb);
+					const uint64_t taken = __builtin_amdgcn_ballot_w64(live & (f0 > k) & accepts(nb));
+					stop = taken != 0 ? min(stop, __builtin_ctzll(taken)) : stop;
+				}
+			}
+			if (stop == 64) { // every point commits, with the bounds kept above
+				for (int q = 0; q < kBounds; ++q) {
+					rb[q] = tb[q];
+				}
+				rsize = tsize;
+				myCluster = live ? f0 : myCluster;
+				myRank = live ? trank : myRank;
+				return pending;
 			}
 			// Commit the points before `stop`: each grown cluster's bounds, size, and its members' ranks.
 			const uint64_t done = pending & below(stop);
