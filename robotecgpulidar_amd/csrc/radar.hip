@@ -281,7 +281,8 @@ __device__ void clusterOne(const RadarItem& it)
 		// point turned a grown cluster's radial-speed bounds finite. The step finds the first point where one of
 		// these happens, from an inclusive scan of each grown cluster's members over the wave, and commits every
 		// point before it. The walk then takes that point, and the step is tried again on the rest of the chunk
-		// while a try commits at least kMinRun points.
+		// while a try commits at least kMinRun points. The step runs while every cluster is held in registers (at
+		// most 64): over clusters held in LDS it is slower than the walk.
 		constexpr int kMinRun = 16;
 		auto below = [](int q) { return q >= 64 ? ~0ull : (1ull << q) - 1; };
 		// The members' values, or the identity of each minimum (+inf) and maximum (-inf) for other lanes. A member's
